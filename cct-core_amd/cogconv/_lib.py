@@ -1,0 +1,330 @@
+"""ctypes binding of libcogconv.so (include/cogconv.h).
+
+Loads the in-tree library and fails loudly if it has not been built: there is
+no fallback implementation behind this package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_ROOT, "libcogconv.so")
+
+# Status codes
+OK, ERR_INVALID_ARG, ERR_ILLEGAL_OP, ERR_RUNTIME, ERR_OOM, ERR_NOT_READY, ERR_UNSUPPORTED, ERR_BUDGET = range(8)
+# VectorMode
+(PLANE_BY_PLANE, PROJECT_FRAME, PROJECT_FRAME_BLOCK_REDUCE_SUM, BACK_PROJECT_FRAME,
+ BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM, FILTER_ADJOINT, FILTER_ADJOINT_BLOCK_REDUCE_SUM) = range(7)
+# BorderPolicy
+BORDER_ZERO, BORDER_CLAMP, BORDER_CYCLIC, BORDER_VALID, BORDER_FULL = range(5)
+# Sampling
+NO_SAMPLING, UPSAMPLE_INPUT, DOWNSAMPLE_OUTPUT = range(3)
+# Orientation
+CROSS_CORRELATION, CONVOLUTION = range(2)
+# Precision
+FP32, BF16, FP16, SPLIT_BF16 = range(4)
+# Algorithm hint
+ALGO_AUTO, ALGO_DIRECT, ALGO_SMALL_OUTPUT, ALGO_IMPLICIT_GEMM = range(4)
+
+
+class CogconvError(RuntimeError):
+    """A non-zero status from libcogconv; `status` holds the code."""
+
+    def __init__(self, status, message):
+        super().__init__("cogconv error %d: %s" % (status, message))
+        self.status = status
+
+
+class FieldTypeC(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int32), ("rows", ctypes.c_int32), ("cols", ctypes.c_int32),
+                ("planes", ctypes.c_int32), ("tensor_order", ctypes.c_int32)]
+
+    def as_tuple(self):
+        return (self.dims, self.rows, self.cols, self.planes, self.tensor_order)
+
+
+class DescC(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("border", ctypes.c_int32), ("orientation", ctypes.c_int32),
+                ("sampling", ctypes.c_int32), ("step", ctypes.c_int32), ("batch", ctypes.c_int32),
+                ("in0", FieldTypeC), ("in1", FieldTypeC), ("precision", ctypes.c_int32),
+                ("algo", ctypes.c_int32)]
+
+
+class DevicePropsC(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 128), ("arch", ctypes.c_char * 64),
+                ("compute_units", ctypes.c_int32), ("wavefront_size", ctypes.c_int32),
+                ("lds_bytes_per_block", ctypes.c_int32), ("max_threads_per_block", ctypes.c_int32),
+                ("clock_khz", ctypes.c_int32), ("memory_clock_khz", ctypes.c_int32),
+                ("memory_bus_width", ctypes.c_int32), ("l2_bytes", ctypes.c_int32),
+                ("hbm_bytes", ctypes.c_uint64)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_SZ = ctypes.c_size_t
+
+# name -> argtypes (restype int unless listed in _RESTYPES)
+SIGNATURES = {
+    "cogconv_last_error": [],
+    "cogconv_abi_version": [],
+    "cogconv_init": [ctypes.c_int32],
+    "cogconv_shutdown": [],
+    "cogconv_device_count": [ctypes.POINTER(ctypes.c_int32)],
+    "cogconv_device_props_get": [ctypes.c_int32, ctypes.POINTER(DevicePropsC)],
+    "cogconv_device_synchronize": [],
+    "cogconv_buf_alloc": [_SZ, ctypes.POINTER(_P)],
+    "cogconv_buf_free": [_P],
+    "cogconv_h2d": [_P, _P, _SZ],
+    "cogconv_d2h": [_P, _P, _SZ],
+    "cogconv_h2d_async": [_P, _P, _SZ, _P],
+    "cogconv_d2h_async": [_P, _P, _SZ, _P],
+    "cogconv_d2d_async": [_P, _P, _SZ, _P],
+    "cogconv_fill_zero": [_P, _SZ, _P],
+    "cogconv_host_alloc": [_SZ, ctypes.POINTER(_P)],
+    "cogconv_host_free": [_P],
+    "cogconv_stream_create": [ctypes.POINTER(_P)],
+    "cogconv_stream_destroy": [_P],
+    "cogconv_stream_synchronize": [_P],
+    "cogconv_event_create": [ctypes.POINTER(_P)],
+    "cogconv_event_destroy": [_P],
+    "cogconv_event_record": [_P, _P],
+    "cogconv_stream_wait_event": [_P, _P],
+    "cogconv_event_query": [_P],
+    "cogconv_event_synchronize": [_P],
+    "cogconv_event_elapsed_ms": [_P, _P, ctypes.POINTER(ctypes.c_float)],
+    "cogconv_output_type": [ctypes.POINTER(DescC), ctypes.POINTER(FieldTypeC)],
+    "cogconv_plan_create": [ctypes.POINTER(DescC), ctypes.POINTER(_P)],
+    "cogconv_plan_destroy": [_P],
+    "cogconv_plan_output_type": [_P, ctypes.POINTER(FieldTypeC)],
+    "cogconv_plan_workspace_bytes": [_P, ctypes.POINTER(_SZ)],
+    "cogconv_plan_flops": [_P, ctypes.POINTER(ctypes.c_double)],
+    "cogconv_plan_bytes": [_P, ctypes.POINTER(ctypes.c_double)],
+    "cogconv_plan_describe": [_P, ctypes.c_char_p, _SZ],
+    "cogconv_launch": [_P, _P, _P, _P, _P, _P],
+    "cogconv_block_reduce_sum": [_P, _P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _P],
+}
+_RESTYPES = {"cogconv_last_error": ctypes.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """The loaded libcogconv (raises if the library has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcogconv.so is not built (expected at %s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RESTYPES.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().cogconv_last_error().decode("utf-8", "replace")
+
+
+def check(status):
+    if status != OK:
+        raise CogconvError(status, last_error())
+    return status
+
+
+def device_count():
+    n = ctypes.c_int32(0)
+    check(lib().cogconv_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def device_props(device=0):
+    p = DevicePropsC()
+    check(lib().cogconv_device_props_get(device, ctypes.byref(p)))
+    return {"name": p.name.decode(), "arch": p.arch.decode(), "compute_units": p.compute_units,
+            "wavefront_size": p.wavefront_size, "lds_bytes_per_block": p.lds_bytes_per_block,
+            "max_threads_per_block": p.max_threads_per_block, "clock_khz": p.clock_khz,
+            "memory_clock_khz": p.memory_clock_khz, "memory_bus_width": p.memory_bus_width,
+            "l2_bytes": p.l2_bytes, "hbm_bytes": p.hbm_bytes}
+
+
+def init(device=0):
+    check(lib().cogconv_init(device))
+
+
+def synchronize():
+    check(lib().cogconv_device_synchronize())
+
+
+class DeviceBuffer:
+    """A device allocation owned by Python (freed on close / garbage collection)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().cogconv_buf_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        b.upload(arr)
+        return b
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(lib().cogconv_h2d(self.ptr, arr.ctypes.data, arr.nbytes))
+
+    def download(self, shape, dtype=np.float32):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().cogconv_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def zero(self, stream=None):
+        check(lib().cogconv_fill_zero(self.ptr, self.nbytes, stream))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().cogconv_buf_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().cogconv_stream_create(ctypes.byref(p)))
+        self.handle = p.value
+
+    def synchronize(self):
+        check(lib().cogconv_stream_synchronize(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().cogconv_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        check(lib().cogconv_event_create(ctypes.byref(p)))
+        self.handle = p.value
+
+    def record(self, stream=None):
+        check(lib().cogconv_event_record(self.handle, stream.handle if isinstance(stream, Stream) else stream))
+
+    def synchronize(self):
+        check(lib().cogconv_event_synchronize(self.handle))
+
+    def query(self):
+        s = lib().cogconv_event_query(self.handle)
+        if s == ERR_NOT_READY:
+            return False
+        check(s)
+        return True
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float()
+        check(lib().cogconv_event_elapsed_ms(self.handle, end.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().cogconv_event_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def field_type(dims, rows, cols, planes, order=None):
+    if order is None:
+        order = 0 if planes == 1 else 1
+    return FieldTypeC(dims, rows, cols, planes, order)
+
+
+def make_desc(mode, border, orientation, sampling, step, batch, in0, in1, precision=FP32, algo=ALGO_AUTO):
+    """in0 / in1: FieldTypeC or (dims, rows, cols, planes[, order]) tuples."""
+    def ft(x):
+        return x if isinstance(x, FieldTypeC) else field_type(*x)
+    return DescC(mode, border, orientation, sampling, step, batch, ft(in0), ft(in1), precision, algo)
+
+
+def output_type(desc):
+    out = FieldTypeC()
+    check(lib().cogconv_output_type(ctypes.byref(desc), ctypes.byref(out)))
+    return out.as_tuple()
+
+
+class Plan:
+    """One ConvolveOp execution plan (cogconv_plan_create / cogconv_launch)."""
+
+    def __init__(self, desc):
+        self.desc = desc
+        p = ctypes.c_void_p()
+        check(lib().cogconv_plan_create(ctypes.byref(desc), ctypes.byref(p)))
+        self.handle = p.value
+        out = FieldTypeC()
+        check(lib().cogconv_plan_output_type(self.handle, ctypes.byref(out)))
+        self.out_type = out.as_tuple()
+        ws = ctypes.c_size_t()
+        check(lib().cogconv_plan_workspace_bytes(self.handle, ctypes.byref(ws)))
+        self.workspace_bytes = ws.value
+        fl = ctypes.c_double()
+        check(lib().cogconv_plan_flops(self.handle, ctypes.byref(fl)))
+        self.flops = fl.value
+        by = ctypes.c_double()
+        check(lib().cogconv_plan_bytes(self.handle, ctypes.byref(by)))
+        self.bytes = by.value
+        buf = ctypes.create_string_buffer(256)
+        check(lib().cogconv_plan_describe(self.handle, buf, 256))
+        self.description = buf.value.decode()
+
+    @property
+    def out_shape(self):
+        _, rows, cols, planes, _ = self.out_type
+        return (planes, rows, cols)
+
+    def launch(self, in0, in1, out, workspace=None, stream=None):
+        def ptr(x):
+            return x.ptr if isinstance(x, DeviceBuffer) else x
+        s = stream.handle if isinstance(stream, Stream) else stream
+        check(lib().cogconv_launch(self.handle, ptr(in0), ptr(in1), ptr(out), ptr(workspace), s))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().cogconv_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def block_reduce_sum(inp, out, points, in_planes, factor, stream=None):
+    def ptr(x):
+        return x.ptr if isinstance(x, DeviceBuffer) else x
+    s = stream.handle if isinstance(stream, Stream) else stream
+    check(lib().cogconv_block_reduce_sum(ptr(inp), ptr(out), points, in_planes, factor, s))

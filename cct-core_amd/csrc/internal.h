@@ -1,0 +1,148 @@
+// Internal declarations of libcogconv (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "cogconv.h"
+
+namespace cogconv {
+
+// Record a message for cogconv_last_error() and return `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int check_hip(hipError_t e, const char* what);
+void clear_error();
+
+// Plane-level geometry of one ConvolveOp, in "materialised" coordinates:
+// operand 1 (in0) up-sampled by (upAr, upAc) is UR x UC, operand 2 (in1)
+// up-sampled by (upFr, upFc) is FHe x FWe.  Output element (R, C) sums
+//   tap(i, j) * A'(R*dnR - hR + i, C*dnC - hC + j)
+// over i < FHe, j < FWe (ConvolveHyperKernel.scala:404-548, 824-830).
+struct Geom {
+    int H, W;        // stored operand-1 plane
+    int FH, FW;      // stored operand-2 plane
+    int OH, OW;      // output plane
+    int upAr, upAc;  // zero insertion of operand 1 (non-FilterAdjoint upsampling)
+    int upFr, upFc;  // zero insertion of operand 2 (FilterAdjoint upsampling)
+    int dnR, dnC;    // output step (downsampling)
+    int UR, UC;      // H*upAr, W*upAc
+    int FHe, FWe;    // FH*upFr, FW*upFc
+    int hR, hC;      // low halo: Full F-1, Valid 0, otherwise (F-1)/2
+    int border;      // COGCONV_BORDER_*
+    int flip;        // 1 for the convolution orientation
+};
+
+// Which operand planes feed output plane o (ConvolveHyperKernel.scala:170-199, 319-340, 575-645).
+struct PlaneMap {
+    int mode;   // COGCONV_* vector mode
+    int B;      // batch
+    int N;      // operand-1 planes per image
+    int T;      // operand-2 planes
+    int PA;     // operand-1 planes
+    int PFn;    // operand-2 planes
+    int nOut;   // output planes
+    int R;      // plane pairs summed per output plane
+};
+
+__host__ __device__ inline void plane_pair(const PlaneMap& m, int o, int r, int& pa, int& pf) {
+    const int B = m.B, N = m.N, T = m.T;
+    switch (m.mode) {
+    case COGCONV_PLANE_BY_PLANE:
+        pa = m.PA == 1 ? 0 : o;
+        pf = m.PFn == 1 ? 0 : o;
+        break;
+    case COGCONV_PROJECT_FRAME: {  // o = b*T + w
+        const int b = o / T, w = o - b * T;
+        pa = b * N + w % N;
+        pf = w;
+        break;
+    }
+    case COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM: {  // o = b*K + k, term r = plane n
+        const int K = T / N, b = o / K, k = o - b * K;
+        pa = b * N + r;
+        pf = k * N + r;
+        break;
+    }
+    case COGCONV_BACK_PROJECT_FRAME: {  // o = b*T + w, L = T/N
+        const int L = T / N, b = o / T, w = o - b * T;
+        pa = b * N + w % N;
+        pf = L * (w % N) + w / N;
+        break;
+    }
+    case COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM: {  // o = b*L + l, term r = plane m
+        const int L = T / N, b = o / L, l = o - b * L;
+        pa = b * N + r;
+        pf = r * L + l;
+        break;
+    }
+    case COGCONV_FILTER_ADJOINT: {  // o = w*B + b (batch-interleaved output)
+        const int K = T / B, w = o / B, b = o - w * B;
+        pa = b * N + w % N;
+        pf = b * K + w / N;
+        break;
+    }
+    default: {  // COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM: o = w, term r = image b
+        const int K = T / B;
+        pa = r * N + o % N;
+        pf = r * K + o / N;
+        break;
+    }
+    }
+}
+
+// Kernel families.
+enum KernelKind {
+    KK_DIRECT = 1,
+    KK_SMALL_OUTPUT = 2,
+    KK_IGEMM = 3,
+};
+
+// LDS-tiled direct kernel configuration.
+struct DirectCfg {
+    int RG;        // row groups of 64 threads; a block is 64*RG threads, tile 4*RG x 64 outputs
+    int FCR, FCC;  // filter chunk staged per pass
+    int ldsCols;   // row pitch of the staged input tile
+    int ldsRows;
+    int tilesR, tilesC;
+    size_t lds_bytes;
+};
+
+// Split-reduction kernel for small outputs (FilterAdjoint to a filter-sized result).
+struct SmallCfg {
+    int splits;   // partial sums per output plane
+    int qchunks;  // ceil(OH*OW / 16)
+};
+
+// Implicit-GEMM configuration (MFMA).
+struct IgemmCfg {
+    int variant;  // which kernel instantiation
+    int Mtile, Ntile;
+    int splits;   // split-K count (FilterAdjoint)
+    int grid;
+};
+
+// Kernel launchers (device code in *.hip).
+hipError_t launch_direct(const Geom& g, const PlaneMap& m, const DirectCfg& c, const float* a, const float* f,
+                         float* out, hipStream_t s);
+hipError_t launch_small_output(const Geom& g, const PlaneMap& m, const SmallCfg& c, const float* a, const float* f,
+                               float* out, float* workspace, hipStream_t s);
+hipError_t launch_block_reduce_sum(const float* in, float* out, int64_t points, int out_planes, int factor,
+                                   hipStream_t s);
+hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
+
+}  // namespace cogconv
+
+struct cogconv_plan_s {
+    cogconv_desc desc;
+    cogconv_field_type out;
+    cogconv::Geom g;
+    cogconv::PlaneMap m;
+    int kind;
+    cogconv::DirectCfg dcfg;
+    cogconv::SmallCfg scfg;
+    cogconv::IgemmCfg icfg;
+    size_t workspace;
+    double flops;
+    double bytes;
+};

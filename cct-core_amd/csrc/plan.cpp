@@ -1,0 +1,362 @@
+// ConvolveOp planning: legality checks, result FieldType, geometry, kernel choice.
+//
+// Host-only code.  Mirrors, for the ConvolveOp path only:
+//   BinaryOperator.apply checks   src/main/scala/cogx/compiler/parser/syntaxtree/BinaryOperator.scala:65-93
+//   outputFieldType               src/main/scala/cogx/compiler/codegenerator/opencl/hyperkernels/ConvolveHyperKernel.scala:1297-1450
+//   convolveDirectly checks       src/main/scala/cogx/compiler/codegenerator/opencl/generator/DynamicConvolutionGenerator.scala:484-539
+//   Up/Down(1) -> NoSampling      src/main/scala/cogx/api/CogFunctionAPI.scala:1140-1144
+//   kernel routing                ConvolveHyperKernel.scala:1003-1148 (re-decided for gfx950, see choose_kernel)
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "internal.h"
+
+using namespace cogconv;
+
+namespace {
+
+bool is_fa(int mode) {
+    return mode == COGCONV_FILTER_ADJOINT || mode == COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM;
+}
+
+int border_extent(int border, int u, int f) {
+    if (border == COGCONV_BORDER_VALID) return u - f + 1;
+    if (border == COGCONV_BORDER_FULL) return u + f - 1;
+    return u;
+}
+
+const char* mode_name(int m) {
+    static const char* names[] = {"PlaneByPlane", "ProjectFrame", "ProjectFrameBlockReduceSum", "BackProjectFrame",
+                                  "BackProjectFrameBlockReduceSum", "FilterAdjoint", "FilterAdjointBlockReduceSum"};
+    return (m >= 0 && m < 7) ? names[m] : "?";
+}
+
+// Validates `d`, fills the result type and (optionally) the geometry and plane map.
+int analyse(const cogconv_desc* d, cogconv_field_type* out, Geom* gout, PlaneMap* mout) {
+    if (!d || !out) return fail(COGCONV_ERR_INVALID_ARG, "null descriptor or result pointer");
+    if (d->mode < 0 || d->mode > 6) return fail(COGCONV_ERR_INVALID_ARG, "bad vector mode %d", d->mode);
+    if (d->border < 0 || d->border > 4) return fail(COGCONV_ERR_INVALID_ARG, "bad border policy %d", d->border);
+    if (d->orientation < 0 || d->orientation > 1)
+        return fail(COGCONV_ERR_INVALID_ARG, "bad filter orientation %d", d->orientation);
+    if (d->sampling < 0 || d->sampling > 2) return fail(COGCONV_ERR_INVALID_ARG, "bad sampling policy %d", d->sampling);
+    if (d->step < 1) return fail(COGCONV_ERR_INVALID_ARG, "sampling step must be >= 1, got %d", d->step);
+    if (d->batch < 1) return fail(COGCONV_ERR_INVALID_ARG, "batchSize must be >= 1, got %d", d->batch);
+    if (d->precision < 0 || d->precision > 3) return fail(COGCONV_ERR_INVALID_ARG, "bad precision %d", d->precision);
+    const cogconv_field_type& a = d->in0;
+    const cogconv_field_type& f = d->in1;
+    if (a.rows < 1 || a.cols < 1 || a.planes < 1 || f.rows < 1 || f.cols < 1 || f.planes < 1)
+        return fail(COGCONV_ERR_INVALID_ARG, "field sizes must be positive");
+    if (a.tensor_order < 0 || a.tensor_order > 1 || f.tensor_order < 0 || f.tensor_order > 1)
+        return fail(COGCONV_ERR_ILLEGAL_OP, "Frame convolve operations not supported on MatrixFields");
+    // BinaryOperator.scala:65-93
+    if (a.dims < 1 || a.dims > 2)
+        return fail(COGCONV_ERR_ILLEGAL_OP, "convolution not supported on field of dimension %d", a.dims);
+    if (a.dims != f.dims)
+        return fail(COGCONV_ERR_ILLEGAL_OP, "1st field's dimension %d must equal 2nd field's dimension %d", a.dims,
+                    f.dims);
+    if (a.dims == 1 && (a.rows != 1 || f.rows != 1))
+        return fail(COGCONV_ERR_INVALID_ARG, "1-D fields must have rows == 1");
+    const int sampling = (d->step == 1) ? COGCONV_NO_SAMPLING : d->sampling;
+    const int step = d->step;
+    if (a.planes % d->batch)
+        return fail(COGCONV_ERR_ILLEGAL_OP, "Image depth %d must be a multiple of the batchSize %d", a.planes,
+                    d->batch);
+    const int ppi = a.planes / d->batch;
+    const int T = f.planes;
+    switch (d->mode) {
+    case COGCONV_PROJECT_FRAME:
+    case COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM:
+    case COGCONV_BACK_PROJECT_FRAME:
+    case COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM:
+        if (T % ppi)
+            return fail(COGCONV_ERR_ILLEGAL_OP, "Filter depth %d must be a multiple of the single image depth %d", T,
+                        ppi);
+        break;
+    case COGCONV_FILTER_ADJOINT:
+    case COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM:
+        if (T % d->batch)
+            return fail(COGCONV_ERR_ILLEGAL_OP, "Filter tensorshape points %d must be a multiple of the batchSize %d",
+                        T, d->batch);
+        break;
+    default:  // PlaneByPlane: ConvolveHyperKernel.scala:1378-1386
+        if (!(a.tensor_order == 0 || f.tensor_order == 0 || a.planes == f.planes))
+            return fail(COGCONV_ERR_ILLEGAL_OP, "PlaneByPlane needs a scalar operand or equal tensor shapes (%d vs %d)",
+                        a.planes, f.planes);
+        break;
+    }
+    // DynamicConvolutionGenerator.scala:500-506
+    if (d->border != COGCONV_BORDER_VALID && d->border != COGCONV_BORDER_FULL) {
+        if ((f.cols % 2) == 0 || (a.dims == 2 && (f.rows % 2) == 0))
+            return fail(COGCONV_ERR_ILLEGAL_OP, "filter must be of odd shape for border policy %d (got %dx%d)",
+                        d->border, f.rows, f.cols);
+    }
+    // L512-526
+    if (sampling == COGCONV_DOWNSAMPLE_OUTPUT) {
+        int cr = 0, cc = 0;
+        if (d->border == COGCONV_BORDER_VALID) { cr = f.rows - 1; cc = f.cols - 1; }
+        if (d->border == COGCONV_BORDER_FULL) { cr = 1 - f.rows; cc = 1 - f.cols; }
+        if ((a.cols - cc) % step || (a.dims == 2 && (a.rows - cr) % step))
+            return fail(COGCONV_ERR_ILLEGAL_OP,
+                        "The size of the input for convolution must be divisible by the downsampling step size "
+                        "after border contraction.");
+    }
+    // L531-536
+    if (sampling == COGCONV_UPSAMPLE_INPUT && d->border == COGCONV_BORDER_CLAMP)
+        return fail(COGCONV_ERR_ILLEGAL_OP, "Integrated upsampling with BorderClamp convolution not supported");
+    // FilterAdjoint up-sampling applies to the 2nd operand and is only defined
+    // for BorderValid (ConvolveHyperKernel.scala:1090-1098 routes it away from
+    // the standard kernel); other borders have no consistent definition.
+    if (sampling == COGCONV_UPSAMPLE_INPUT && is_fa(d->mode) && d->border != COGCONV_BORDER_VALID)
+        return fail(COGCONV_ERR_ILLEGAL_OP, "FilterAdjoint with UpsampleInputConvolution requires BorderValid");
+
+    Geom g;
+    std::memset(&g, 0, sizeof g);
+    g.H = a.rows; g.W = a.cols; g.FH = f.rows; g.FW = f.cols;
+    g.upAr = g.upAc = g.upFr = g.upFc = g.dnR = g.dnC = 1;
+    const bool two = a.dims == 2;
+    if (sampling == COGCONV_UPSAMPLE_INPUT) {
+        if (is_fa(d->mode)) { g.upFc = step; g.upFr = two ? step : 1; }
+        else { g.upAc = step; g.upAr = two ? step : 1; }
+    } else if (sampling == COGCONV_DOWNSAMPLE_OUTPUT) {
+        g.dnC = step; g.dnR = two ? step : 1;
+    }
+    g.UR = g.H * g.upAr; g.UC = g.W * g.upAc;
+    g.FHe = g.FH * g.upFr; g.FWe = g.FW * g.upFc;
+    int br = border_extent(d->border, g.UR, g.FHe);
+    int bc = border_extent(d->border, g.UC, g.FWe);
+    if (!two) br = 1;
+    if (br < 1 || bc < 1)
+        return fail(COGCONV_ERR_ILLEGAL_OP, "filter (%dx%d) larger than input (%dx%d) for BorderValid", g.FHe, g.FWe,
+                    g.UR, g.UC);
+    g.OH = (br + g.dnR - 1) / g.dnR;
+    g.OW = (bc + g.dnC - 1) / g.dnC;
+    if (d->border == COGCONV_BORDER_FULL) { g.hR = g.FHe - 1; g.hC = g.FWe - 1; }
+    else if (d->border == COGCONV_BORDER_VALID) { g.hR = 0; g.hC = 0; }
+    else { g.hR = (g.FHe - 1) / 2; g.hC = (g.FWe - 1) / 2; }
+    if (!two) g.hR = 0;
+    g.border = d->border;
+    g.flip = d->orientation == COGCONV_CONVOLUTION;
+
+    int planes = 0, order = 1;
+    switch (d->mode) {
+    case COGCONV_PROJECT_FRAME:
+    case COGCONV_BACK_PROJECT_FRAME:
+        planes = T * d->batch;
+        order = 1;
+        break;
+    case COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM:
+    case COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM:
+        planes = d->batch * (T / ppi);
+        order = (planes == 1 && a.planes <= 4) ? 0 : 1;  // ConvolveHyperKernel.scala:1393-1410
+        break;
+    case COGCONV_FILTER_ADJOINT:
+        planes = ppi * (T / d->batch) * d->batch;
+        order = planes > 1;
+        break;
+    case COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM:
+        planes = ppi * (T / d->batch);
+        order = planes > 1;
+        break;
+    default:
+        if (a.tensor_order == 0) { planes = f.planes; order = f.tensor_order; }
+        else { planes = a.planes; order = a.tensor_order; }
+        break;
+    }
+    out->dims = a.dims;
+    out->rows = g.OH;
+    out->cols = g.OW;
+    out->planes = planes;
+    out->tensor_order = order;
+
+    if (gout) *gout = g;
+    if (mout) {
+        PlaneMap m;
+        m.mode = d->mode;
+        m.B = d->batch;
+        m.N = ppi;
+        m.T = T;
+        m.PA = a.planes;
+        m.PFn = f.planes;
+        m.nOut = planes;
+        switch (d->mode) {
+        case COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM:
+        case COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM: m.R = ppi; break;
+        case COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM: m.R = d->batch; break;
+        default: m.R = 1; break;
+        }
+        *mout = m;
+    }
+    return COGCONV_OK;
+}
+
+DirectCfg choose_direct(const Geom& g, int nOut) {
+    DirectCfg c;
+    c.RG = 4;
+    c.FCR = std::min(g.FHe, 16);
+    c.FCC = std::min(g.FWe, 32);
+    const size_t budget = 48 * 1024;
+    for (;;) {
+        const int rows = (4 * c.RG - 1) * g.dnR + c.FCR;
+        const int cols = 63 * g.dnC + c.FCC;
+        const size_t bytes = (size_t)(rows * cols + c.FCR * c.FCC) * sizeof(float);
+        if (bytes <= budget || (c.RG == 1 && c.FCR == 1 && c.FCC == 1)) {
+            c.ldsRows = rows;
+            c.ldsCols = cols;
+            c.lds_bytes = bytes;
+            break;
+        }
+        if (c.RG > 1) c.RG /= 2;
+        else if (c.FCR > 1) c.FCR = (c.FCR + 1) / 2;
+        else c.FCC = (c.FCC + 1) / 2;
+    }
+    c.tilesR = (g.OH + 4 * c.RG - 1) / (4 * c.RG);
+    c.tilesC = (g.OW + 63) / 64;
+    (void)nOut;
+    return c;
+}
+
+SmallCfg choose_small(const Geom& g, const PlaneMap& m) {
+    SmallCfg c;
+    c.qchunks = (g.OH * g.OW + 15) / 16;
+    const long rowsTotal = (long)m.R * g.FHe;
+    // Aim for a few thousand blocks so that all 256 CUs are busy, with at least
+    // ~8 filter rows of reduction per block.
+    long want = (4096 + (long)m.nOut * c.qchunks - 1) / ((long)m.nOut * c.qchunks);
+    long maxs = std::max(1L, rowsTotal / 8);
+    c.splits = (int)std::max(1L, std::min(want, maxs));
+    return c;
+}
+
+}  // namespace
+
+extern "C" int cogconv_output_type(const cogconv_desc* desc, cogconv_field_type* out) {
+    clear_error();
+    return analyse(desc, out, nullptr, nullptr);
+}
+
+extern "C" int cogconv_plan_create(const cogconv_desc* desc, cogconv_plan* plan) {
+    clear_error();
+    if (!plan) return fail(COGCONV_ERR_INVALID_ARG, "null plan pointer");
+    *plan = nullptr;
+    cogconv_field_type out;
+    Geom g;
+    PlaneMap m;
+    int e = analyse(desc, &out, &g, &m);
+    if (e) return e;
+    if (desc->precision != COGCONV_FP32)
+        return fail(COGCONV_ERR_UNSUPPORTED, "precision %d is not offered for %s in this build", desc->precision,
+                    mode_name(desc->mode));
+    cogconv_plan p = new (std::nothrow) cogconv_plan_s;
+    if (!p) return fail(COGCONV_ERR_OOM, "host allocation failed");
+    std::memset(p, 0, sizeof *p);
+    p->desc = *desc;
+    p->out = out;
+    p->g = g;
+    p->m = m;
+
+    const long outPts = (long)g.OH * g.OW;
+    const bool small_out = g.border == COGCONV_BORDER_VALID && is_fa(desc->mode) && outPts <= 256 &&
+                           (long)g.FHe * g.FWe >= 64;
+    int algo = desc->algo;
+    if (algo == COGCONV_ALGO_IMPLICIT_GEMM) algo = COGCONV_ALGO_AUTO;  // FP32 plans use the VALU kernels
+    if (algo == COGCONV_ALGO_SMALL_OUTPUT && outPts > 4096) {
+        delete p;
+        return fail(COGCONV_ERR_UNSUPPORTED, "small-output kernel needs an output plane of at most 4096 points");
+    }
+    if (algo == COGCONV_ALGO_SMALL_OUTPUT || (algo == COGCONV_ALGO_AUTO && small_out)) {
+        p->kind = KK_SMALL_OUTPUT;
+        p->scfg = choose_small(g, m);
+        p->workspace = (size_t)p->scfg.splits * m.nOut * outPts * sizeof(float);
+    } else {
+        p->kind = KK_DIRECT;
+        p->dcfg = choose_direct(g, m.nOut);
+        p->workspace = 0;
+    }
+    // Algorithmic work: every multiply-add whose tap multiplies a stored
+    // (non-inserted) sample.  Up-sampled operand 1 leaves FH*FW/(upAr*upAc)
+    // such taps per output on average; FilterAdjoint up-sampling of operand 2
+    // leaves its FH*FW stored taps.
+    double taps = (double)g.FH * g.FW / ((double)g.upAr * g.upAc);
+    p->flops = 2.0 * m.nOut * (double)m.R * (double)outPts * taps;
+    p->bytes = 4.0 * ((double)desc->in0.planes * desc->in0.rows * desc->in0.cols +
+                      (double)desc->in1.planes * desc->in1.rows * desc->in1.cols + (double)m.nOut * outPts);
+    *plan = p;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_plan_destroy(cogconv_plan plan) {
+    clear_error();
+    delete plan;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_plan_output_type(cogconv_plan plan, cogconv_field_type* out) {
+    clear_error();
+    if (!plan || !out) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    *out = plan->out;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_plan_workspace_bytes(cogconv_plan plan, size_t* bytes) {
+    clear_error();
+    if (!plan || !bytes) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    *bytes = plan->workspace;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_plan_flops(cogconv_plan plan, double* flops) {
+    clear_error();
+    if (!plan || !flops) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    *flops = plan->flops;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_plan_bytes(cogconv_plan plan, double* bytes) {
+    clear_error();
+    if (!plan || !bytes) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    *bytes = plan->bytes;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len) {
+    clear_error();
+    if (!plan || !buf || len == 0) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    const Geom& g = plan->g;
+    switch (plan->kind) {
+    case KK_DIRECT:
+        snprintf(buf, len, "direct %s out=%dx%dx%d tile=%dx64 fchunk=%dx%d lds=%zu", mode_name(plan->desc.mode),
+                 plan->m.nOut, g.OH, g.OW, 4 * plan->dcfg.RG, plan->dcfg.FCR, plan->dcfg.FCC, plan->dcfg.lds_bytes);
+        break;
+    case KK_SMALL_OUTPUT:
+        snprintf(buf, len, "small_output %s out=%dx%dx%d splits=%d", mode_name(plan->desc.mode), plan->m.nOut, g.OH,
+                 g.OW, plan->scfg.splits);
+        break;
+    default:
+        snprintf(buf, len, "igemm %s variant=%d", mode_name(plan->desc.mode), plan->icfg.variant);
+        break;
+    }
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_launch(cogconv_plan plan, const float* in0, const float* in1, float* out, void* workspace,
+                              cogconv_stream stream) {
+    clear_error();
+    if (!plan || !in0 || !in1 || !out) return fail(COGCONV_ERR_INVALID_ARG, "null plan or buffer");
+    if (plan->workspace && !workspace)
+        return fail(COGCONV_ERR_INVALID_ARG, "plan needs %zu bytes of workspace", plan->workspace);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipError_t e = hipSuccess;
+    switch (plan->kind) {
+    case KK_DIRECT:
+        e = launch_direct(plan->g, plan->m, plan->dcfg, in0, in1, out, s);
+        break;
+    case KK_SMALL_OUTPUT:
+        e = launch_small_output(plan->g, plan->m, plan->scfg, in0, in1, out, static_cast<float*>(workspace), s);
+        break;
+    default:
+        return fail(COGCONV_ERR_RUNTIME, "internal: unknown kernel kind %d", plan->kind);
+    }
+    return check_hip(e, "convolution launch");
+}

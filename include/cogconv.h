@@ -1,0 +1,231 @@
+/*
+ * cogconv.h -- C ABI of the MI355X-native Field-convolution path (libcogconv.so).
+ *
+ * Drop-in boundary for cct-core's ConvolveOp path.  The reference reaches the
+ * device through JOCL (OpenCL over JNI); this library replaces both
+ *   (a) the OpenCL platform layer (device / buffer / command-queue / event
+ *       management, src/main/scala/cogx/platform/opencl/), and
+ *   (b) the convolution HyperKernels (ConvolveHyperKernel,
+ *       ConvolveTiledHyperKernel, ConvolveToSmallField*HyperKernel,
+ *       TensorReduceHyperKernel in
+ *       src/main/scala/cogx/compiler/codegenerator/opencl/hyperkernels/).
+ *
+ * Conventions (follow src/main/java/cogjni/CogJNIWrapper.java:29-33,72-85):
+ *   - every function returns int status, 0 = COGCONV_OK;
+ *   - on failure cogconv_last_error() returns a thread-local message; no C++
+ *     exception or abort ever crosses this ABI;
+ *   - handles are opaque pointers (a Java caller holds them as `long`);
+ *   - the caller owns buffers, streams, events and plans;
+ *   - launches are asynchronous (they return after enqueue).
+ *
+ * Data layout of every field buffer: planar float32,
+ *   addr(plane, row, col) = plane*rows*cols + row*cols + col
+ * (src/main/scala/cogx/platform/types/FieldMemoryLayoutImpl.scala:250-311,
+ *  src/main/scala/cogx/compiler/codegenerator/opencl/fragments/FieldIO.scala:308-343).
+ * A batch of B images of N planes stores image b's planes at b*N .. b*N+N-1.
+ * A 1-D field of length L is rows = 1, cols = L, dims = 1.
+ */
+#ifndef COGCONV_H
+#define COGCONV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COGCONV_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+    COGCONV_OK = 0,
+    COGCONV_ERR_INVALID_ARG = 1, /* null pointer, bad enum, bad size            */
+    COGCONV_ERR_ILLEGAL_OP = 2,  /* the reference rejects this ConvolveOp       */
+    COGCONV_ERR_RUNTIME = 3,     /* HIP runtime error                           */
+    COGCONV_ERR_OOM = 4,         /* device allocation failed                    */
+    COGCONV_ERR_NOT_READY = 5,   /* event / request not complete yet            */
+    COGCONV_ERR_UNSUPPORTED = 6, /* legal op, but not offered in this precision */
+    COGCONV_ERR_BUDGET = 7       /* serving admission budget exceeded           */
+};
+
+/* VectorMode: src/main/scala/cogx/compiler/parser/op/BinaryOpcode.scala:39-48 */
+enum {
+    COGCONV_PLANE_BY_PLANE = 0,
+    COGCONV_PROJECT_FRAME = 1,
+    COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM = 2,
+    COGCONV_BACK_PROJECT_FRAME = 3,
+    COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM = 4,
+    COGCONV_FILTER_ADJOINT = 5,
+    COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM = 6
+};
+
+/* BorderPolicy: src/main/scala/cogx/platform/types/BorderPolicy.scala:30-134 */
+enum {
+    COGCONV_BORDER_ZERO = 0,
+    COGCONV_BORDER_CLAMP = 1,
+    COGCONV_BORDER_CYCLIC = 2,
+    COGCONV_BORDER_VALID = 3,
+    COGCONV_BORDER_FULL = 4
+};
+
+/* ConvolutionSamplingPolicy: src/main/scala/cogx/platform/types/ConvolutionSamplingPolicy.scala:27-31 */
+enum {
+    COGCONV_NO_SAMPLING = 0,
+    COGCONV_UPSAMPLE_INPUT = 1,   /* UpsampleInputConvolution(step)   */
+    COGCONV_DOWNSAMPLE_OUTPUT = 2 /* DownsampleOutputConvolution(step) */
+};
+
+/* FilterOrientation: src/main/scala/cogx/platform/types/FilterOrientation.scala:28-30 */
+enum {
+    COGCONV_CROSS_CORRELATION = 0,
+    COGCONV_CONVOLUTION = 1
+};
+
+/* Arithmetic of the plan.  FP32 is the plain form (FP32 products, FP32
+ * accumulation).  The others are the precision variants of the
+ * contraction modes (ProjectFrame / BackProjectFrame / FilterAdjoint with
+ * block reduction): MFMA implicit GEMM with FP32 accumulation. */
+enum {
+    COGCONV_FP32 = 0,
+    COGCONV_BF16 = 1,
+    COGCONV_FP16 = 2,
+    COGCONV_SPLIT_BF16 = 3 /* three-term x_hi*w_hi + x_hi*w_lo + x_lo*w_hi */
+};
+
+/* Kernel-family hint; COGCONV_ALGO_AUTO lets the plan (and its autotuner) pick. */
+enum {
+    COGCONV_ALGO_AUTO = 0,
+    COGCONV_ALGO_DIRECT = 1,        /* LDS-tiled direct kernel, any op         */
+    COGCONV_ALGO_SMALL_OUTPUT = 2,  /* split-reduction kernel, small outputs   */
+    COGCONV_ALGO_IMPLICIT_GEMM = 3  /* MFMA implicit GEMM, contraction modes   */
+};
+
+/* FieldType of one operand / result (src/main/scala/cogx/platform/types/FieldType.scala).
+ * tensor_order is 0 for a ScalarField, 1 for a VectorField (planes = vector length). */
+typedef struct {
+    int32_t dims;   /* 1 or 2 */
+    int32_t rows;   /* 1 for dims == 1 */
+    int32_t cols;
+    int32_t planes; /* tensor points */
+    int32_t tensor_order;
+} cogconv_field_type;
+
+/* ConvolveOp(border, orientation, sampling, vectorMode, batchSize) plus the two
+ * input FieldTypes: BinaryOpcode.scala:64-71, BinaryOperator.scala:59-93. */
+typedef struct {
+    int32_t mode;        /* COGCONV_PLANE_BY_PLANE .. */
+    int32_t border;      /* COGCONV_BORDER_* */
+    int32_t orientation; /* COGCONV_CROSS_CORRELATION / COGCONV_CONVOLUTION */
+    int32_t sampling;    /* COGCONV_NO_SAMPLING / _UPSAMPLE_INPUT / _DOWNSAMPLE_OUTPUT */
+    int32_t step;        /* sampling step; 1 means no sampling */
+    int32_t batch;       /* batchSize */
+    cogconv_field_type in0; /* image (or dy for BackProjectFrame, x for FilterAdjoint) */
+    cogconv_field_type in1; /* filter bank (or dy for FilterAdjoint) */
+    int32_t precision;   /* COGCONV_FP32 .. */
+    int32_t algo;        /* COGCONV_ALGO_* */
+} cogconv_desc;
+
+typedef struct cogconv_plan_s* cogconv_plan;
+typedef struct cogconv_stream_s* cogconv_stream; /* NULL = the device's default stream */
+typedef struct cogconv_event_s* cogconv_event;
+
+typedef struct {
+    char name[128];
+    char arch[64];           /* e.g. "gfx950" */
+    int32_t compute_units;
+    int32_t wavefront_size;
+    int32_t lds_bytes_per_block;
+    int32_t max_threads_per_block;
+    int32_t clock_khz;
+    int32_t memory_clock_khz;
+    int32_t memory_bus_width;
+    int32_t l2_bytes;
+    uint64_t hbm_bytes;
+} cogconv_device_props;
+
+/* ---- errors and library state (replaces OpenCLPlatform, $OCL/OpenCLPlatform.scala:38-244) ---- */
+
+/* Thread-local message of the last failure on this thread ("" if none). */
+const char* cogconv_last_error(void);
+int cogconv_abi_version(void);
+/* Select `device` as current for this thread; idempotent. */
+int cogconv_init(int32_t device);
+int cogconv_shutdown(void);
+int cogconv_device_count(int32_t* count);
+/* OpenCLDevice properties ($OCL/OpenCLDevice.scala:39-402). */
+int cogconv_device_props_get(int32_t device, cogconv_device_props* props);
+int cogconv_device_synchronize(void);
+
+/* ---- buffers (replaces OpenCLBuffer, $OCL/OpenCLBuffer.scala:53-523) ---- */
+
+int cogconv_buf_alloc(size_t bytes, void** dptr);
+int cogconv_buf_free(void* dptr);
+/* Blocking copies (OpenCLBuffer.read/write, L207-231). */
+int cogconv_h2d(void* dst, const void* src, size_t bytes);
+int cogconv_d2h(void* dst, const void* src, size_t bytes);
+/* Asynchronous copies on a stream; host memory should come from cogconv_host_alloc. */
+int cogconv_h2d_async(void* dst, const void* src, size_t bytes, cogconv_stream stream);
+int cogconv_d2h_async(void* dst, const void* src, size_t bytes, cogconv_stream stream);
+int cogconv_d2d_async(void* dst, const void* src, size_t bytes, cogconv_stream stream);
+/* Zero-fill by a kernel (safe inside captured graphs). */
+int cogconv_fill_zero(void* dptr, size_t bytes, cogconv_stream stream);
+/* Pinned host memory (the reference's Cog.pinnedBuffers option). */
+int cogconv_host_alloc(size_t bytes, void** hptr);
+int cogconv_host_free(void* hptr);
+
+/* ---- streams and events (replaces OpenCLParallelCommandQueue, $OCL/OpenCLParallelCommandQueue.scala:45-283) ---- */
+
+int cogconv_stream_create(cogconv_stream* stream);
+int cogconv_stream_destroy(cogconv_stream stream);
+int cogconv_stream_synchronize(cogconv_stream stream);
+int cogconv_event_create(cogconv_event* event);
+int cogconv_event_destroy(cogconv_event event);
+int cogconv_event_record(cogconv_event event, cogconv_stream stream);
+/* Make `stream` wait for `event` (the wait list of a launch, OpenCLDeviceKernel.scala:67-100). */
+int cogconv_stream_wait_event(cogconv_stream stream, cogconv_event event);
+/* COGCONV_OK when complete, COGCONV_ERR_NOT_READY when pending (phase2Clock's status check). */
+int cogconv_event_query(cogconv_event event);
+int cogconv_event_synchronize(cogconv_event event);
+int cogconv_event_elapsed_ms(cogconv_event start, cogconv_event end, float* ms);
+
+/* ---- the convolution (replaces ConvolveHyperKernel & co.) ---- */
+
+/* Result FieldType of a ConvolveOp, with every legality check the reference
+ * applies when the graph is built: ConvolveHyperKernel.outputFieldType
+ * ($HK/ConvolveHyperKernel.scala:1297-1450), BinaryOperator.scala:65-93 and
+ * DynamicConvolutionGenerator.convolveDirectly (L484-539).  Pure host code:
+ * needs no device.  Illegal ops return COGCONV_ERR_ILLEGAL_OP. */
+int cogconv_output_type(const cogconv_desc* desc, cogconv_field_type* out);
+
+/* Build an execution plan (kernel choice, tile shape, workspace size).  Replaces
+ * ConvolveHyperKernel.apply ($HK/ConvolveHyperKernel.scala:1003-1148).  Needs no
+ * device either: kernels are loaded at the first launch. */
+int cogconv_plan_create(const cogconv_desc* desc, cogconv_plan* plan);
+int cogconv_plan_destroy(cogconv_plan plan);
+int cogconv_plan_output_type(cogconv_plan plan, cogconv_field_type* out);
+/* Device scratch a launch needs (0 for most plans); the caller allocates it. */
+int cogconv_plan_workspace_bytes(cogconv_plan plan, size_t* bytes);
+/* Algorithmic FLOPs of one launch: 2 * (multiply-adds with a non-zero tap). */
+int cogconv_plan_flops(cogconv_plan plan, double* flops);
+/* Algorithmic HBM bytes of one launch: each operand read once, output written once. */
+int cogconv_plan_bytes(cogconv_plan plan, double* bytes);
+/* Human-readable kernel choice, e.g. "direct tile=16x64 fchunk=16x16". */
+int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len);
+/* Enqueue one ConvolveOp: out = op(in0, in1).  in0/in1/out are planar float32
+ * device buffers of the plan's FieldTypes; `workspace` holds
+ * cogconv_plan_workspace_bytes bytes (may be NULL when that is 0). */
+int cogconv_launch(cogconv_plan plan, const float* in0, const float* in1, float* out, void* workspace,
+                   cogconv_stream stream);
+
+/* blockReduceSum(field, factor) ($M/api/CogFunctionAPI.scala:755-756,
+ * $HK/TensorReduceHyperKernel.scala:39-207): out plane p = sum_{j<factor} in plane p*factor+j.
+ * `points` = rows*cols of one plane. */
+int cogconv_block_reduce_sum(const float* in, float* out, int64_t points, int32_t in_planes, int32_t factor,
+                             cogconv_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COGCONV_H */
