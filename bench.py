@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--serial", action="store_true", help="run the three ops of a step back to back on one stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -100,7 +101,7 @@ def main():
     import cogconv._lib as L
     L.init(0)
     props = L.device_props(0)
-    log("device:", props["name"], props["arch"], "CUs", props["compute_units"])
+    log("device:", props)
 
     x, w, dy = make_inputs()
     bx, bw, bdy = (L.DeviceBuffer.from_array(a) for a in (x, w, dy))
@@ -112,40 +113,64 @@ def main():
         ins = {"projectFrame+BRS": (bx, bw), "backProjectFrame+BRS": (bdy, bw), "filterAdjoint+BRS": (bx, bdy)}[name]
         ops.append({"name": name, "plan": plan, "in": ins, "out": out, "ws": ws})
         log("plan:", name, plan.description, "GFLOP %.4f" % (plan.flops / 1e9))
-    stream = L.Stream()
     flops_step = sum(op["plan"].flops for op in ops)
+    streams = [L.Stream() for _ in ops]
+    main_stream = streams[0]
 
-    def run_step(evs=None):
-        for i, op in enumerate(ops):
-            if evs is not None:
-                evs[i][0].record(stream)
-            op["plan"].launch(op["in"][0], op["in"][1], op["out"], op["ws"], stream)
-            if evs is not None:
-                evs[i][1].record(stream)
+    def launch(i, stream):
+        op = ops[i]
+        op["plan"].launch(op["in"][0], op["in"][1], op["out"], op["ws"], stream)
 
-    for _ in range(args.warmup):
-        run_step()
-    stream.synchronize()
+    # Phase 1: each op alone on one stream, timed with HIP events on that
+    # stream (average launch duration; this feeds the roofline).
+    iso_reps = max(20, args.steps // 4)
+    per_op_ms = []
+    for i in range(len(ops)):
+        for _ in range(5):
+            launch(i, main_stream)
+        e0, e1 = L.Event(), L.Event()
+        e0.record(main_stream)
+        for _ in range(iso_reps):
+            launch(i, main_stream)
+        e1.record(main_stream)
+        e1.synchronize()
+        per_op_ms.append(e0.elapsed_ms(e1) / iso_reps)
 
-    # Timed region: K steps, per-op HIP events on the launch stream.
-    K = args.steps
-    per_op_ms = [0.0] * len(ops)
-    evs = [[(L.Event(), L.Event()) for _ in ops] for _ in range(K)]
-    t_start, t_end = L.Event(), L.Event()
+    # Phase 2: the timed steps.  The three ops of a step are independent
+    # (PF reads x, W; BPF reads dy, W; FA reads x, dy), so a step forks them
+    # onto three streams and joins; --serial runs them back to back on one.
+    fork = [L.Event() for _ in range(args.steps + args.warmup)]
+    done = [[L.Event() for _ in ops] for _ in range(args.steps + args.warmup)]
+
+    def run_step(k):
+        if args.serial:
+            for i in range(len(ops)):
+                launch(i, main_stream)
+            return
+        fork[k].record(main_stream)
+        for i, st in enumerate(streams):
+            if i:
+                L.check(L.lib().cogconv_stream_wait_event(st.handle, fork[k].handle))
+            launch(i, st)
+            if i:
+                done[k][i].record(st)
+        for i in range(1, len(ops)):
+            L.check(L.lib().cogconv_stream_wait_event(main_stream.handle, done[k][i].handle))
+
+    for k in range(args.warmup):
+        run_step(k)
     L.synchronize()
+    K = args.steps
+    t_start, t_end = L.Event(), L.Event()
     wall0 = time.perf_counter()
-    t_start.record(stream)
+    t_start.record(main_stream)
     for k in range(K):
-        run_step(evs[k])
-    t_end.record(stream)
-    stream.synchronize()
+        run_step(args.warmup + k)
+    t_end.record(main_stream)
+    t_end.synchronize()
     L.synchronize()
     wall = time.perf_counter() - wall0
     total_ms = t_start.elapsed_ms(t_end)
-    for k in range(K):
-        for i in range(len(ops)):
-            per_op_ms[i] += evs[k][i][0].elapsed_ms(evs[k][i][1])
-    per_op_ms = [t / K for t in per_op_ms]
     ms_per_step = max(total_ms, wall * 1e3) / K
     value = flops_step / (ms_per_step * 1e-3) / 1e9
 
@@ -165,6 +190,7 @@ def main():
     roof["algorithmic_flops"] = dplan.flops
     roof["algorithmic_bytes"] = dplan.bytes
     roof["avg_launch_ms"] = round(per_op_ms[dom], 5)
+    roof["timing"] = "HIP events around %d back-to-back launches of this op alone on its stream" % iso_reps
 
     cpu = None if args.no_cpu else cpu_baseline(budget_s=args.cpu_budget)
     line = {
@@ -182,9 +208,10 @@ def main():
         "data": "synthetic (seeded uniform[-1,1) x and dy, He-normal filters)",
         "config": {"workload": "FP32 CNN-layer triple: PF+BRS Valid, BPF+BRS Full, FA+BRS Valid; batch 16 "
                                "images 64x64x16, 32 filters 3x3x16", "batch": 16, "image": [64, 64, 16],
-                   "filters": [32, 3, 3, 16], "flops_per_step": flops_step},
-        "per_op": {op["name"]: {"ms": round(t, 5), "gflops": round(op["plan"].flops / (t * 1e-3) / 1e9, 1),
-                                "kernel": op["plan"].description} for op, t in zip(ops, per_op_ms)},
+                   "filters": [32, 3, 3, 16], "flops_per_step": flops_step,
+                   "step": "serial on one stream" if args.serial else "three independent ops on three streams"},
+        "per_op_isolated": {op["name"]: {"ms": round(t, 5), "gflops": round(op["plan"].flops / (t * 1e-3) / 1e9, 1),
+                                         "kernel": op["plan"].description} for op, t in zip(ops, per_op_ms)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "device": props["name"] + " " + props["arch"],

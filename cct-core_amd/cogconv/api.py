@@ -240,21 +240,25 @@ class ComputeGraph:
         self._stream = L.Stream()
         self._buffers = {}
         self._steps = []
-        skip = set()
         order = list(self._fields)  # creation order is a topological order
+        fused = {}  # id(blockReduceSum field) -> fused vector mode
+        skip = set()  # convolutions absorbed into a fused reduce
+        if self.optimize:
+            for f in order:
+                if f.op == "blockReduceSum":
+                    mode = self._fusable(f)
+                    if mode is not None:
+                        fused[id(f)] = mode
+                        skip.add(id(f.inputs[0]))
         for f in order:
             if f.op is None:
                 self._buffers[id(f)] = L.DeviceBuffer.from_array(f.data)
         for f in order:
             if f.op is None or id(f) in skip:
                 continue
-            if f.op == "blockReduceSum" and self.optimize:
-                mode = self._fusable(f)
-                if mode is not None:
-                    conv = f.inputs[0]
-                    skip.add(id(conv))
-                    self._add_conv(f, conv, mode)
-                    continue
+            if id(f) in fused:
+                self._add_conv(f, f.inputs[0], fused[id(f)])
+                continue
             if f.op == "convolve":
                 self._add_conv(f, f, f.params["mode"])
             else:

@@ -1,0 +1,304 @@
+// FP32 implicit GEMM on MFMA for FilterAdjointBlockReduceSum with BorderValid
+// (the weight gradient of a layer):
+//
+//   dW[k*N + n](R, C) = sum_b sum_{i,j} tap_{b,k}(i, j) * X_{b*N + n}(R*dn + i, C*dn + j)
+//
+// tap = dy (cross-correlation) or dy flipped (convolution), zero-inserted when
+// up-sampled (ConvolveHyperKernel.scala:1310-1336; ConvolveToSmallFieldPipelinedTiledHyperKernel.scala:45-127).
+// As a GEMM: M = K (dy planes per image), columns j = (n, R, C) (N*OH*OW of
+// them, which is the output layout), and the reduction runs over
+// (image b, tap row i, tap column j) -- tens of thousands of terms.
+//
+// gfx950 mapping: v_mfma_f32_16x16x4_f32; each MFMA consumes four consecutive
+// tap columns of one tap row (the four k-lanes, lane/16).  A block (4 wave64)
+// stages RR tap rows of all K dy planes of one image (stored k-fastest with a
+// row pitch of 16 mod 32 words, so the A-operand reads are bank-conflict free)
+// plus the x rows they touch, for the input planes its column group needs;
+// staging runs one row per wave pass with lanes across columns.  Wave w owns column
+// blocks w, w+4, .. (CBW of them) and all MB row blocks.  Blocks split the
+// (image, tap-row chunk) space; each writes one partial dW, and the partials
+// are added in a fixed order by a two-level reduction, so results do not
+// depend on scheduling.
+#include "internal.h"
+
+namespace cogconv {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <int MB, int CBW>
+__global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
+                                                      const float* __restrict__ DY, float* __restrict__ ws) {
+    // Staged dy rows are k-fastest with a pitch of 16*MB padded to 16 mod 32 words.
+    constexpr int MTP = MB == 1 ? 16 : (MB == 4 ? 80 : 48);
+    extern __shared__ float smem[];
+    const int MTp = MTP;
+    float* sD = smem;                                  // [RR][DWp][MTP]
+    float* sX = smem + (size_t)c.RR * c.DWp * MTP;     // [nCnt][xRows][xCols]
+    const int FHW = g.FH * g.FW;                       // 32-bit offsets: the plan checks the sizes
+    const int HWx = g.H * g.W;
+
+    const int cg = blockIdx.x % c.colGroups;
+    const int s = blockIdx.x / c.colGroups;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g4 = lane >> 4, l16 = lane & 15;
+    const int N = m.N, K = c.K;
+    // Per-lane column offsets (two 64-lane passes at most) of the staged dy
+    // rows (flip and up-sampling lattice applied) and x rows.
+    const bool dTwo = c.DWp > 64, xTwo = c.xCols > 64;
+    int dCol0, dCol1, xCol0, xCol1;
+    bool dIn0, dIn1, xIn0, xIn1;
+    {
+        int dc[2], xc[2];
+        bool di[2], xi[2];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const int jj = lane + 64 * p;
+            int tj = g.flip ? g.FWe - 1 - jj : jj;
+            bool on = jj < g.FWe;
+            if (g.upFc > 1) { on = on && (tj % g.upFc) == 0; tj /= g.upFc; }
+            dc[p] = min(max(tj, 0), g.FW - 1);
+            di[p] = on;
+            xc[p] = min(jj, g.W - 1);
+            xi[p] = jj < g.W;
+        }
+        dCol0 = dc[0]; dCol1 = dc[1]; dIn0 = di[0]; dIn1 = di[1];
+        xCol0 = xc[0]; xCol1 = xc[1]; xIn0 = xi[0]; xIn1 = xi[1];
+    }
+    const int P = g.OH * g.OW;
+    const int Ncols = N * P;
+    const int colBase = cg * 4 * CBW * 16;
+    const int n_lo = colBase / P;
+    const int n_hi = min(N - 1, (min(colBase + 4 * CBW * 16, Ncols) - 1) / P);
+    const int nCnt = n_hi - n_lo + 1;
+    const int xPlane = c.xRows * c.xCols;
+
+    int xoff[CBW];
+#pragma unroll
+    for (int q = 0; q < CBW; q++) {
+        const int j = colBase + (wave + 4 * q) * 16 + l16;
+        int off = 0;
+        if (j < Ncols) {
+            const int n = j / P, rem = j - n * P;
+            const int R = rem / g.OW, C = rem - R * g.OW;
+            off = (n - n_lo) * xPlane + R * g.dnR * c.xCols + C * g.dnC;
+        }
+        xoff[q] = off + g4;
+    }
+
+    floatx4 acc[MB][CBW];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int q = 0; q < CBW; q++) acc[i][q] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const long items = (long)m.B * c.rowChunks;
+    const long it0 = items * s / c.splits, it1 = items * (s + 1) / c.splits;
+    for (long it = it0; it < it1; it++) {
+        const int b = (int)(it / c.rowChunks);
+        const int i0 = (int)(it - (long)b * c.rowChunks) * c.RR;
+        const int rr = min(c.RR, g.FHe - i0);
+        __syncthreads();
+        // dy tap rows and x rows: one row per wave pass, lanes across columns
+        // (at most 128).  Every item loads unconditionally from a clamped,
+        // in-bounds address and masks afterwards, so the U loads of a wave are
+        // all in flight before the first LDS store.
+        // Staging: one row per wave pass; the row of an item is wave-uniform
+        // (scalar work), the lanes' column offsets were computed once per
+        // block; loads are unconditional from in-bounds offsets and all U of a
+        // wave are in flight before the first LDS store.
+        constexpr int U = 24;
+        const int dRows = MTP * c.RR;  // (il, k)
+        const float* DYb = DY + (size_t)b * K * FHW;
+        for (int base = wave; base < dRows; base += 4 * U) {
+            float v0[U], v1[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int row = base + 4 * u;
+                const int il = row / MTP;
+                const int k = row - il * MTP;
+                int ti = i0 + il;
+                if (g.flip) ti = g.FHe - 1 - ti;
+                bool on = row < dRows && k < K && il < rr;
+                if (g.upFr > 1) { on = on && (ti % g.upFr) == 0; ti /= g.upFr; }
+                ti = min(max(ti, 0), g.FH - 1);
+                const float* src = DYb + min(k, K - 1) * FHW + ti * g.FW;
+                const float a0 = src[dCol0];
+                v0[u] = (on && dIn0) ? a0 : 0.f;
+                if (dTwo) {
+                    const float a1 = src[dCol1];
+                    v1[u] = (on && dIn1) ? a1 : 0.f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int row = base + 4 * u;
+                if (row < dRows) {
+                    const int il = row / MTP;
+                    float* dst = sD + (size_t)il * c.DWp * MTP + (row - il * MTP);
+                    if (lane < c.DWp) dst[lane * MTP] = v0[u];
+                    if (dTwo && lane + 64 < c.DWp) dst[(lane + 64) * MTP] = v1[u];
+                }
+            }
+        }
+        const int xRowsAll = nCnt * c.xRows;  // (plane, row)
+        const float* Xb = X + ((size_t)b * N + n_lo) * HWx;
+        for (int base = wave; base < xRowsAll; base += 4 * U) {
+            float v0[U], v1[U];
+            int nlq = base / c.xRows;
+            int lrq = base - nlq * c.xRows;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int gy = i0 + lrq;
+                const bool on = base + 4 * u < xRowsAll && gy < g.H;
+                const float* src = Xb + min(nlq, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W;
+                const float a0 = src[xCol0];
+                v0[u] = (on && xIn0) ? a0 : 0.f;
+                if (xTwo) {
+                    const float a1 = src[xCol1];
+                    v1[u] = (on && xIn1) ? a1 : 0.f;
+                }
+                lrq += 4;
+                while (lrq >= c.xRows) { lrq -= c.xRows; nlq++; }
+            }
+            nlq = base / c.xRows;
+            lrq = base - nlq * c.xRows;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (base + 4 * u < xRowsAll) {
+                    float* dst = sX + (size_t)nlq * xPlane + lrq * c.xCols;
+                    if (lane < c.xCols) dst[lane] = v0[u];
+                    if (xTwo && lane + 64 < c.xCols) dst[lane + 64] = v1[u];
+                }
+                lrq += 4;
+                while (lrq >= c.xRows) { lrq -= c.xRows; nlq++; }
+            }
+        }
+        __syncthreads();
+        for (int il = 0; il < rr; il++) {
+            const float* dRow = sD + ((size_t)il * c.DWp + g4) * MTp + l16;
+            const float* xRow = sX + il * c.xCols;
+            for (int jb = 0; jb < c.DWp; jb += 4) {
+                float av[MB];
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++) av[mb] = dRow[(size_t)jb * MTp + mb * 16];
+                float bv[CBW];
+#pragma unroll
+                for (int q = 0; q < CBW; q++) bv[q] = xRow[xoff[q] + jb];
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                    for (int q = 0; q < CBW; q++)
+                        acc[mb][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mb], bv[q], acc[mb][q], 0, 0, 0);
+            }
+        }
+    }
+    // Partial dW of this split, ws[s][k][j]: staged through LDS as
+    // [16*MB rows][64*CBW columns] and written row by row with coalesced stores.
+    // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
+    __syncthreads();
+    constexpr int GC = 64 * CBW;
+    float* sP = smem;
+#pragma unroll
+    for (int q = 0; q < CBW; q++)
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                sP[(mb * 16 + 4 * g4 + v) * GC + (wave + 4 * q) * 16 + l16] = acc[mb][q][v];
+    __syncthreads();
+    float* out = ws + (size_t)s * K * Ncols + colBase;
+    const int ncol = min(GC, Ncols - colBase);
+    for (int k = wave; k < K; k += 4)
+        for (int jl = lane; jl < ncol; jl += 64) out[(size_t)k * Ncols + jl] = sP[k * GC + jl];
+}
+
+// Sum `splits` partial arrays of `total` floats in a fixed order.  Groups of
+// 16 partials are first added in place (thread (group, idx) owns those 16
+// slots), then the group sums are added into `out`.
+__global__ void k_reduce_groups(float* __restrict__ ws, int splits, long total) {
+    const int groups = (splits + 15) / 16;
+    const long n = (long)groups * total;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int grp = (int)(t / total);
+        const long idx = t - (long)grp * total;
+        const int s0 = grp * 16;
+        const int cnt = min(16, splits - s0);
+        float v[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = i < cnt ? ws[(size_t)(s0 + i) * total + idx] : 0.f;
+        float acc = 0.f;
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc += v[i];
+        ws[(size_t)s0 * total + idx] = acc;
+    }
+}
+
+__global__ void k_reduce_final(const float* __restrict__ ws, float* __restrict__ out, int splits, long total) {
+    for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+        float acc = 0.f;
+        for (int s0 = 0; s0 < splits; s0 += 256) {
+            float v[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int s = s0 + 16 * i;
+                v[i] = s < splits ? ws[(size_t)s * total + idx] : 0.f;
+            }
+#pragma unroll
+            for (int i = 0; i < 16; i++) acc += v[i];
+        }
+        out[idx] = acc;
+    }
+}
+
+hipError_t launch_reduce_partials(float* ws, float* out, int splits, long total, hipStream_t s) {
+    if (splits > 16) {
+        const long n = (long)((splits + 15) / 16) * total;
+        const long blocks = std::min((n + 255) / 256, 65536L);
+        hipLaunchKernelGGL(k_reduce_groups, dim3((unsigned)blocks), dim3(256), 0, s, ws, splits, total);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const long blocks = std::min((total + 255) / 256, 65536L);
+    if (splits > 16) {
+        hipLaunchKernelGGL(k_reduce_final, dim3((unsigned)blocks), dim3(256), 0, s, ws, out, splits, total);
+    } else {
+        // 16 or fewer partials: one group, summed directly.
+        hipLaunchKernelGGL(k_reduce_groups, dim3((unsigned)std::min((total + 255) / 256, 65536L)), dim3(256), 0, s,
+                           ws, splits, total);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_reduce_final, dim3((unsigned)blocks), dim3(256), 0, s, ws, out, 1, total);
+    }
+    return hipGetLastError();
+}
+
+template <int MB>
+static hipError_t launch_mb(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
+                            float* ws, hipStream_t s, dim3 grid) {
+    if (c.CBW == 3)
+        hipLaunchKernelGGL((k_fa_igemm_f32<MB, 3>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws);
+    else
+        hipLaunchKernelGGL((k_fa_igemm_f32<MB, 2>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws);
+    return hipGetLastError();
+}
+
+hipError_t launch_fa_igemm_f32(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
+                               float* out, float* ws, hipStream_t s) {
+    const long blocks = (long)c.colGroups * c.splits;
+    if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    const dim3 grid((unsigned)blocks);
+    hipError_t e;
+    switch (c.MB) {
+    case 1: e = launch_mb<1>(g, m, c, x, dy, ws, s, grid); break;
+    case 2: e = launch_mb<2>(g, m, c, x, dy, ws, s, grid); break;
+    case 3: e = launch_mb<3>(g, m, c, x, dy, ws, s, grid); break;
+    default: e = launch_mb<4>(g, m, c, x, dy, ws, s, grid); break;
+    }
+    if (e != hipSuccess) return e;
+    const long total = (long)c.K * m.N * g.OH * g.OW;
+    return launch_reduce_partials(ws, out, c.splits, total, s);
+}
+
+}  // namespace cogconv

@@ -1,0 +1,264 @@
+// FP32 implicit-GEMM kernel on MFMA for ProjectFrameBlockReduceSum and
+// BackProjectFrameBlockReduceSum (the contraction modes of the conv path).
+//
+//   out[b*Mout + m](y, x) = sum_{r < R} sum_{t < F*F} A[m][r][t] * X_{b*N + r}(y*dn - h + ti, x*dn - h + tj)
+//
+// ProjectFrameBlockReduceSum:     A[m][r][t] = W[m*N + r][t]   (ConvolveHyperKernel.scala:153-167, 327-328)
+// BackProjectFrameBlockReduceSum: A[m][r][t] = W[r*L + m][t]   (ConvolveHyperKernel.scala:330-332)
+// with the filter flipped for the convolution orientation (a flip of both axes
+// of a row-major F x F filter reverses its flat tap index).  Borders are
+// applied while the input tile is staged in LDS (ConvolveHyperKernel.scala:235-317).
+//
+// gfx950 mapping: v_mfma_f32_16x16x4_f32.  A block of 512 threads (8 wave64)
+// owns an 8 x 32 pixel tile of one image and 16*MB output planes.  Waves w and
+// w+4 own pixel rows 2(w%4), 2(w%4)+1 (four 16-pixel MFMA column blocks);
+// wave w < 4 reduces over the first half of each staged plane chunk, wave
+// w >= 4 over the second half, and the two partial sums are added through LDS
+// at the end.  The four k-lanes of one MFMA (lane/16) carry four consecutive
+// input planes at the same tap.  Input planes are staged RC at a time: the
+// tile [RC][rows][cols] (one row per wave pass, lanes across columns) and the
+// filter slice [RC][F*F][16*MB] (each thread copies the F*F contiguous taps of
+// one (plane, filter) pair).  Both staged plane strides are 16 mod 32 words so
+// that the four k-lane groups of an MFMA read different LDS banks.
+#include "internal.h"
+
+namespace cogconv {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <int F, int MB>
+__global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg c, const float* __restrict__ X,
+                                                   const float* __restrict__ Wt, float* __restrict__ O) {
+    constexpr int TAPS = F * F;
+    constexpr int MT = 16 * MB;
+    extern __shared__ float smem[];
+    float* sB = smem;
+    float* sA = smem + (size_t)c.RC * c.planeStride;
+    const int aStride = c.aStride;  // floats per staged plane of the filter slice (TAPS*MT padded)
+
+    int bid = blockIdx.x;
+    const int tc = bid % c.tilesC;
+    bid /= c.tilesC;
+    const int tr = bid % c.tilesR;
+    bid /= c.tilesR;
+    const int mt = bid % c.mTiles;
+    const int b = bid / c.mTiles;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int rw = wave & 3, pg = wave >> 2;
+    const int g4 = lane >> 4, l16 = lane & 15;
+    const int Mout = c.Mout;
+    const int m0 = mt * MT;
+    const int y0 = tr * 8 * g.dnR - g.hR;
+    const int x0 = tc * 32 * g.dnC - g.hC;
+    const int TRp = c.tileRows, TCp = c.tileCols;
+    const bool bpf = m.mode == COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM;
+    const int L = m.T / m.N;
+    const int HW = g.H * g.W;                       // 32-bit offsets: the plan checks the sizes
+    const float* Xb = X + (size_t)b * m.N * HW;
+    // Per-lane column offsets of the staged tile (two 64-lane passes at most)
+    // with the border applied: the same for every staged row and plane.
+    const bool twoPass = TCp > 64;
+    int colOff0, colOff1;
+    bool colIn0, colIn1;
+    {
+        int xs[2];
+        bool ins[2];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            int x = x0 + lane + 64 * p;
+            bool in = (unsigned)x < (unsigned)g.W;
+            if (g.border == COGCONV_BORDER_CYCLIC) {
+                x %= g.W;
+                x += x < 0 ? g.W : 0;
+                in = true;
+            } else {
+                in = in || g.border == COGCONV_BORDER_CLAMP;
+                x = min(max(x, 0), g.W - 1);
+            }
+            xs[p] = x;
+            ins[p] = in;
+        }
+        colOff0 = xs[0]; colOff1 = xs[1];
+        colIn0 = ins[0]; colIn1 = ins[1];
+    }
+    const int half = c.RC >> 1;
+
+    floatx4 acc[MB][4];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    int boff[4];
+#pragma unroll
+    for (int pb = 0; pb < 4; pb++) {
+        const int row = 2 * rw + (pb >> 1);
+        const int col = (pb & 1) * 16 + l16;
+        boff[pb] = (row * g.dnR) * TCp + col * g.dnC;
+    }
+
+    for (int r0 = 0; r0 < m.R; r0 += c.RC) {
+        __syncthreads();
+        // Input tile: one (plane, row) item per wave pass.  The row of an item
+        // is the same for all lanes of a wave (scalar work); the lanes' column
+        // offsets and border masks were computed once per block.  Loads are
+        // unconditional from in-bounds offsets and the IU items of a wave are
+        // all in flight before the first LDS store.
+        const int nrows = c.RC * TRp;
+        constexpr int IU = 24;
+        for (int base = wave; base < nrows; base += 8 * IU) {
+            float v0[IU], v1[IU];
+            int rcq = base / TRp;
+            int tyq = base - rcq * TRp;
+#pragma unroll
+            for (int u = 0; u < IU; u++) {
+                const int r = r0 + rcq;
+                int y = y0 + tyq;
+                bool ym = (unsigned)y < (unsigned)g.H;
+                if (g.border == COGCONV_BORDER_CYCLIC) {
+                    y %= g.H;
+                    y += y < 0 ? g.H : 0;
+                    ym = true;
+                } else {
+                    ym = ym || g.border == COGCONV_BORDER_CLAMP;
+                    y = min(max(y, 0), g.H - 1);
+                }
+                const bool rowOn = ym && r < m.R && base + 8 * u < nrows;
+                const float* src = Xb + min(r, m.R - 1) * HW + y * g.W;
+                const float a0 = src[colOff0];
+                v0[u] = (rowOn && colIn0) ? a0 : 0.f;
+                if (twoPass) {
+                    const float a1 = src[colOff1];
+                    v1[u] = (rowOn && colIn1) ? a1 : 0.f;
+                }
+                tyq += 8;
+                if (tyq >= TRp) { tyq -= TRp; rcq++; }
+            }
+            rcq = base / TRp;
+            tyq = base - rcq * TRp;
+#pragma unroll
+            for (int u = 0; u < IU; u++) {
+                if (base + 8 * u < nrows) {
+                    float* dst = sB + rcq * c.planeStride + tyq * TCp;
+                    if (lane < TCp) dst[lane] = v0[u];
+                    if (twoPass && lane + 64 < TCp) dst[lane + 64] = v1[u];
+                }
+                tyq += 8;
+                if (tyq >= TRp) { tyq -= TRp; rcq++; }
+            }
+        }
+        // Filter slice: thread -> (plane rc, filter mm); the F*F taps are
+        // contiguous in W.  Loads are unconditional from a clamped pair.
+        for (int q = tid; q < c.RC * MT; q += 512) {
+            const int mm = q % MT, rc = q / MT;
+            const int r = r0 + rc, mg = m0 + mm;
+            const bool ok = r < m.R && mg < Mout;
+            const int rr = min(r, m.R - 1), mgc = min(mg, Mout - 1);
+            const float* src = Wt + (size_t)(bpf ? rr * L + mgc : mgc * m.N + rr) * TAPS;
+            float t[TAPS];
+#pragma unroll
+            for (int i = 0; i < TAPS; i++) t[i] = src[i];
+            float* dst = sA + (size_t)rc * aStride + mm;
+#pragma unroll
+            for (int i = 0; i < TAPS; i++) dst[i * MT] = ok ? (g.flip ? t[TAPS - 1 - i] : t[i]) : 0.f;
+        }
+        __syncthreads();
+        for (int p = pg * half; p < (pg + 1) * half; p += 4) {
+            const float* bb = sB + (p + g4) * c.planeStride;
+            const float* ab = sA + (size_t)(p + g4) * aStride + l16;
+#pragma unroll
+            for (int ti = 0; ti < F; ti++) {
+#pragma unroll
+                for (int tj = 0; tj < F; tj++) {
+                    const int t = ti * F + tj;
+                    float av[MB];
+#pragma unroll
+                    for (int mb = 0; mb < MB; mb++) av[mb] = ab[t * MT + mb * 16];
+                    float bv[4];
+#pragma unroll
+                    for (int pb = 0; pb < 4; pb++) bv[pb] = bb[boff[pb] + ti * TCp + tj];
+#pragma unroll
+                    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                        for (int pb = 0; pb < 4; pb++)
+                            acc[mb][pb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mb], bv[pb], acc[mb][pb], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // Epilogue: the two plane groups' sums meet in an LDS output tile
+    // [16*MB planes][8 rows][32 cols] (plane pitch 260 words), which is then
+    // written row by row with coalesced stores.
+    // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
+    __syncthreads();
+    constexpr int OP = 260;
+    float* sO = smem;
+    if (pg == 1) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int pb = 0; pb < 4; pb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int mm = mb * 16 + 4 * g4 + v;
+                    sO[mm * OP + (2 * rw + (pb >> 1)) * 32 + (pb & 1) * 16 + l16] = acc[mb][pb][v];
+                }
+    }
+    __syncthreads();
+    if (pg == 0) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int pb = 0; pb < 4; pb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int mm = mb * 16 + 4 * g4 + v;
+                    sO[mm * OP + (2 * rw + (pb >> 1)) * 32 + (pb & 1) * 16 + l16] += acc[mb][pb][v];
+                }
+    }
+    __syncthreads();
+    const int col = tid & 31;
+    const int ocol = tc * 32 + col;
+    for (int item = tid >> 5; item < MT * 8; item += 16) {
+        const int mm = item >> 3, row = item & 7;
+        const int orow = tr * 8 + row;
+        if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
+            O[((size_t)(b * Mout + m0 + mm) * g.OH + orow) * g.OW + ocol] = sO[mm * OP + row * 32 + col];
+    }
+}
+
+template <int F>
+static hipError_t launch_f(int MB, const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x,
+                           const float* w, float* out, hipStream_t s, dim3 grid) {
+    switch (MB) {
+    case 1: hipLaunchKernelGGL((k_igemm_f32<F, 1>), grid, dim3(512), c.lds_bytes, s, g, m, c, x, w, out); break;
+    case 2: hipLaunchKernelGGL((k_igemm_f32<F, 2>), grid, dim3(512), c.lds_bytes, s, g, m, c, x, w, out); break;
+    default: hipLaunchKernelGGL((k_igemm_f32<F, 4>), grid, dim3(512), c.lds_bytes, s, g, m, c, x, w, out); break;
+    }
+    return hipGetLastError();
+}
+
+bool igemm_f32_supported(int F) { return F == 1 || F == 3 || F == 5 || F == 7 || F == 9; }
+
+// LDS floats the output tile of k_igemm_f32's epilogue needs.
+size_t igemm_f32_exchange_floats(int MB) { return (size_t)16 * MB * 260; }
+
+hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
+                            float* out, hipStream_t s) {
+    const long blocks = (long)c.tilesC * c.tilesR * c.mTiles * m.B;
+    if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    const dim3 grid((unsigned)blocks);
+    switch (g.FH) {
+    case 1: return launch_f<1>(c.MB, g, m, c, x, w, out, s, grid);
+    case 3: return launch_f<3>(c.MB, g, m, c, x, w, out, s, grid);
+    case 5: return launch_f<5>(c.MB, g, m, c, x, w, out, s, grid);
+    case 7: return launch_f<7>(c.MB, g, m, c, x, w, out, s, grid);
+    case 9: return launch_f<9>(c.MB, g, m, c, x, w, out, s, grid);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace cogconv

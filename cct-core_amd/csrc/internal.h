@@ -91,11 +91,22 @@ __host__ __device__ inline void plane_pair(const PlaneMap& m, int o, int r, int&
     }
 }
 
+// n / d for 0 <= n < 2^22 and 1 <= d, from a float reciprocal plus one
+// correction step each way (exact in that range).  Used by the staging loops
+// in place of an integer division.
+__device__ __forceinline__ int fast_div(int n, int d, float inv_d) {
+    int q = (int)((float)n * inv_d);
+    q -= (q * d > n) ? 1 : 0;
+    q += ((q + 1) * d <= n) ? 1 : 0;
+    return q;
+}
+
 // Kernel families.
 enum KernelKind {
     KK_DIRECT = 1,
     KK_SMALL_OUTPUT = 2,
     KK_IGEMM = 3,
+    KK_FA_IGEMM = 4,
 };
 
 // LDS-tiled direct kernel configuration.
@@ -116,10 +127,37 @@ struct SmallCfg {
 
 // Implicit-GEMM configuration (MFMA).
 struct IgemmCfg {
-    int variant;  // which kernel instantiation
-    int Mtile, Ntile;
-    int splits;   // split-K count (FilterAdjoint)
-    int grid;
+    int variant;      // kernel family (IgemmVariant)
+    int MB;           // 16-plane output blocks per workgroup
+    int RC;           // input planes staged in LDS per pass (multiple of 4)
+    int Mout;         // output planes per image
+    int mTiles;       // ceil(Mout / (16*MB))
+    int tilesR, tilesC;
+    int tileRows, tileCols;  // staged input tile (rows x cols)
+    int planeStride;  // LDS floats per staged input plane (padded)
+    int aStride;      // LDS floats per staged filter-slice plane (padded)
+    int splits;       // split-K count (FilterAdjoint)
+    size_t lds_bytes;
+};
+
+// FilterAdjointBlockReduceSum implicit GEMM (split reduction).
+struct FaCfg {
+    int K;          // dy planes per image (GEMM rows)
+    int MB;         // 16-row blocks (K <= 16*MB <= 64)
+    int CBW;        // 16-column blocks per wave
+    int RR;         // tap rows staged per item
+    int DWp;        // tap columns rounded up to 4
+    int MTp;        // staged dy row pitch: 16*MB padded to 16 mod 32
+    int xRows, xCols;
+    int nCntMax;    // input planes staged per column group
+    int colGroups;  // column groups of 64*CBW columns
+    int rowChunks;  // ceil(FHe / RR)
+    int splits;     // partial results
+    size_t lds_bytes;
+};
+
+enum IgemmVariant {
+    IV_F32_FRAME = 1,  // FP32 MFMA, ProjectFrame/BackProjectFrame block-reduce modes
 };
 
 // Kernel launchers (device code in *.hip).
@@ -127,6 +165,12 @@ hipError_t launch_direct(const Geom& g, const PlaneMap& m, const DirectCfg& c, c
                          float* out, hipStream_t s);
 hipError_t launch_small_output(const Geom& g, const PlaneMap& m, const SmallCfg& c, const float* a, const float* f,
                                float* out, float* workspace, hipStream_t s);
+bool igemm_f32_supported(int F);
+size_t igemm_f32_exchange_floats(int MB);
+hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
+                            float* out, hipStream_t s);
+hipError_t launch_fa_igemm_f32(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
+                               float* out, float* ws, hipStream_t s);
 hipError_t launch_block_reduce_sum(const float* in, float* out, int64_t points, int out_planes, int factor,
                                    hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
@@ -142,6 +186,7 @@ struct cogconv_plan_s {
     cogconv::DirectCfg dcfg;
     cogconv::SmallCfg scfg;
     cogconv::IgemmCfg icfg;
+    cogconv::FaCfg fcfg;
     size_t workspace;
     double flops;
     double bytes;

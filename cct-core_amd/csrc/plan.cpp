@@ -217,6 +217,95 @@ DirectCfg choose_direct(const Geom& g, int nOut) {
     return c;
 }
 
+// FP32 MFMA implicit GEMM for the frame block-reduce modes: square odd-or-1
+// filters up to 9x9 on 2-D fields without input up-sampling.
+bool igemm_frame_eligible(const cogconv_desc* d, const Geom& g) {
+    if (d->mode != COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM && d->mode != COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM)
+        return false;
+    if (d->in0.dims != 2 || g.FH != g.FW || g.upAr != 1 || g.upAc != 1 || g.dnR != g.dnC) return false;
+    if (31 * g.dnC + g.FW > 128) return false;  // staged tile rows are at most two 64-lane passes
+    if ((long)d->in0.planes * g.H * g.W >= (1L << 31)) return false;  // 32-bit staging offsets
+    return igemm_f32_supported(g.FH);
+}
+
+IgemmCfg choose_igemm_frame(const Geom& g, const PlaneMap& m) {
+    const size_t limit = 80 * 1024;  // two blocks per CU (160 KB of LDS)
+    IgemmCfg c;
+    std::memset(&c, 0, sizeof c);
+    c.variant = IV_F32_FRAME;
+    c.Mout = m.T / m.N;
+    c.tilesR = (g.OH + 7) / 8;
+    c.tilesC = (g.OW + 31) / 32;
+    c.tileRows = 7 * g.dnR + g.FH;
+    c.tileCols = 31 * g.dnC + g.FW;
+    int ps = c.tileRows * c.tileCols;
+    ps += ((16 - ps % 32) + 32) % 32;  // plane stride = 16 mod 32 words: the four k-lane groups hit different banks
+    c.planeStride = ps;
+    const int taps = g.FH * g.FW;
+    const int maxRC = std::min(32, ((m.R + 7) / 8) * 8);
+    // Largest output block (MB) and plane chunk (RC, a multiple of 8: two
+    // plane groups of whole MFMA k-quads) whose staging fits in LDS.
+    for (int MB = c.Mout <= 16 ? 1 : (c.Mout <= 32 ? 2 : 4); MB >= 1; MB = MB == 4 ? 2 : MB - 1) {
+        int as = taps * 16 * MB;
+        as += ((16 - as % 32) + 32) % 32;
+        for (int rc = maxRC; rc >= 8; rc -= 8) {
+            const int RC = rc;
+            const size_t stage = (size_t)RC * (ps + as) * sizeof(float);
+            const size_t xch = igemm_f32_exchange_floats(MB) * sizeof(float);
+            const size_t bytes = std::max(stage, xch);
+            if (bytes <= limit) {
+                c.MB = MB;
+                c.RC = RC;
+                c.aStride = as;
+                c.lds_bytes = bytes;
+                c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
+                return c;
+            }
+        }
+        if (MB == 1) break;
+    }
+    c.MB = 0;  // does not fit: caller falls back
+    return c;
+}
+
+// FP32 MFMA split-reduction GEMM for FilterAdjointBlockReduceSum, BorderValid.
+bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out) {
+    if (d->mode != COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM || d->border != COGCONV_BORDER_VALID || d->in0.dims != 2)
+        return false;
+    FaCfg c;
+    std::memset(&c, 0, sizeof c);
+    c.K = m.T / m.B;
+    if (c.K > 64) return false;
+    c.MB = (c.K + 15) / 16;
+    const int P = g.OH * g.OW;
+    const long ncols = (long)m.N * P;
+    c.CBW = ncols <= 192 ? 3 : 2;
+    const long colBlocks = (ncols + 15) / 16;
+    c.colGroups = (int)((colBlocks + 4 * c.CBW - 1) / (4 * c.CBW));
+    c.nCntMax = std::min(m.N, (64 * c.CBW + P - 1) / P + 1);
+    c.DWp = (g.FWe + 3) / 4 * 4;
+    c.xCols = (g.OW - 1) * g.dnC + c.DWp;
+    if (c.DWp > 128 || c.xCols > 128) return false;  // staged rows are at most two 64-lane passes
+    c.MTp = c.MB == 1 ? 16 : (c.MB == 4 ? 80 : 48);  // 16*MB padded to 16 mod 32 (k_fa_igemm_f32's MTP)
+    if ((long)m.B * m.N * g.H * g.W >= (1L << 31) || (long)m.B * c.K * g.FH * g.FW >= (1L << 31)) return false;
+    c.RR = 0;
+    for (int rr = std::min(g.FHe, 16); rr >= 1; rr--) {
+        const int xRows = rr + (g.OH - 1) * g.dnR;
+        const size_t bytes = ((size_t)c.MTp * rr * c.DWp + (size_t)c.nCntMax * xRows * c.xCols) * sizeof(float);
+        if (bytes <= 48 * 1024) { c.RR = rr; break; }
+    }
+    if (c.RR == 0) return false;
+    c.xRows = c.RR + (g.OH - 1) * g.dnR;
+    c.lds_bytes = ((size_t)c.MTp * c.RR * c.DWp + (size_t)c.nCntMax * c.xRows * c.xCols) * sizeof(float);
+    c.lds_bytes = std::max(c.lds_bytes, (size_t)16 * c.MB * 64 * c.CBW * sizeof(float));  // epilogue tile
+    c.rowChunks = (g.FHe + c.RR - 1) / c.RR;
+    const long items = (long)m.B * c.rowChunks;
+    long want = (768 + c.colGroups - 1) / c.colGroups;
+    c.splits = (int)std::max(1L, std::min(items, want));
+    *out = c;
+    return true;
+}
+
 SmallCfg choose_small(const Geom& g, const PlaneMap& m) {
     SmallCfg c;
     c.qchunks = (g.OH * g.OW + 15) / 16;
@@ -259,13 +348,28 @@ extern "C" int cogconv_plan_create(const cogconv_desc* desc, cogconv_plan* plan)
     const long outPts = (long)g.OH * g.OW;
     const bool small_out = g.border == COGCONV_BORDER_VALID && is_fa(desc->mode) && outPts <= 256 &&
                            (long)g.FHe * g.FWe >= 64;
-    int algo = desc->algo;
-    if (algo == COGCONV_ALGO_IMPLICIT_GEMM) algo = COGCONV_ALGO_AUTO;  // FP32 plans use the VALU kernels
+    const int algo = desc->algo;
     if (algo == COGCONV_ALGO_SMALL_OUTPUT && outPts > 4096) {
         delete p;
         return fail(COGCONV_ERR_UNSUPPORTED, "small-output kernel needs an output plane of at most 4096 points");
     }
-    if (algo == COGCONV_ALGO_SMALL_OUTPUT || (algo == COGCONV_ALGO_AUTO && small_out)) {
+    const bool igemm_ok = igemm_frame_eligible(desc, g) && choose_igemm_frame(g, m).MB > 0;
+    FaCfg probe_fa;
+    if (algo == COGCONV_ALGO_IMPLICIT_GEMM && !igemm_ok && !choose_fa_igemm(desc, g, m, &probe_fa)) {
+        delete p;
+        return fail(COGCONV_ERR_UNSUPPORTED, "implicit GEMM needs a block-reduce frame mode with a square 1/3/5/7/9 "
+                                             "filter on 2-D fields and no input up-sampling, or FilterAdjoint with "
+                                             "block reduction, BorderValid and at most 64 planes per image in dy");
+    }
+    if ((algo == COGCONV_ALGO_AUTO || algo == COGCONV_ALGO_IMPLICIT_GEMM) && igemm_ok) {
+        p->kind = KK_IGEMM;
+        p->icfg = choose_igemm_frame(g, m);
+        p->workspace = 0;
+    } else if ((algo == COGCONV_ALGO_AUTO || algo == COGCONV_ALGO_IMPLICIT_GEMM) &&
+               choose_fa_igemm(desc, g, m, &p->fcfg)) {
+        p->kind = KK_FA_IGEMM;
+        p->workspace = (size_t)p->fcfg.splits * p->fcfg.K * m.N * outPts * sizeof(float);
+    } else if (algo == COGCONV_ALGO_SMALL_OUTPUT || (algo == COGCONV_ALGO_AUTO && small_out)) {
         p->kind = KK_SMALL_OUTPUT;
         p->scfg = choose_small(g, m);
         p->workspace = (size_t)p->scfg.splits * m.nOut * outPts * sizeof(float);
@@ -333,8 +437,14 @@ extern "C" int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len) {
         snprintf(buf, len, "small_output %s out=%dx%dx%d splits=%d", mode_name(plan->desc.mode), plan->m.nOut, g.OH,
                  g.OW, plan->scfg.splits);
         break;
+    case KK_FA_IGEMM:
+        snprintf(buf, len, "fa_igemm_f32 %s out=%dx%dx%d MB=%d CBW=%d RR=%d groups=%d splits=%d lds=%zu",
+                 mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->fcfg.MB, plan->fcfg.CBW, plan->fcfg.RR,
+                 plan->fcfg.colGroups, plan->fcfg.splits, plan->fcfg.lds_bytes);
+        break;
     default:
-        snprintf(buf, len, "igemm %s variant=%d", mode_name(plan->desc.mode), plan->icfg.variant);
+        snprintf(buf, len, "igemm_f32 %s out=%dx%dx%d MB=%d RC=%d tile=8x32 threads=512 lds=%zu", mode_name(plan->desc.mode),
+                 plan->m.nOut, g.OH, g.OW, plan->icfg.MB, plan->icfg.RC, plan->icfg.lds_bytes);
         break;
     }
     return COGCONV_OK;
@@ -354,6 +464,12 @@ extern "C" int cogconv_launch(cogconv_plan plan, const float* in0, const float* 
         break;
     case KK_SMALL_OUTPUT:
         e = launch_small_output(plan->g, plan->m, plan->scfg, in0, in1, out, static_cast<float*>(workspace), s);
+        break;
+    case KK_IGEMM:
+        e = launch_igemm_f32(plan->g, plan->m, plan->icfg, in0, in1, out, s);
+        break;
+    case KK_FA_IGEMM:
+        e = launch_fa_igemm_f32(plan->g, plan->m, plan->fcfg, in0, in1, out, static_cast<float*>(workspace), s);
         break;
     default:
         return fail(COGCONV_ERR_RUNTIME, "internal: unknown kernel kind %d", plan->kind);

@@ -123,7 +123,7 @@ def test_plan_flops_match_the_algorithmic_counts():
 
 def test_plan_is_host_only_and_describes_its_kernel():
     p = c2_plan(L.FILTER_ADJOINT_BLOCK_REDUCE_SUM, L.BORDER_VALID)
-    assert p.description.startswith(("small_output", "igemm"))
+    assert p.description.startswith(("small_output", "fa_igemm", "igemm"))
     assert p.workspace_bytes >= 0
     q = c2_plan(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_CYCLIC)
     assert q.out_shape == (512, 64, 64)

@@ -164,6 +164,43 @@ def test_frame_modes_gpu(mode, border, sampling, step, orient):
     assert_parity(got, ref, mag, what="%s b%d s%d" % (mode, border, sampling))
 
 
+ALGO_CASES = []
+for mode in ("pf_brs", "bpf_brs", "fa_brs"):
+    for border in (O.ZERO, O.CLAMP, O.CYCLIC, O.VALID, O.FULL):
+        for sampling, step in ((O.NONE, 1), (O.DOWN, 2)):
+            ALGO_CASES.append((mode, border, sampling, step))
+
+
+@pytest.mark.parametrize("mode,border,sampling,step", ALGO_CASES)
+@pytest.mark.parametrize("algo", (L.ALGO_DIRECT, L.ALGO_IMPLICIT_GEMM, L.ALGO_SMALL_OUTPUT))
+def test_every_kernel_family_gpu(mode, border, sampling, step, algo):
+    """Each kernel family, forced through the algorithm hint, on square-filter frame cases."""
+    rng = np.random.default_rng(zlib.crc32(repr((mode, border, sampling, step, algo)).encode()))
+    B, N, K, H, W = 2, 5, 3, 12, 20
+    if mode == "fa_brs":
+        a = rng.uniform(-1, 1, (B * N, H, W)).astype(np.float32)
+        k = rng.uniform(-1, 1, (B * K, H - 2, W - 2) if border == O.VALID else (B * K, 5, 5)).astype(np.float32)
+    else:
+        a = rng.uniform(-1, 1, (B * N, H, W)).astype(np.float32)
+        k = rng.uniform(-1, 1, (K * N, 3, 3)).astype(np.float32)
+    od, cd = desc_pair(MODES[mode], border, O.CONV if mode == "bpf_brs" else O.XCORR, sampling, step, B,
+                       a.shape, k.shape, algo=algo)
+    try:
+        O.output_type(od)
+    except O.OracleError:
+        return
+    try:
+        plan = L.Plan(cd)
+    except L.CogconvError as e:
+        assert e.status == L.ERR_UNSUPPORTED
+        return
+    announce("algo", algo, plan.description)
+    plan.close()
+    got = run_gpu(cd, a, k)
+    ref, mag = oracle_ref(od, a, k)
+    assert_parity(got, ref, mag, what="%s algo %d" % (mode, algo))
+
+
 def test_c1_project_frame_block_reduce_sum():
     """x 32x32x3 uniform[-1,1), 8 filters 5x5x3, BorderZero: projectFrame + blockReduceSum(3) -> 32x32x8."""
     announce("c1")
@@ -176,6 +213,7 @@ def test_c1_project_frame_block_reduce_sum():
         cogconv.probe(y)
     cg.step()
     assert [p.description.split()[1] for p in cg.plans()] == ["ProjectFrameBlockReduceSum"]
+    print(cg.plans()[0].description)
     got = cg.read(y)
     cg.release()
     od = O.make_desc(O.PF_BRS, O.ZERO, O.XCORR, O.NONE, 1, 1, x.shape, w.shape)
