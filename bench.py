@@ -84,6 +84,80 @@ def cpu_baseline(sample_images=16, budget_s=10.0):
                       % (B, flops / 1e9, len(times))}
 
 
+def precision_workload(args):
+    """B32 128x128x64 planes, 64 filters 3x3x64, BorderZero: ProjectFrame+BRS and
+    BackProjectFrame+BRS in FP32, BF16, FP16 and split-BF16; GFLOP/s from HIP
+    events, errors against the FP64 oracle on a 2-image slice (uniform[-1,1)
+    and wide-range 10^U(-3,3) with random signs)."""
+    import cogconv._lib as L
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    L.init(0)
+    rng = np.random.default_rng(3)
+    B, N, K, H = 32, 64, 64, 128
+    x = rng.uniform(-1, 1, (B * N, H, H)).astype(np.float32)
+    w = rng.uniform(-1, 1, (K * N, 3, 3)).astype(np.float32)
+    dy = rng.uniform(-1, 1, (B * K, H, H)).astype(np.float32)
+    bx, bw, bdy = (L.DeviceBuffer.from_array(a) for a in (x, w, dy))
+    f = lambda planes: (2, H, H, planes)  # noqa: E731
+    stream = L.Stream()
+    names = {L.FP32: "fp32", L.BF16: "bf16", L.FP16: "fp16", L.SPLIT_BF16: "split_bf16"}
+    res = {}
+    for prec in (L.FP32, L.BF16, L.FP16, L.SPLIT_BF16):
+        for opname, mode, orient, a, ab in (("projectFrame+BRS", L.PROJECT_FRAME_BLOCK_REDUCE_SUM,
+                                             L.CROSS_CORRELATION, x, bx),
+                                            ("backProjectFrame+BRS", L.BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM,
+                                             L.CONVOLUTION, dy, bdy)):
+            plan = L.Plan(L.make_desc(mode, L.BORDER_ZERO, orient, L.NO_SAMPLING, 1, B, f(a.shape[0]),
+                                      (2, 3, 3, K * N), prec))
+            out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
+            ws = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
+            for _ in range(3):
+                plan.launch(ab, bw, out, ws, stream)
+            e0, e1 = L.Event(), L.Event()
+            reps = 20
+            e0.record(stream)
+            for _ in range(reps):
+                plan.launch(ab, bw, out, ws, stream)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1) / reps
+            res.setdefault(names[prec], {})[opname] = {
+                "ms": round(ms, 4), "gflops": round(plan.flops / (ms * 1e-3) / 1e9, 1), "kernel": plan.description}
+            log(names[prec], opname, plan.description, "%.4f ms" % ms)
+            out.close()
+            if ws:
+                ws.close()
+    # Errors on a 2-image slice, both input sets.
+    errs = {}
+    for label, gen in (("uniform", lambda sh: rng.uniform(-1, 1, sh)),
+                       ("wide", lambda sh: 10.0 ** rng.uniform(-3, 3, sh) * rng.choice([-1.0, 1.0], sh))):
+        xs = gen((2 * N, H, H)).astype(np.float32)
+        ws_ = gen((K * N, 3, 3)).astype(np.float32)
+        od = O.make_desc(O.PF_BRS, O.ZERO, O.XCORR, O.NONE, 1, 2, xs.shape, ws_.shape)
+        ref = O.conv(od, xs, ws_, threads=16)
+        for prec in (L.FP32, L.BF16, L.FP16, L.SPLIT_BF16):
+            plan = L.Plan(L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CROSS_CORRELATION,
+                                      L.NO_SAMPLING, 1, 2, f(2 * N), (2, 3, 3, K * N), prec))
+            a, k = L.DeviceBuffer.from_array(xs), L.DeviceBuffer.from_array(ws_)
+            out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
+            wsb = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
+            plan.launch(a, k, out, wsb, None)
+            L.synchronize()
+            got = out.download(plan.out_shape).astype(np.float64)
+            d = np.abs(got - ref)
+            errs.setdefault(names[prec], {})[label] = {
+                "max_abs": float(d.max()), "max_rel_normwise": float(d.max() / np.abs(ref).max()),
+                "rel_l2": float(np.sqrt((d ** 2).sum() / (ref ** 2).sum()))}
+    fp32 = res["fp32"]["projectFrame+BRS"]["gflops"]
+    line = {"metric": "conv GFLOP/s & % roofline (FP32 vs BF16-MFMA); p50/p99 request latency",
+            "workload": "precision variants: batch 32 images 128x128x64, 64 filters 3x3x64, BorderZero",
+            "results": res, "errors_vs_fp64": errs,
+            "bf16_over_fp32_projectFrame": round(res["bf16"]["projectFrame+BRS"]["gflops"] / fp32, 2),
+            "peaks": {"fp32_tflops": PEAK_FP32_TFLOPS, "bf16_tflops": PEAK_BF16_TFLOPS, "hbm_gbs": PEAK_HBM_GBS}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,16 +166,31 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--serial", action="store_true", help="run the three ops of a step back to back on one stream")
+    ap.add_argument("--workload", default="cnn-triple", choices=("cnn-triple", "precision"))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run three 64 MiB device-to-device copies (a known-size reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "precision":
+        precision_workload(args)
+        return
 
     import cogconv._lib as L
     L.init(0)
     props = L.device_props(0)
     log("device:", props)
+
+    if args.copy_calibration:
+        src, dst = L.DeviceBuffer(64 << 20), L.DeviceBuffer(64 << 20)
+        src.zero()
+        for _ in range(3):
+            L.check(L.lib().cogconv_d2d_async(dst.ptr, src.ptr, 64 << 20, None))
+        L.synchronize()
+        src.close()
+        dst.close()
 
     x, w, dy = make_inputs()
     bx, bw, bdy = (L.DeviceBuffer.from_array(a) for a in (x, w, dy))
@@ -187,6 +276,13 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None}
     roof["kernel"] = ops[dom]["name"] + ": " + dplan.description
+    tfile = os.path.join(ROOT, "profiles", "traffic_cnn_triple.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            t = json.load(fh).get(ops[dom]["name"])
+        if t:
+            roof["traffic"] = t["hbm_bytes_per_launch"]
+            roof["traffic_source"] = "profiles/traffic_cnn_triple.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
     roof["algorithmic_flops"] = dplan.flops
     roof["algorithmic_bytes"] = dplan.bytes
     roof["avg_launch_ms"] = round(per_op_ms[dom], 5)

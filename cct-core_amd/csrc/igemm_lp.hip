@@ -1,0 +1,347 @@
+// Precision variants of the frame block-reduce modes: BF16 / FP16 MFMA
+// implicit GEMM with FP32 accumulation, and split-BF16 (three-term compensated
+// products).  Same contraction as igemm_f32.hip:
+//
+//   out[b*Mout + m](y, x) = sum_{r < R} sum_{t < F*F} A[m][r][t] * X_{b*N + r}(y*dn - h + ti, x*dn - h + tj)
+//
+// with A from the filter bank (ProjectFrame: W[m*N + r], BackProjectFrame:
+// W[r*L + m], flipped for the convolution orientation;
+// ConvolveHyperKernel.scala:153-167, 327-332) and borders applied while the
+// input tile is staged (ConvolveHyperKernel.scala:235-317).  Operands arrive
+// in FP32 (the Field storage type) and are rounded to the MFMA input type
+// while they are staged.
+//
+// gfx950 mapping: v_mfma_f32_16x16x32_{bf16,f16}.  Lane l of an MFMA holds
+// A[l%16][8*(l/16) .. +7] and B[8*(l/16) .. +7][l%16]: the 32-deep k slice is
+// 32 consecutive input planes at one tap, so LDS holds both operands
+// planes-fastest ("channels last"): a lane's eight k values are one 16-byte
+// read.  A block (4 wave64) owns an 8 x 32 pixel tile and 16*MB output planes;
+// wave w owns pixel rows 2w, 2w+1.  Input planes are staged 32 at a time.
+//
+// Split-BF16: x = xh + xl with xh = bf16(x), xl = bf16(x - xh) (same for the
+// filter); x*w ~ xh*wh + xh*wl + xl*wh, each term on the MFMA, all into the
+// same FP32 accumulator.  BF16 has FP32's exponent range, so the split needs
+// no rescaling except for values below ~2^-118 whose low part underflows.
+#include "internal.h"
+
+namespace cogconv {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
+constexpr int LP_RC = 32;   // planes per staged chunk (one MFMA k slice)
+constexpr int LP_RCP = 40;  // LDS pitch of a staged pixel / filter row, in 16-bit elements
+
+template <int PREC>
+struct LpType;
+template <>
+struct LpType<COGCONV_BF16> {
+    typedef __bf16 T;
+    typedef bf16x8 V;
+};
+template <>
+struct LpType<COGCONV_SPLIT_BF16> {
+    typedef __bf16 T;
+    typedef bf16x8 V;
+};
+template <>
+struct LpType<COGCONV_FP16> {
+    typedef _Float16 T;
+    typedef halfx8 V;
+};
+
+template <int PREC>
+__device__ __forceinline__ floatx4 lp_mfma(typename LpType<PREC>::V a, typename LpType<PREC>::V b, floatx4 c) {
+    if constexpr (PREC == COGCONV_FP16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Filter pre-pack: Wp[term][t][m][r] in the MFMA input type, m padded to
+// Mpad, r padded to Rpad (zeros), taps flipped for the convolution
+// orientation.  term 0 = rounded value, term 1 (split only) = rounded residual.
+template <int PREC>
+__global__ void k_pack_filters_lp(Geom g, PlaneMap m, int Mout, int Mpad, int Rpad, const float* __restrict__ Wt,
+                                  typename LpType<PREC>::T* __restrict__ Wp) {
+    typedef typename LpType<PREC>::T T;
+    const int taps = g.FH * g.FW;
+    const long total = (long)taps * Mpad * Rpad;
+    const bool bpf = m.mode == COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM;
+    const int L = m.T / m.N;
+    for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(idx % Rpad);
+        const long rest = idx / Rpad;
+        const int mm = (int)(rest % Mpad);
+        const int t = (int)(rest / Mpad);
+        float w = 0.f;
+        if (r < m.R && mm < Mout) {
+            const int pf = bpf ? r * L + mm : mm * m.N + r;
+            w = Wt[(size_t)pf * taps + (g.flip ? taps - 1 - t : t)];
+        }
+        const T hi = (T)w;
+        Wp[idx] = hi;
+        if (PREC == COGCONV_SPLIT_BF16) Wp[total + idx] = (T)(w - (float)hi);
+    }
+}
+
+template <int F, int MB, int PREC>
+__global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c, const float* __restrict__ X,
+                                                  const typename LpType<PREC>::T* __restrict__ Wp,
+                                                  float* __restrict__ O) {
+    typedef typename LpType<PREC>::T T;
+    typedef typename LpType<PREC>::V V;
+    constexpr int TAPS = F * F;
+    constexpr int MT = 16 * MB;
+    constexpr bool SPLIT = PREC == COGCONV_SPLIT_BF16;
+    constexpr int TERMS = SPLIT ? 2 : 1;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lp_smem[];
+    const int TRp = c.tileRows, TCp = c.tileCols;
+    const int npix = TRp * TCp;
+    T* sB = reinterpret_cast<T*>(lp_smem);                    // [TERMS][npix][RCP]
+    T* sA = sB + (size_t)TERMS * npix * LP_RCP;               // [TERMS][TAPS][MT][RCP]
+
+    int bid = blockIdx.x;
+    const int tc = bid % c.tilesC;
+    bid /= c.tilesC;
+    const int tr = bid % c.tilesR;
+    bid /= c.tilesR;
+    const int mt = bid % c.mTiles;
+    const int b = bid / c.mTiles;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g4 = lane >> 4, l16 = lane & 15;
+    const int Mout = c.Mout;
+    const int m0 = mt * MT;
+    const int y0 = tr * 8 * g.dnR - g.hR;
+    const int x0 = tc * 32 * g.dnC - g.hC;
+    const int HW = g.H * g.W;
+    const float* Xb = X + (size_t)b * m.N * HW;
+    const int Mpad = c.mTiles * MT;
+    const int Rpad = c.RC;  // planes padded to a multiple of 32
+    const long wTerm = (long)TAPS * Mpad * Rpad;
+
+    // Per-lane column offset of the staged tile (TCp <= 64) with the border applied.
+    int colOff;
+    bool colIn;
+    {
+        int x = x0 + lane;
+        bool in = (unsigned)x < (unsigned)g.W;
+        if (g.border == COGCONV_BORDER_CYCLIC) {
+            x %= g.W;
+            x += x < 0 ? g.W : 0;
+            in = true;
+        } else {
+            in = in || g.border == COGCONV_BORDER_CLAMP;
+            x = min(max(x, 0), g.W - 1);
+        }
+        colOff = x;
+        colIn = in && lane < TCp;
+    }
+
+    floatx4 acc[MB][4];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    int boff[4];
+#pragma unroll
+    for (int pb = 0; pb < 4; pb++) {
+        const int row = 2 * wave + (pb >> 1);
+        const int col = (pb & 1) * 16 + l16;
+        boff[pb] = ((row * g.dnR) * TCp + col * g.dnC) * LP_RCP + 8 * g4;
+    }
+
+    for (int r0 = 0; r0 < m.R; r0 += LP_RC) {
+        __syncthreads();
+        // Input tile: items (8-plane group, row); each lane loads the eight
+        // planes of its column, rounds them and writes one 16-byte LDS vector.
+        const int nitems = 4 * TRp;
+        constexpr int IU = 4;
+        for (int base = wave; base < nitems; base += 4 * IU) {
+            float v[IU][8];
+#pragma unroll
+            for (int u = 0; u < IU; u++) {
+                const int item = base + 4 * u;
+                const int grp = item / TRp;  // wave-uniform
+                const int ty = item - grp * TRp;
+                int y = y0 + ty;
+                bool ym = (unsigned)y < (unsigned)g.H;
+                if (g.border == COGCONV_BORDER_CYCLIC) {
+                    y %= g.H;
+                    y += y < 0 ? g.H : 0;
+                    ym = true;
+                } else {
+                    ym = ym || g.border == COGCONV_BORDER_CLAMP;
+                    y = min(max(y, 0), g.H - 1);
+                }
+                const bool on = item < nitems && ym && colIn;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int r = r0 + 8 * grp + e;
+                    const float a = Xb[min(r, m.R - 1) * HW + y * g.W + colOff];
+                    v[u][e] = (on && r < m.R) ? a : 0.f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < IU; u++) {
+                const int item = base + 4 * u;
+                if (item < nitems && lane < TCp) {
+                    const int grp = item / TRp;
+                    const int ty = item - grp * TRp;
+                    V hv, lv;
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const T h = (T)v[u][e];
+                        hv[e] = h;
+                        if (SPLIT) lv[e] = (T)(v[u][e] - (float)h);
+                    }
+                    const size_t o = (size_t)(ty * TCp + lane) * LP_RCP + 8 * grp;
+                    *reinterpret_cast<V*>(sB + o) = hv;
+                    if (SPLIT) *reinterpret_cast<V*>(sB + (size_t)npix * LP_RCP + o) = lv;
+                }
+            }
+        }
+        // Filter slice from the pre-packed bank: 16-byte units (term, t, m, q).
+        const int units = TERMS * TAPS * MT * 4;
+        for (int q = tid; q < units; q += 256) {
+            const int qq = q & 3;
+            const int rest = q >> 2;
+            const int mm = rest % MT;
+            const int rest2 = rest / MT;
+            const int t = rest2 % TAPS;
+            const int term = rest2 / TAPS;
+            const V val = *reinterpret_cast<const V*>(Wp + term * wTerm + ((long)t * Mpad + m0 + mm) * Rpad + r0 +
+                                                      8 * qq);
+            *reinterpret_cast<V*>(sA + ((size_t)(term * TAPS + t) * MT + mm) * LP_RCP + 8 * qq) = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ti = 0; ti < F; ti++) {
+#pragma unroll
+            for (int tj = 0; tj < F; tj++) {
+                const int t = ti * F + tj;
+                const int toff = (ti * TCp + tj) * LP_RCP;
+                V av[MB], bv[4];
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++)
+                    av[mb] = *reinterpret_cast<const V*>(sA + ((size_t)t * MT + mb * 16 + l16) * LP_RCP + 8 * g4);
+#pragma unroll
+                for (int pb = 0; pb < 4; pb++) bv[pb] = *reinterpret_cast<const V*>(sB + boff[pb] + toff);
+                if (SPLIT) {
+                    V al[MB], bl[4];
+#pragma unroll
+                    for (int mb = 0; mb < MB; mb++)
+                        al[mb] = *reinterpret_cast<const V*>(sA + ((size_t)(TAPS + t) * MT + mb * 16 + l16) * LP_RCP +
+                                                             8 * g4);
+#pragma unroll
+                    for (int pb = 0; pb < 4; pb++)
+                        bl[pb] = *reinterpret_cast<const V*>(sB + (size_t)npix * LP_RCP + boff[pb] + toff);
+#pragma unroll
+                    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                        for (int pb = 0; pb < 4; pb++) {
+                            acc[mb][pb] = lp_mfma<PREC>(al[mb], bv[pb], acc[mb][pb]);
+                            acc[mb][pb] = lp_mfma<PREC>(av[mb], bl[pb], acc[mb][pb]);
+                        }
+                }
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                    for (int pb = 0; pb < 4; pb++) acc[mb][pb] = lp_mfma<PREC>(av[mb], bv[pb], acc[mb][pb]);
+            }
+        }
+    }
+    // Epilogue: LDS output tile [MT][8][32] (plane pitch 260 floats), then
+    // coalesced row stores.  D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4.
+    __syncthreads();
+    constexpr int OP = 260;
+    float* sO = reinterpret_cast<float*>(lp_smem);
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+        for (int pb = 0; pb < 4; pb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                sO[(mb * 16 + 4 * g4 + v) * OP + (2 * wave + (pb >> 1)) * 32 + (pb & 1) * 16 + l16] = acc[mb][pb][v];
+    __syncthreads();
+    const int col = tid & 31;
+    const int ocol = tc * 32 + col;
+    for (int item = tid >> 5; item < MT * 8; item += 8) {
+        const int mm = item >> 3, row = item & 7;
+        const int orow = tr * 8 + row;
+        if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
+            O[((size_t)(b * Mout + m0 + mm) * g.OH + orow) * g.OW + ocol] = sO[mm * OP + row * 32 + col];
+    }
+}
+
+size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision) {
+    const int terms = precision == COGCONV_SPLIT_BF16 ? 2 : 1;
+    const size_t stage = (size_t)terms * ((size_t)c.tileRows * c.tileCols + (size_t)F * F * 16 * c.MB) * LP_RCP * 2;
+    const size_t epi = (size_t)16 * c.MB * 260 * sizeof(float);
+    return stage > epi ? stage : epi;
+}
+
+size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision) {
+    const int terms = precision == COGCONV_SPLIT_BF16 ? 2 : 1;
+    return (size_t)terms * F * F * (size_t)c.mTiles * 16 * c.MB * c.RC * 2;
+}
+
+template <int F, int MB, int PREC>
+static hipError_t launch_lp3(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
+                             float* out, void* ws, hipStream_t s) {
+    typedef typename LpType<PREC>::T T;
+    T* wp = static_cast<T*>(ws);
+    const long packed = (long)F * F * c.mTiles * 16 * MB * c.RC;
+    const long pblocks = std::min((packed + 255) / 256, 4096L);
+    hipLaunchKernelGGL((k_pack_filters_lp<PREC>), dim3((unsigned)pblocks), dim3(256), 0, s, g, m, c.Mout,
+                       c.mTiles * 16 * MB, c.RC, w, wp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const long blocks = (long)c.tilesC * c.tilesR * c.mTiles * m.B;
+    hipLaunchKernelGGL((k_igemm_lp<F, MB, PREC>), dim3((unsigned)blocks), dim3(256), c.lds_bytes, s, g, m, c, x, wp,
+                       out);
+    return hipGetLastError();
+}
+
+template <int F, int PREC>
+static hipError_t launch_lp2(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
+                             float* out, void* ws, hipStream_t s) {
+    switch (c.MB) {
+    case 1: return launch_lp3<F, 1, PREC>(g, m, c, x, w, out, ws, s);
+    case 2: return launch_lp3<F, 2, PREC>(g, m, c, x, w, out, ws, s);
+    default: return launch_lp3<F, 4, PREC>(g, m, c, x, w, out, ws, s);
+    }
+}
+
+template <int PREC>
+static hipError_t launch_lp1(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
+                             float* out, void* ws, hipStream_t s) {
+    switch (g.FH) {
+    case 1: return launch_lp2<1, PREC>(g, m, c, x, w, out, ws, s);
+    case 3: return launch_lp2<3, PREC>(g, m, c, x, w, out, ws, s);
+    case 5: return launch_lp2<5, PREC>(g, m, c, x, w, out, ws, s);
+    case 7: return launch_lp2<7, PREC>(g, m, c, x, w, out, ws, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+bool igemm_lp_supported(int F) { return F == 1 || F == 3 || F == 5 || F == 7; }
+
+hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x,
+                           const float* w, float* out, void* ws, hipStream_t s) {
+    const long blocks = (long)c.tilesC * c.tilesR * c.mTiles * m.B;
+    if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    switch (precision) {
+    case COGCONV_BF16: return launch_lp1<COGCONV_BF16>(g, m, c, x, w, out, ws, s);
+    case COGCONV_FP16: return launch_lp1<COGCONV_FP16>(g, m, c, x, w, out, ws, s);
+    case COGCONV_SPLIT_BF16: return launch_lp1<COGCONV_SPLIT_BF16>(g, m, c, x, w, out, ws, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace cogconv

@@ -107,6 +107,7 @@ enum KernelKind {
     KK_SMALL_OUTPUT = 2,
     KK_IGEMM = 3,
     KK_FA_IGEMM = 4,
+    KK_IGEMM_LP = 5,
 };
 
 // LDS-tiled direct kernel configuration.
@@ -171,6 +172,11 @@ hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c,
                             float* out, hipStream_t s);
 hipError_t launch_fa_igemm_f32(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
                                float* out, float* ws, hipStream_t s);
+bool igemm_lp_supported(int F);
+size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision);
+size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision);
+hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x,
+                           const float* w, float* out, void* ws, hipStream_t s);
 hipError_t launch_block_reduce_sum(const float* in, float* out, int64_t points, int out_planes, int factor,
                                    hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);

@@ -306,6 +306,36 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     return true;
 }
 
+// BF16 / FP16 / split-BF16 MFMA implicit GEMM for the frame block-reduce modes.
+bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, IgemmCfg* out) {
+    if (d->mode != COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM && d->mode != COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM)
+        return false;
+    if (d->in0.dims != 2 || g.FH != g.FW || !igemm_lp_supported(g.FH) || g.upAr != 1 || g.upAc != 1 ||
+        g.dnR != g.dnC)
+        return false;
+    if (31 * g.dnC + g.FW > 64) return false;  // one 64-lane pass per staged row
+    if ((long)d->in0.planes * g.H * g.W >= (1L << 31)) return false;
+    IgemmCfg c;
+    std::memset(&c, 0, sizeof c);
+    c.Mout = m.T / m.N;
+    c.tilesR = (g.OH + 7) / 8;
+    c.tilesC = (g.OW + 31) / 32;
+    c.tileRows = 7 * g.dnR + g.FH;
+    c.tileCols = 31 * g.dnC + g.FW;
+    c.RC = (m.R + 31) / 32 * 32;  // planes padded to whole 32-deep MFMA slices
+    for (int MB = c.Mout <= 16 ? 1 : (c.Mout <= 32 ? 2 : 4); MB >= 1; MB = MB == 4 ? 2 : MB - 1) {
+        c.MB = MB;
+        c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
+        c.lds_bytes = igemm_lp_lds_bytes(c, g.FH, d->precision);
+        if (c.lds_bytes <= 160 * 1024) {
+            *out = c;
+            return true;
+        }
+        if (MB == 1) break;
+    }
+    return false;
+}
+
 SmallCfg choose_small(const Geom& g, const PlaneMap& m) {
     SmallCfg c;
     c.qchunks = (g.OH * g.OW + 15) / 16;
@@ -334,9 +364,13 @@ extern "C" int cogconv_plan_create(const cogconv_desc* desc, cogconv_plan* plan)
     PlaneMap m;
     int e = analyse(desc, &out, &g, &m);
     if (e) return e;
-    if (desc->precision != COGCONV_FP32)
-        return fail(COGCONV_ERR_UNSUPPORTED, "precision %d is not offered for %s in this build", desc->precision,
-                    mode_name(desc->mode));
+    IgemmCfg lp;
+    if (desc->precision != COGCONV_FP32 && !choose_igemm_lp(desc, g, m, &lp))
+        return fail(COGCONV_ERR_UNSUPPORTED,
+                    "precision %d is offered for ProjectFrame/BackProjectFrame with block reduction on 2-D fields with "
+                    "a square 1/3/5/7 filter, no input up-sampling and a 32-column tile that fits one wavefront; "
+                    "not for this %s",
+                    desc->precision, mode_name(desc->mode));
     cogconv_plan p = new (std::nothrow) cogconv_plan_s;
     if (!p) return fail(COGCONV_ERR_OOM, "host allocation failed");
     std::memset(p, 0, sizeof *p);
@@ -353,15 +387,23 @@ extern "C" int cogconv_plan_create(const cogconv_desc* desc, cogconv_plan* plan)
         delete p;
         return fail(COGCONV_ERR_UNSUPPORTED, "small-output kernel needs an output plane of at most 4096 points");
     }
+    if (desc->precision != COGCONV_FP32) {
+        p->kind = KK_IGEMM_LP;
+        p->icfg = lp;
+        p->workspace = igemm_lp_workspace_bytes(lp, g.FH, desc->precision);
+    }
     const bool igemm_ok = igemm_frame_eligible(desc, g) && choose_igemm_frame(g, m).MB > 0;
     FaCfg probe_fa;
-    if (algo == COGCONV_ALGO_IMPLICIT_GEMM && !igemm_ok && !choose_fa_igemm(desc, g, m, &probe_fa)) {
+    if (p->kind != KK_IGEMM_LP && algo == COGCONV_ALGO_IMPLICIT_GEMM && !igemm_ok &&
+        !choose_fa_igemm(desc, g, m, &probe_fa)) {
         delete p;
         return fail(COGCONV_ERR_UNSUPPORTED, "implicit GEMM needs a block-reduce frame mode with a square 1/3/5/7/9 "
                                              "filter on 2-D fields and no input up-sampling, or FilterAdjoint with "
                                              "block reduction, BorderValid and at most 64 planes per image in dy");
     }
-    if ((algo == COGCONV_ALGO_AUTO || algo == COGCONV_ALGO_IMPLICIT_GEMM) && igemm_ok) {
+    if (p->kind == KK_IGEMM_LP) {
+        // precision variant chosen above
+    } else if ((algo == COGCONV_ALGO_AUTO || algo == COGCONV_ALGO_IMPLICIT_GEMM) && igemm_ok) {
         p->kind = KK_IGEMM;
         p->icfg = choose_igemm_frame(g, m);
         p->workspace = 0;
@@ -437,6 +479,12 @@ extern "C" int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len) {
         snprintf(buf, len, "small_output %s out=%dx%dx%d splits=%d", mode_name(plan->desc.mode), plan->m.nOut, g.OH,
                  g.OW, plan->scfg.splits);
         break;
+    case KK_IGEMM_LP: {
+        static const char* pn[] = {"fp32", "bf16", "fp16", "split_bf16"};
+        snprintf(buf, len, "igemm_%s %s out=%dx%dx%d MB=%d tile=8x32 threads=256 lds=%zu", pn[plan->desc.precision],
+                 mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->icfg.MB, plan->icfg.lds_bytes);
+        break;
+    }
     case KK_FA_IGEMM:
         snprintf(buf, len, "fa_igemm_f32 %s out=%dx%dx%d MB=%d CBW=%d RR=%d groups=%d splits=%d lds=%zu",
                  mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->fcfg.MB, plan->fcfg.CBW, plan->fcfg.RR,
@@ -470,6 +518,9 @@ extern "C" int cogconv_launch(cogconv_plan plan, const float* in0, const float* 
         break;
     case KK_FA_IGEMM:
         e = launch_fa_igemm_f32(plan->g, plan->m, plan->fcfg, in0, in1, out, static_cast<float*>(workspace), s);
+        break;
+    case KK_IGEMM_LP:
+        e = launch_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->icfg, in0, in1, out, workspace, s);
         break;
     default:
         return fail(COGCONV_ERR_RUNTIME, "internal: unknown kernel kind %d", plan->kind);

@@ -1,0 +1,88 @@
+"""Precision variants (BF16 / FP16 MFMA, split-BF16) against the FP64 oracle.
+
+Tolerances, normwise (max |gpu - oracle_f64| / max |oracle_f64|):
+  BF16 and FP16 implicit GEMM: <= 1e-2
+  split-BF16 (three-term):     <= 1e-5
+on uniform[-1,1) inputs and on wide-range inputs |v| = 10^U(-3,3) with random
+signs.  All borders, both block-reduce frame modes, 1/3/5/7 filters.
+"""
+import numpy as np
+import pytest
+
+import cogconv._lib as L
+import oracle as O
+from gpuhelp import desc_pair, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+TOL = {L.BF16: 1e-2, L.FP16: 1e-2, L.SPLIT_BF16: 1e-5}
+NAMES = {L.BF16: "bf16", L.FP16: "fp16", L.SPLIT_BF16: "split_bf16"}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    L.init(0)
+    yield
+    L.synchronize()
+
+
+def inputs(rng, shape, wide):
+    if not wide:
+        return rng.uniform(-1, 1, shape).astype(np.float32)
+    mag = 10.0 ** rng.uniform(-3, 3, shape)
+    return (mag * rng.choice([-1.0, 1.0], shape)).astype(np.float32)
+
+
+def normwise(got, ref):
+    return float(np.abs(got.astype(np.float64) - ref).max() / np.abs(ref).max())
+
+
+CASES = []
+for mode in (O.PF_BRS, O.BPF_BRS):
+    for border in (O.ZERO, O.CLAMP, O.CYCLIC, O.VALID, O.FULL):
+        for F in (3, 1, 5):
+            CASES.append((mode, border, F))
+
+
+@pytest.mark.parametrize("precision", (L.BF16, L.FP16, L.SPLIT_BF16))
+@pytest.mark.parametrize("mode,border,F", CASES)
+def test_precision_variant_accuracy(precision, mode, border, F):
+    rng = np.random.default_rng(1000 * precision + 10 * mode + border + F)
+    B, N, K, H, W = 2, 40, 24, 19, 37  # N not a multiple of 32, ragged tiles
+    for wide in (False, True):
+        a = inputs(rng, (B * N, H, W), wide)
+        k = inputs(rng, (K * N, F, F), wide)
+        orient = O.CONV if mode == O.BPF_BRS else O.XCORR
+        od, cd = desc_pair(mode, border, orient, O.NONE, 1, B, a.shape, k.shape, precision=precision)
+        print("CASE precision", NAMES[precision], mode, border, F, "wide" if wide else "uniform", flush=True)
+        got = run_gpu(cd, a, k)
+        ref = O.conv(od, a, k)
+        err = normwise(got, ref)
+        print("  normwise error %.3g" % err)
+        assert got.shape == ref.shape
+        assert err <= TOL[precision], (NAMES[precision], wide, err)
+
+
+def test_precision_c3_shape_uniform():
+    """B32 128x128x64 planes, 64 filters 3x3x64, BorderZero on a 4-image slice."""
+    rng = np.random.default_rng(33)
+    B, N, K = 4, 64, 64
+    x = rng.uniform(-1, 1, (B * N, 128, 128)).astype(np.float32)
+    w = rng.uniform(-1, 1, (K * N, 3, 3)).astype(np.float32)
+    od = O.make_desc(O.PF_BRS, O.ZERO, O.XCORR, O.NONE, 1, B, x.shape, w.shape)
+    ref = O.conv(od, x, w, threads=16)
+    for precision in (L.BF16, L.FP16, L.SPLIT_BF16):
+        _, cd = desc_pair(O.PF_BRS, O.ZERO, O.XCORR, O.NONE, 1, B, x.shape, w.shape, precision=precision)
+        print("CASE c3 slice", NAMES[precision], flush=True)
+        got = run_gpu(cd, x, w)
+        err = normwise(got, ref)
+        print("  normwise error %.3g" % err)
+        assert err <= TOL[precision]
+
+
+def test_precision_rejects_unsupported_modes():
+    """Precision variants exist for the contraction modes only; others fail at plan time."""
+    _, cd = desc_pair(O.PBP, O.ZERO, O.XCORR, O.NONE, 1, 1, (1, 16, 16), (1, 3, 3), precision=L.BF16)
+    with pytest.raises(L.CogconvError) as e:
+        L.Plan(cd)
+    assert e.value.status == L.ERR_UNSUPPORTED
