@@ -60,6 +60,27 @@ class DevicePropsC(ctypes.Structure):
                 ("hbm_bytes", ctypes.c_uint64)]
 
 
+class ServerConfigC(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int32),
+                ("streams", ctypes.c_int32), ("max_stack", ctypes.c_int32)]
+
+
+class RequestC(ctypes.Structure):
+    _fields_ = [("desc", DescC), ("bank", ctypes.c_int32), ("flags", ctypes.c_int32), ("host_in0", ctypes.c_void_p),
+                ("host_in1", ctypes.c_void_p), ("host_out", ctypes.c_void_p)]
+
+
+REQUEST_RESIDENT = 1
+
+
+class ServerStatsC(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("launches", ctypes.c_uint64), ("stacked", ctypes.c_uint64), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("first_submit_s", ctypes.c_double), ("last_complete_s", ctypes.c_double),
+                ("requests_per_s", ctypes.c_double), ("gflop", ctypes.c_double), ("peak_bytes", ctypes.c_uint64)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _SZ = ctypes.c_size_t
@@ -103,6 +124,17 @@ SIGNATURES = {
     "cogconv_plan_describe": [_P, ctypes.c_char_p, _SZ],
     "cogconv_launch": [_P, _P, _P, _P, _P, _P],
     "cogconv_block_reduce_sum": [_P, _P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _P],
+    "cogconv_server_create": [ctypes.POINTER(ServerConfigC), ctypes.POINTER(_P)],
+    "cogconv_server_destroy": [_P],
+    "cogconv_server_register_bank": [_P, _P, ctypes.POINTER(FieldTypeC), ctypes.POINTER(ctypes.c_int32)],
+    "cogconv_server_submit": [_P, ctypes.POINTER(RequestC), ctypes.POINTER(ctypes.c_uint64)],
+    "cogconv_server_query": [_P, ctypes.c_uint64],
+    "cogconv_server_submit_timed": [_P, ctypes.POINTER(RequestC), ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
+                                    ctypes.POINTER(ctypes.c_uint64)],
+    "cogconv_server_wait": [_P, ctypes.c_uint64],
+    "cogconv_server_drain": [_P],
+    "cogconv_server_stats_get": [_P, ctypes.POINTER(ServerStatsC)],
+    "cogconv_server_stats_reset": [_P],
 }
 _RESTYPES = {"cogconv_last_error": ctypes.c_char_p}
 
@@ -328,3 +360,126 @@ def block_reduce_sum(inp, out, points, in_planes, factor, stream=None):
         return x.ptr if isinstance(x, DeviceBuffer) else x
     s = stream.handle if isinstance(stream, Stream) else stream
     check(lib().cogconv_block_reduce_sum(ptr(inp), ptr(out), points, in_planes, factor, s))
+
+
+class HostBuffer:
+    """Pinned host memory viewed as a float32 numpy array."""
+
+    def __init__(self, shape):
+        self.shape = tuple(shape)
+        n = int(np.prod(self.shape)) * 4
+        p = ctypes.c_void_p()
+        check(lib().cogconv_host_alloc(n, ctypes.byref(p)))
+        self.ptr = p.value
+        self.array = np.ctypeslib.as_array((ctypes.c_float * (n // 4)).from_address(self.ptr)).reshape(self.shape)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.array = None
+            lib().cogconv_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop of libcogconv (cogconv_server_*)."""
+
+    def __init__(self, budget_bytes=1 << 30, max_inflight=8, streams=4, max_stack=8, device=0):
+        cfg = ServerConfigC(device, budget_bytes, max_inflight, streams, max_stack)
+        p = ctypes.c_void_p()
+        check(lib().cogconv_server_create(ctypes.byref(cfg), ctypes.byref(p)))
+        self.handle = p.value
+        self._keep = {}
+
+    def register_bank(self, weights):
+        """weights: planar (planes, rows, cols) float32; returns the bank id."""
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        ft = FieldTypeC(2, w.shape[1], w.shape[2], w.shape[0], 0 if w.shape[0] == 1 else 1)
+        bid = ctypes.c_int32()
+        check(lib().cogconv_server_register_bank(self.handle, w.ctypes.data, ctypes.byref(ft), ctypes.byref(bid)))
+        return bid.value
+
+    def submit(self, desc, in0, out, bank=-1, in1=None):
+        """in0 / in1 / out: numpy float32 arrays (pinned HostBuffer arrays for overlap),
+        or DeviceBuffers / device addresses (then no copies: the request's state is
+        resident in HBM); they must stay alive until the ticket completes."""
+        resident = isinstance(in0, (DeviceBuffer, int))
+
+        def addr(x):
+            if x is None:
+                return None
+            if isinstance(x, DeviceBuffer):
+                return x.ptr
+            if isinstance(x, int):
+                return x
+            return x.ctypes.data
+        req = RequestC(desc, bank, REQUEST_RESIDENT if resident else 0, addr(in0), addr(in1), addr(out))
+        t = ctypes.c_uint64()
+        check(lib().cogconv_server_submit(self.handle, ctypes.byref(req), ctypes.byref(t)))
+        self._keep[t.value] = (in0, in1, out)
+        return t.value
+
+    def submit_timed(self, requests, arrivals):
+        """requests: list of RequestC (built by request()); arrivals: seconds from now.
+        Submits natively at those times; returns the tickets (0 = refused)."""
+        n = len(requests)
+        arr = (RequestC * n)(*requests)
+        ts = (ctypes.c_double * n)(*[float(a) for a in arrivals])
+        tk = (ctypes.c_uint64 * n)()
+        st = lib().cogconv_server_submit_timed(self.handle, arr, ts, n, tk)
+        if st not in (OK, ERR_BUDGET):
+            check(st)
+        return list(tk)
+
+    @staticmethod
+    def request(desc, in0, out, bank=-1, in1=None, resident=False):
+        """A RequestC for submit_timed; buffers are addresses (ints) or numpy arrays."""
+        def addr(x):
+            if x is None:
+                return None
+            if isinstance(x, DeviceBuffer):
+                return x.ptr
+            if isinstance(x, int):
+                return x
+            return x.ctypes.data
+        return RequestC(desc, bank, REQUEST_RESIDENT if resident else 0, addr(in0), addr(in1), addr(out))
+
+    def done(self, ticket):
+        s = lib().cogconv_server_query(self.handle, ticket)
+        if s == ERR_NOT_READY:
+            return False
+        check(s)
+        self._keep.pop(ticket, None)
+        return True
+
+    def wait(self, ticket):
+        check(lib().cogconv_server_wait(self.handle, ticket))
+        self._keep.pop(ticket, None)
+
+    def drain(self):
+        check(lib().cogconv_server_drain(self.handle))
+        self._keep.clear()
+
+    def stats(self):
+        st = ServerStatsC()
+        check(lib().cogconv_server_stats_get(self.handle, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in ServerStatsC._fields_}
+
+    def reset_stats(self):
+        check(lib().cogconv_server_stats_reset(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().cogconv_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

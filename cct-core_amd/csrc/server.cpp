@@ -1,0 +1,626 @@
+// Serving loop: continuous admission of convolution requests on one GPU.
+//
+// The reference specifies a reentrant, thread-safe evaluate() Function Interface
+// (doc/CogFunctionInterfaceERS.docx; src/main/java/cogjni/CogJNIWrapper.java:85-115)
+// and shares device buffers under a budget (Cog.bufferSharing,
+// src/main/scala/cogx/parameters/Cog.scala:83), but has no request scheduler.
+// This server:
+//   - keeps filter banks resident in HBM (registered once, named by id);
+//   - preallocates its whole device budget once and sub-allocates request buffers
+//     from it (first fit, 256-byte granules), so the loop never calls hipMalloc;
+//   - admits queued requests in arrival order whenever a launch slot and enough
+//     budget are free (a request that arrives while others run is admitted as soon
+//     as they release memory: continuous admission);
+//   - stacks queued requests of one shape bucket (same op, same bank, same field
+//     shape) into a single launch when the vector mode keeps images independent
+//     (ProjectFrame / BackProjectFrame, with or without block reduction: batch is
+//     folded into planes, so stacking is concatenation of planes);
+//   - spreads launches over several streams: copies and kernels of different
+//     launches overlap on the device;
+//   - records submit -> result-ready latency per request (result in host memory,
+//     or in HBM for requests whose state is resident there).
+// Kernels and their summation order do not depend on how many images share a
+// launch, so a request's result is the same alone and in the mix.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "internal.h"
+
+using namespace cogconv;
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over one device region.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+    size_t in_use = 0, peak = 0;
+
+    static size_t round(size_t b) { return (b + 255) & ~size_t(255); }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = round(bytes ? bytes : 1);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes;
+                const size_t at = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[at] = rest;
+                in_use += bytes;
+                peak = std::max(peak, in_use);
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = round(bytes ? bytes : 1);
+        in_use -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct Bank {
+    float* dptr;
+    cogconv_field_type type;
+};
+
+struct Req {
+    uint64_t ticket;
+    cogconv_request r;
+    cogconv_field_type out;
+    size_t in0_bytes, in1_bytes, out_bytes;
+    double flops;
+    double t_submit = 0, t_done = 0;
+    bool done = false;
+    int status = COGCONV_OK;
+    std::string error;
+};
+
+struct Launch {
+    std::vector<Req*> reqs;
+    int stream;
+    hipEvent_t ev;
+    std::vector<std::pair<size_t, size_t>> blocks;  // arena (offset, bytes)
+};
+
+bool stackable(int mode) {
+    return mode == COGCONV_PROJECT_FRAME || mode == COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM ||
+           mode == COGCONV_BACK_PROJECT_FRAME || mode == COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM;
+}
+
+// Shape bucket: everything but the batch count.
+std::string bucket_key(const Req* q) {
+    cogconv_desc d = q->r.desc;
+    const int per_image = d.in0.planes / d.batch;
+    d.in0.planes = per_image;
+    d.batch = 1;
+    std::string k(reinterpret_cast<const char*>(&d), sizeof d);
+    k.append(reinterpret_cast<const char*>(&q->r.bank), sizeof q->r.bank);
+    return k;
+}
+
+}  // namespace
+
+struct cogconv_server_s {
+    cogconv_server_config cfg;
+    Arena arena;
+    std::vector<hipStream_t> streams;
+    std::vector<Bank> banks;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req*> pending;
+    std::vector<Launch*> inflight;
+    std::map<uint64_t, Req*> reqs;
+    std::map<std::string, cogconv_plan> plans;
+    std::map<std::string, std::pair<size_t, double>> plan_info;  // desc -> (workspace, flops), for submit
+    std::vector<hipEvent_t> event_pool;
+    std::thread worker;
+    bool stop = false;
+    uint64_t next_ticket = 1;
+    int rr = 0;
+    // statistics
+    std::vector<double> lat_ms;
+    uint64_t submitted = 0, completed = 0, rejected = 0, launches = 0, stacked = 0;
+    double first_submit = 0, last_complete = 0, gflop = 0;
+
+    cogconv_plan plan_for(const cogconv_desc& d, int* err) {
+        std::string k(reinterpret_cast<const char*>(&d), sizeof d);
+        auto it = plans.find(k);
+        if (it != plans.end()) return it->second;
+        cogconv_plan p = nullptr;
+        *err = cogconv_plan_create(&d, &p);
+        if (*err) return nullptr;
+        plans[k] = p;
+        return p;
+    }
+
+    hipError_t take_event(hipEvent_t* ev) {
+        if (!event_pool.empty()) {
+            *ev = event_pool.back();
+            event_pool.pop_back();
+            return hipSuccess;
+        }
+        return hipEventCreateWithFlags(ev, hipEventDisableTiming);
+    }
+
+    void fail_reqs(const std::vector<Req*>& g, int status, const std::string& msg) {
+        for (Req* q : g) {
+            q->status = status;
+            q->error = msg;
+            q->done = true;
+            q->t_done = now_s();
+        }
+        cv_done.notify_all();
+    }
+
+    // Issue one launch for the requests in g (one bucket).  Called with mu held.
+    // Returns false when the arena cannot hold it now (nothing issued).
+    bool issue(const std::vector<Req*>& g) {
+        const Req* h = g.front();
+        cogconv_desc d = h->r.desc;
+        int B = 0, P0 = 0;
+        size_t in0 = 0, outb = 0;
+        for (Req* q : g) {
+            B += q->r.desc.batch;
+            P0 += q->r.desc.in0.planes;
+            in0 += q->in0_bytes;
+            outb += q->out_bytes;
+        }
+        d.batch = B;
+        d.in0.planes = P0;
+        int err = 0;
+        cogconv_plan plan = plan_for(d, &err);
+        if (!plan) {
+            fail_reqs(g, err, cogconv_last_error());
+            return true;
+        }
+        size_t ws = 0;
+        cogconv_plan_workspace_bytes(plan, &ws);
+        const bool resident = (h->r.flags & COGCONV_REQUEST_RESIDENT) != 0;
+        if (resident) {
+            // State is already in HBM: only the workspace comes from the budget.
+            size_t o_ws = 0;
+            if (ws && !arena.alloc(ws, &o_ws)) return false;
+            std::vector<std::pair<size_t, size_t>> blocks;
+            if (ws) blocks.push_back({o_ws, ws});
+            const int si = rr++ % (int)streams.size();
+            hipStream_t s = streams[si];
+            const float* in1 = h->r.bank < 0 ? h->r.host_in1 : banks[h->r.bank].dptr;
+            int st = cogconv_launch(plan, h->r.host_in0, in1, h->r.host_out, ws ? arena.base + o_ws : nullptr,
+                                    reinterpret_cast<cogconv_stream>(s));
+            hipEvent_t ev = nullptr;
+            hipError_t e = hipSuccess;
+            if (st == COGCONV_OK) e = take_event(&ev);
+            if (st == COGCONV_OK && e == hipSuccess) e = hipEventRecord(ev, s);
+            if (st != COGCONV_OK || e != hipSuccess) {
+                (void)hipStreamSynchronize(s);
+                for (auto& b : blocks) arena.release(b.first, b.second);
+                if (ev) event_pool.push_back(ev);
+                fail_reqs(g, st ? st : COGCONV_ERR_RUNTIME, st ? cogconv_last_error() : hipGetErrorString(e));
+                return true;
+            }
+            inflight.push_back(new Launch{g, si, ev, blocks});
+            launches++;
+            return true;
+        }
+        const bool sendIn1 = h->r.bank < 0;
+        std::vector<std::pair<size_t, size_t>> blocks;
+        auto take = [&](size_t bytes, size_t* off) {
+            if (!arena.alloc(bytes, off)) return false;
+            blocks.push_back({*off, bytes});
+            return true;
+        };
+        size_t o_in0 = 0, o_out = 0, o_ws = 0, o_in1 = 0;
+        if (!take(in0, &o_in0) || !take(outb, &o_out) || (ws && !take(ws, &o_ws)) ||
+            (sendIn1 && !take(h->in1_bytes, &o_in1))) {
+            for (auto& b : blocks) arena.release(b.first, b.second);
+            return false;
+        }
+        const int si = rr++ % (int)streams.size();
+        hipStream_t s = streams[si];
+        char* base = arena.base;
+        size_t at = 0;
+        hipError_t e = hipSuccess;
+        for (Req* q : g) {
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(base + o_in0 + at, q->r.host_in0, q->in0_bytes, hipMemcpyHostToDevice, s);
+            at += q->in0_bytes;
+        }
+        const float* in1 = sendIn1 ? reinterpret_cast<const float*>(base + o_in1) : banks[h->r.bank].dptr;
+        if (e == hipSuccess && sendIn1)
+            e = hipMemcpyAsync(base + o_in1, h->r.host_in1, h->in1_bytes, hipMemcpyHostToDevice, s);
+        int st = COGCONV_OK;
+        if (e == hipSuccess)
+            st = cogconv_launch(plan, reinterpret_cast<const float*>(base + o_in0), in1,
+                                reinterpret_cast<float*>(base + o_out), ws ? base + o_ws : nullptr,
+                                reinterpret_cast<cogconv_stream>(s));
+        at = 0;
+        for (Req* q : g) {
+            if (e == hipSuccess && st == COGCONV_OK)
+                e = hipMemcpyAsync(q->r.host_out, base + o_out + at, q->out_bytes, hipMemcpyDeviceToHost, s);
+            at += q->out_bytes;
+        }
+        hipEvent_t ev = nullptr;
+        if (e == hipSuccess && st == COGCONV_OK) e = take_event(&ev);
+        if (e == hipSuccess && st == COGCONV_OK) e = hipEventRecord(ev, s);
+        if (e != hipSuccess || st != COGCONV_OK) {
+            (void)hipStreamSynchronize(s);
+            for (auto& b : blocks) arena.release(b.first, b.second);
+            if (ev) event_pool.push_back(ev);
+            fail_reqs(g, st ? st : COGCONV_ERR_RUNTIME, st ? cogconv_last_error() : hipGetErrorString(e));
+            return true;
+        }
+        Launch* L = new Launch{g, si, ev, blocks};
+        inflight.push_back(L);
+        launches++;
+        if (g.size() > 1) stacked += g.size();
+        return true;
+    }
+
+    void run() {
+        (void)hipSetDevice(cfg.device);
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            // Completions.
+            for (size_t i = 0; i < inflight.size();) {
+                Launch* L = inflight[i];
+                const hipError_t q = hipEventQuery(L->ev);
+                if (q == hipErrorNotReady) {
+                    i++;
+                    continue;
+                }
+                const double t = now_s();
+                for (Req* r : L->reqs) {
+                    r->done = true;
+                    r->t_done = t;
+                    if (q != hipSuccess) {
+                        r->status = COGCONV_ERR_RUNTIME;
+                        r->error = hipGetErrorString(q);
+                    } else {
+                        completed++;
+                        lat_ms.push_back((t - r->t_submit) * 1e3);
+                        gflop += r->flops * 1e-9;
+                        last_complete = t;
+                    }
+                }
+                for (auto& b : L->blocks) arena.release(b.first, b.second);
+                event_pool.push_back(L->ev);
+                delete L;
+                inflight.erase(inflight.begin() + i);
+                cv_done.notify_all();
+            }
+            // Admissions, in arrival order, while slots and budget allow.
+            while (!pending.empty() && (int)inflight.size() < cfg.max_inflight) {
+                Req* h = pending.front();
+                std::vector<Req*> g{h};
+                std::vector<size_t> idx{0};
+                if (stackable(h->r.desc.mode) && cfg.max_stack > 1 && h->r.bank >= 0 &&
+                    !(h->r.flags & COGCONV_REQUEST_RESIDENT)) {
+                    const std::string key = bucket_key(h);
+                    for (size_t j = 1; j < pending.size() && (int)g.size() < cfg.max_stack; j++) {
+                        if (bucket_key(pending[j]) == key) {
+                            g.push_back(pending[j]);
+                            idx.push_back(j);
+                        }
+                    }
+                }
+                if (!issue(g)) {
+                    if (g.size() > 1) {
+                        g.resize(1);
+                        idx.resize(1);
+                        if (!issue(g)) break;
+                    } else {
+                        break;  // wait for memory
+                    }
+                }
+                for (size_t k = idx.size(); k-- > 0;) pending.erase(pending.begin() + idx[k]);
+            }
+            if (stop && pending.empty() && inflight.empty()) break;
+            if (inflight.empty()) {
+                cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+            } else {
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(10));
+                lk.lock();
+            }
+        }
+    }
+};
+
+extern "C" int cogconv_server_create(const cogconv_server_config* config, cogconv_server* server) {
+    clear_error();
+    if (!config || !server) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    *server = nullptr;
+    if (config->hbm_budget_bytes < 4096 || config->max_inflight < 1 || config->streams < 1 || config->max_stack < 1)
+        return fail(COGCONV_ERR_INVALID_ARG, "bad server configuration");
+    int e = cogconv_init(config->device);
+    if (e) return e;
+    cogconv_server s = new (std::nothrow) cogconv_server_s;
+    if (!s) return fail(COGCONV_ERR_OOM, "host allocation failed");
+    s->cfg = *config;
+    void* base = nullptr;
+    e = check_hip(hipMalloc(&base, config->hbm_budget_bytes), "server budget allocation");
+    if (e) {
+        delete s;
+        return e;
+    }
+    s->arena.base = static_cast<char*>(base);
+    s->arena.size = config->hbm_budget_bytes & ~size_t(255);
+    s->arena.free_[0] = s->arena.size;
+    for (int i = 0; i < config->streams; i++) {
+        hipStream_t st;
+        e = check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream create");
+        if (e) {
+            (void)cogconv_server_destroy(s);
+            return e;
+        }
+        s->streams.push_back(st);
+    }
+    s->worker = std::thread([s] { s->run(); });
+    *server = s;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_server_destroy(cogconv_server s) {
+    clear_error();
+    if (!s) return COGCONV_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (auto st : s->streams) (void)hipStreamDestroy(st);
+    for (auto ev : s->event_pool) (void)hipEventDestroy(ev);
+    for (auto& b : s->banks) (void)hipFree(b.dptr);
+    for (auto& kv : s->plans) cogconv_plan_destroy(kv.second);
+    for (auto& kv : s->reqs) delete kv.second;
+    if (s->arena.base) (void)hipFree(s->arena.base);
+    delete s;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_server_register_bank(cogconv_server s, const float* host_weights,
+                                            const cogconv_field_type* type, int32_t* bank_id) {
+    clear_error();
+    if (!s || !host_weights || !type || !bank_id) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    const size_t bytes = (size_t)type->rows * type->cols * type->planes * sizeof(float);
+    if (!bytes) return fail(COGCONV_ERR_INVALID_ARG, "empty bank");
+    float* d = nullptr;
+    int e = check_hip(hipMalloc(&d, bytes), "bank allocation");
+    if (e) return e;
+    e = check_hip(hipMemcpy(d, host_weights, bytes, hipMemcpyHostToDevice), "bank upload");
+    if (e) {
+        (void)hipFree(d);
+        return e;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->banks.push_back({d, *type});
+    *bank_id = (int32_t)s->banks.size() - 1;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_server_submit(cogconv_server s, const cogconv_request* request, uint64_t* ticket) {
+    clear_error();
+    if (!s || !request || !ticket) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    if (!request->host_in0 || !request->host_out) return fail(COGCONV_ERR_INVALID_ARG, "null buffer");
+    cogconv_desc d = request->desc;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (request->bank >= 0) {
+            if (request->bank >= (int32_t)s->banks.size())
+                return fail(COGCONV_ERR_INVALID_ARG, "unknown filter bank %d", request->bank);
+            d.in1 = s->banks[request->bank].type;
+        } else if (!request->host_in1) {
+            return fail(COGCONV_ERR_INVALID_ARG, "request without bank needs host_in1");
+        }
+    }
+    cogconv_field_type out;
+    int e = cogconv_output_type(&d, &out);
+    if (e) return e;
+    Req* q = new (std::nothrow) Req;
+    if (!q) return fail(COGCONV_ERR_OOM, "host allocation failed");
+    q->r = *request;
+    q->r.desc = d;
+    q->out = out;
+    q->in0_bytes = (size_t)d.in0.rows * d.in0.cols * d.in0.planes * sizeof(float);
+    q->in1_bytes = (size_t)d.in1.rows * d.in1.cols * d.in1.planes * sizeof(float);
+    q->out_bytes = (size_t)out.rows * out.cols * out.planes * sizeof(float);
+    {
+        size_t ws = 0;
+        const std::string key(reinterpret_cast<const char*>(&d), sizeof d);
+        bool cached = false;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            auto it = s->plan_info.find(key);
+            if (it != s->plan_info.end()) {
+                ws = it->second.first;
+                q->flops = it->second.second;
+                cached = true;
+            }
+        }
+        if (!cached) {
+            cogconv_plan p = nullptr;
+            const int pe = cogconv_plan_create(&d, &p);
+            if (pe) {
+                delete q;
+                return pe;
+            }
+            cogconv_plan_workspace_bytes(p, &ws);
+            cogconv_plan_flops(p, &q->flops);
+            cogconv_plan_destroy(p);
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->plan_info[key] = {ws, q->flops};
+        }
+        const bool resident = (request->flags & COGCONV_REQUEST_RESIDENT) != 0;
+        const size_t need = resident ? Arena::round(ws)
+                                     : Arena::round(q->in0_bytes) + Arena::round(q->out_bytes) + Arena::round(ws) +
+                                           (request->bank < 0 ? Arena::round(q->in1_bytes) : 0);
+        if (need > s->arena.size) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->rejected++;
+            delete q;
+            return fail(COGCONV_ERR_BUDGET, "request needs %zu bytes, budget is %zu", need, s->arena.size);
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        q->ticket = s->next_ticket++;
+        q->t_submit = now_s();
+        if (!s->submitted) s->first_submit = q->t_submit;
+        s->submitted++;
+        s->reqs[q->ticket] = q;
+        s->pending.push_back(q);
+        *ticket = q->ticket;
+    }
+    s->cv_work.notify_one();
+    return COGCONV_OK;
+}
+
+static int ticket_status(cogconv_server s, uint64_t ticket, bool* done) {
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(COGCONV_ERR_INVALID_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    *done = it->second->done;
+    if (it->second->done && it->second->status)
+        return fail(it->second->status, "request %llu failed: %s", (unsigned long long)ticket,
+                    it->second->error.c_str());
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_server_query(cogconv_server s, uint64_t ticket) {
+    clear_error();
+    if (!s) return fail(COGCONV_ERR_INVALID_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    bool done = false;
+    int e = ticket_status(s, ticket, &done);
+    if (e) return e;
+    return done ? COGCONV_OK : COGCONV_ERR_NOT_READY;
+}
+
+extern "C" int cogconv_server_wait(cogconv_server s, uint64_t ticket) {
+    clear_error();
+    if (!s) return fail(COGCONV_ERR_INVALID_ARG, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    bool done = false;
+    int e = ticket_status(s, ticket, &done);
+    if (e) return e;
+    s->cv_done.wait(lk, [&] { return s->reqs[ticket]->done; });
+    return ticket_status(s, ticket, &done);
+}
+
+extern "C" int cogconv_server_drain(cogconv_server s) {
+    clear_error();
+    if (!s) return fail(COGCONV_ERR_INVALID_ARG, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->inflight.empty(); });
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_server_stats_get(cogconv_server s, cogconv_server_stats* out) {
+    clear_error();
+    if (!s || !out) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(out, 0, sizeof *out);
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->rejected = s->rejected;
+    out->launches = s->launches;
+    out->stacked = s->stacked;
+    out->first_submit_s = s->first_submit;
+    out->last_complete_s = s->last_complete;
+    out->gflop = s->gflop;
+    out->peak_bytes = s->arena.peak;
+    if (!s->lat_ms.empty()) {
+        std::vector<double> v = s->lat_ms;
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) {
+            const double pos = p * (v.size() - 1);
+            const size_t i = (size_t)pos;
+            const double f = pos - i;
+            return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+        };
+        out->p50_ms = pct(0.50);
+        out->p99_ms = pct(0.99);
+        double sum = 0;
+        for (double x : v) sum += x;
+        out->mean_ms = sum / v.size();
+        out->max_ms = v.back();
+        const double span = s->last_complete - s->first_submit;
+        out->requests_per_s = span > 0 ? s->completed / span : 0;
+    }
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_server_stats_reset(cogconv_server s) {
+    clear_error();
+    if (!s) return fail(COGCONV_ERR_INVALID_ARG, "null server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lat_ms.clear();
+    s->submitted = s->completed = s->rejected = s->launches = s->stacked = 0;
+    s->first_submit = s->last_complete = s->gflop = 0;
+    s->arena.peak = s->arena.in_use;
+    for (auto it = s->reqs.begin(); it != s->reqs.end();) {
+        if (it->second->done) {
+            delete it->second;
+            it = s->reqs.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_server_submit_timed(cogconv_server s, const cogconv_request* requests, const double* arrival_s,
+                                           int64_t n, uint64_t* tickets) {
+    clear_error();
+    if (!s || (n > 0 && (!requests || !arrival_s || !tickets))) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    const double t0 = now_s();
+    int first_error = COGCONV_OK;
+    std::string msg;
+    for (int64_t i = 0; i < n; i++) {
+        const double due = t0 + arrival_s[i];
+        for (;;) {
+            const double left = due - now_s();
+            if (left <= 0) break;
+            if (left > 200e-6) std::this_thread::sleep_for(std::chrono::microseconds((long)((left - 100e-6) * 1e6)));
+        }
+        const int e = cogconv_server_submit(s, &requests[i], &tickets[i]);
+        if (e) {
+            tickets[i] = 0;
+            if (!first_error) {
+                first_error = e;
+                msg = cogconv_last_error();
+            }
+        }
+    }
+    if (first_error) return fail(first_error, "%s", msg.c_str());
+    return COGCONV_OK;
+}

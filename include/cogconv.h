@@ -224,6 +224,72 @@ int cogconv_launch(cogconv_plan plan, const float* in0, const float* in1, float*
 int cogconv_block_reduce_sum(const float* in, float* out, int64_t points, int32_t in_planes, int32_t factor,
                              cogconv_stream stream);
 
+/* ---- serving loop ----
+ * No counterpart inside the reference's convolution path: the reference specifies a
+ * reentrant evaluate() Function Interface (doc/CogFunctionInterfaceERS.docx,
+ * src/main/java/cogjni/CogJNIWrapper.java:85-115) but has no request scheduler.  The
+ * server keeps filter banks resident in HBM, admits requests against a device-memory
+ * budget as request slots free up, stacks requests of one shape bucket (same op, same
+ * bank, same field shape) into one launch when the mode allows (batch folded into
+ * planes), spreads work over several streams so that transfers and kernels of
+ * different requests overlap, and reports latency percentiles. */
+
+typedef struct cogconv_server_s* cogconv_server;
+
+typedef struct {
+    int32_t device;
+    uint64_t hbm_budget_bytes; /* device memory for request buffers; preallocated at create */
+    int32_t max_inflight;      /* launches on the device at once */
+    int32_t streams;           /* HIP streams work is spread over */
+    int32_t max_stack;         /* requests stacked into one launch (1 = no stacking) */
+} cogconv_server_config;
+
+/* Request flags. */
+enum {
+    COGCONV_REQUEST_RESIDENT = 1 /* in0 / in1 / out are device pointers (state resident in HBM): no copies */
+};
+
+typedef struct {
+    cogconv_desc desc;     /* in0: the request's field; in1: the bank's type when bank >= 0 */
+    int32_t bank;          /* resident filter bank, or -1 to send in1 with the request */
+    int32_t flags;         /* COGCONV_REQUEST_* */
+    const float* host_in0; /* planar fp32; caller keeps it valid until completion */
+    const float* host_in1; /* used when bank < 0 */
+    float* host_out;       /* planar fp32 result (the op's output FieldType) */
+} cogconv_request;
+
+typedef struct {
+    uint64_t submitted, completed, rejected;
+    uint64_t launches;         /* kernel launches issued */
+    uint64_t stacked;          /* requests that shared a launch with another */
+    double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> result in host memory */
+    double first_submit_s, last_complete_s; /* monotonic clock */
+    double requests_per_s;     /* completed / (last_complete - first_submit) */
+    double gflop;              /* algorithmic GFLOP of completed requests */
+    uint64_t peak_bytes;       /* peak device bytes held for requests */
+} cogconv_server_stats;
+
+int cogconv_server_create(const cogconv_server_config* config, cogconv_server* server);
+/* Waits for in-flight work, then frees everything. */
+int cogconv_server_destroy(cogconv_server server);
+/* Copy a filter bank into HBM once; requests then name it by id. */
+int cogconv_server_register_bank(cogconv_server server, const float* host_weights, const cogconv_field_type* type,
+                                 int32_t* bank_id);
+/* Non-blocking.  Illegal ops fail here (COGCONV_ERR_ILLEGAL_OP); a request larger than the
+ * whole budget fails with COGCONV_ERR_BUDGET; otherwise it queues and gets a ticket. */
+int cogconv_server_submit(cogconv_server server, const cogconv_request* request, uint64_t* ticket);
+/* COGCONV_OK when the ticket's result is in host memory, COGCONV_ERR_NOT_READY before. */
+int cogconv_server_query(cogconv_server server, uint64_t ticket);
+/* Submit n requests from the calling thread, request i at (call time + arrival_s[i])
+ * (a replayed arrival trace; all zeros = submit back to back).  Blocks until the
+ * last one is submitted; tickets[i] receives each ticket (0 if it was refused). */
+int cogconv_server_submit_timed(cogconv_server server, const cogconv_request* requests, const double* arrival_s,
+                                int64_t n, uint64_t* tickets);
+int cogconv_server_wait(cogconv_server server, uint64_t ticket);
+int cogconv_server_drain(cogconv_server server);
+int cogconv_server_stats_get(cogconv_server server, cogconv_server_stats* stats);
+int cogconv_server_stats_reset(cogconv_server server);
+
 #ifdef __cplusplus
 }
 #endif
