@@ -260,7 +260,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
-    ap.add_argument("--serial", action="store_true", help="run the three ops of a step back to back on one stream")
+    ap.add_argument("--concurrent", action="store_true",
+                    help="fork the three independent ops of a step onto three streams (default: back to back on one)")
     ap.add_argument("--workload", default="cnn-triple", choices=("cnn-triple", "precision", "serving"))
     ap.add_argument("--requests", type=int, default=3000, help="requests of the serving workload")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -283,8 +284,12 @@ def main():
     log("device:", props)
 
     if args.copy_calibration:
-        src, dst = L.DeviceBuffer(64 << 20), L.DeviceBuffer(64 << 20)
+        # Known-size reference for the HBM counters: fill 1 GiB, then copy its
+        # first 64 MiB (long evicted from the 256 MB Infinity Cache by the fill)
+        # three times.
+        src, dst = L.DeviceBuffer(1 << 30), L.DeviceBuffer(64 << 20)
         src.zero()
+        L.synchronize()
         for _ in range(3):
             L.check(L.lib().cogconv_d2d_async(dst.ptr, src.ptr, 64 << 20, None))
         L.synchronize()
@@ -325,13 +330,13 @@ def main():
         per_op_ms.append(e0.elapsed_ms(e1) / iso_reps)
 
     # Phase 2: the timed steps.  The three ops of a step are independent
-    # (PF reads x, W; BPF reads dy, W; FA reads x, dy), so a step forks them
-    # onto three streams and joins; --serial runs them back to back on one.
+    # (PF reads x, W; BPF reads dy, W; FA reads x, dy); by default they run back
+    # to back on one stream, --concurrent forks them onto three streams and joins.
     fork = [L.Event() for _ in range(args.steps + args.warmup)]
     done = [[L.Event() for _ in ops] for _ in range(args.steps + args.warmup)]
 
     def run_step(k):
-        if args.serial:
+        if not args.concurrent:
             for i in range(len(ops)):
                 launch(i, main_stream)
             return
@@ -404,7 +409,7 @@ def main():
         "config": {"workload": "FP32 CNN-layer triple: PF+BRS Valid, BPF+BRS Full, FA+BRS Valid; batch 16 "
                                "images 64x64x16, 32 filters 3x3x16", "batch": 16, "image": [64, 64, 16],
                    "filters": [32, 3, 3, 16], "flops_per_step": flops_step,
-                   "step": "serial on one stream" if args.serial else "three independent ops on three streams"},
+                   "step": "three independent ops on three streams" if args.concurrent else "ops back to back on one stream"},
         "per_op_isolated": {op["name"]: {"ms": round(t, 5), "gflops": round(op["plan"].flops / (t * 1e-3) / 1e9, 1),
                                          "kernel": op["plan"].description} for op, t in zip(ops, per_op_ms)},
         "roofline": roof,
