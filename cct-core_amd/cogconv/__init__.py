@@ -6,7 +6,7 @@ reference's Field / ComputeGraph operator API for this path.
 from . import _lib as lib  # noqa: F401
 from ._lib import CogconvError  # noqa: F401
 from .api import (  # noqa: F401
-    BorderClamp, BorderCyclic, BorderFull, BorderValid, BorderZero, ComputeGraph,
+    BorderClamp, BorderCyclic, BorderFull, BorderValid, BorderZero, Cog, ComputeGraph,
     DownsampleOutputConvolution, NoSamplingConvolution, ScalarField, UpsampleInputConvolution, VectorField,
     backProjectFrame, blockReduceSum, convolve, convolveFilterAdjoint, crossCorrelate,
     crossCorrelateFilterAdjoint, probe, projectFrame)

@@ -189,6 +189,21 @@ def blockReduceSum(field, factor):
                  params={"factor": factor})
 
 
+class Cog:
+    """Global switches of the convolution path, mirroring the reference's
+    `Cog` flags (src/main/scala/cogx/parameters/Cog.scala:64-72), with the same
+    defaults.  The merging flags gate the reduce fusion per vector mode;
+    `fastProjectFrame` lets the planner use multi-plane blocks for the fused
+    ProjectFrame (the MFMA implicit GEMM) rather than one output plane per
+    block; `localMemoryMerging` has no counterpart here (no elementwise
+    epilogue fusion yet) and is kept for source compatibility."""
+    fastProjectFrame = True
+    projectFrameMerging = True
+    backProjectFrameMerging = True
+    filterAdjointMerging = True
+    localMemoryMerging = True
+
+
 _FUSED = {L.PROJECT_FRAME: L.PROJECT_FRAME_BLOCK_REDUCE_SUM,
           L.BACK_PROJECT_FRAME: L.BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM,
           L.FILTER_ADJOINT: L.FILTER_ADJOINT_BLOCK_REDUCE_SUM}
@@ -225,6 +240,10 @@ class ComputeGraph:
             return None
         mode = conv.params["mode"]
         if mode not in _FUSED:
+            return None
+        enabled = {L.PROJECT_FRAME: Cog.projectFrameMerging, L.BACK_PROJECT_FRAME: Cog.backProjectFrameMerging,
+                   L.FILTER_ADJOINT: Cog.filterAdjointMerging}[mode]
+        if not enabled:
             return None
         batch = conv.params["batch"]
         p0 = conv.inputs[0].ftype[3]
@@ -277,8 +296,10 @@ class ComputeGraph:
         a, k = conv.inputs
         pr = conv.params
         precision = self.precision if mode in _FUSED.values() else L.FP32
+        algo = L.ALGO_AUTO if Cog.fastProjectFrame else L.ALGO_DIRECT
         desc = L.make_desc(mode, pr["border"].code, pr["orientation"], pr["sampling"].code, pr["sampling"].step,
-                           pr["batch"], a.ftype, k.ftype, precision=precision)
+                           pr["batch"], a.ftype, k.ftype, precision=precision,
+                           algo=algo if precision == L.FP32 else L.ALGO_AUTO)
         plan = L.Plan(desc)
         _, r, c, p, _ = result.ftype
         if plan.out_shape != (p, r, c):

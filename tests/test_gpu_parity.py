@@ -294,6 +294,35 @@ def test_fused_equals_unfused():
         assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
 
 
+def test_cog_merging_flags_gpu():
+    """Cog.projectFrameMerging / fastProjectFrame (Cog.scala:64-72) switch the
+    fusion and the kernel family; the values do not change."""
+    announce("Cog flags")
+    rng = np.random.default_rng(78)
+    x = rng.uniform(-1, 1, (2 * 4, 18, 18)).astype(np.float32)
+    w = rng.uniform(-1, 1, (3 * 4, 3, 3)).astype(np.float32)
+    results = []
+    saved = (cogconv.Cog.projectFrameMerging, cogconv.Cog.fastProjectFrame)
+    try:
+        for merging, fast in ((True, True), (False, True), (True, False)):
+            cogconv.Cog.projectFrameMerging, cogconv.Cog.fastProjectFrame = merging, fast
+            with cogconv.ComputeGraph() as cg:
+                y = cogconv.blockReduceSum(cogconv.projectFrame(cogconv.VectorField(x), cogconv.VectorField(w),
+                                                                cogconv.BorderZero, batchSize=2), 4)
+                cogconv.probe(y)
+            cg.step()
+            kinds = [p.description.split()[0] for p in cg.plans()]
+            results.append((cg.read(y), kinds))
+            cg.release()
+    finally:
+        cogconv.Cog.projectFrameMerging, cogconv.Cog.fastProjectFrame = saved
+    assert results[0][1] == ["igemm_f32"]
+    assert results[1][1] == ["direct"]          # unfused: ProjectFrame then blockReduceSum
+    assert results[2][1] == ["direct"]          # fused, one output plane per block
+    for r, _ in results[1:]:
+        assert np.abs(r - results[0][0]).max() <= 1e-5 * np.abs(results[0][0]).max()
+
+
 def test_block_reduce_sum_gpu():
     announce("blockReduceSum")
     rng = np.random.default_rng(3)
