@@ -81,6 +81,16 @@ class ServerStatsC(ctypes.Structure):
                 ("requests_per_s", ctypes.c_double), ("gflop", ctypes.c_double), ("peak_bytes", ctypes.c_uint64)]
 
 
+class TuneOptionsC(ctypes.Structure):
+    _fields_ = [("warmup_steps", ctypes.c_int32), ("run_steps", ctypes.c_int32), ("flush_bytes", ctypes.c_int64),
+                ("force", ctypes.c_int32)]
+
+
+class TuneResultC(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int32), ("variants", ctypes.c_int32), ("avg_us", ctypes.c_double),
+                ("from_cache", ctypes.c_int32), ("description", ctypes.c_char * 192)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _SZ = ctypes.c_size_t
@@ -124,6 +134,13 @@ SIGNATURES = {
     "cogconv_plan_describe": [_P, ctypes.c_char_p, _SZ],
     "cogconv_launch": [_P, _P, _P, _P, _P, _P],
     "cogconv_block_reduce_sum": [_P, _P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _P],
+    "cogconv_plan_variant_count": [ctypes.POINTER(DescC), ctypes.POINTER(ctypes.c_int32)],
+    "cogconv_plan_create_variant": [ctypes.POINTER(DescC), ctypes.c_int32, ctypes.POINTER(_P)],
+    "cogconv_plan_variant": [_P, ctypes.POINTER(ctypes.c_int32)],
+    "cogconv_autotune": [ctypes.POINTER(DescC), ctypes.POINTER(TuneOptionsC), _P, ctypes.POINTER(TuneResultC)],
+    "cogconv_tune_cache_clear": [],
+    "cogconv_tune_cache_save": [ctypes.c_char_p],
+    "cogconv_tune_cache_load": [ctypes.c_char_p],
     "cogconv_server_create": [ctypes.POINTER(ServerConfigC), ctypes.POINTER(_P)],
     "cogconv_server_destroy": [_P],
     "cogconv_server_register_bank": [_P, _P, ctypes.POINTER(FieldTypeC), ctypes.POINTER(ctypes.c_int32)],
@@ -309,13 +326,20 @@ def output_type(desc):
 
 
 class Plan:
-    """One ConvolveOp execution plan (cogconv_plan_create / cogconv_launch)."""
+    """One ConvolveOp execution plan (cogconv_plan_create / cogconv_launch);
+    `variant` fixes one of the plan's candidate configurations."""
 
-    def __init__(self, desc):
+    def __init__(self, desc, variant=None):
         self.desc = desc
         p = ctypes.c_void_p()
-        check(lib().cogconv_plan_create(ctypes.byref(desc), ctypes.byref(p)))
+        if variant is None:
+            check(lib().cogconv_plan_create(ctypes.byref(desc), ctypes.byref(p)))
+        else:
+            check(lib().cogconv_plan_create_variant(ctypes.byref(desc), variant, ctypes.byref(p)))
         self.handle = p.value
+        v = ctypes.c_int32()
+        check(lib().cogconv_plan_variant(self.handle, ctypes.byref(v)))
+        self.variant = v.value
         out = FieldTypeC()
         check(lib().cogconv_plan_output_type(self.handle, ctypes.byref(out)))
         self.out_type = out.as_tuple()
@@ -353,6 +377,35 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def variant_count(desc):
+    n = ctypes.c_int32()
+    check(lib().cogconv_plan_variant_count(ctypes.byref(desc), ctypes.byref(n)))
+    return n.value
+
+
+def autotune(desc, warmup_steps=1, run_steps=5, flush_bytes=512 << 20, force=False, stream=None):
+    """Times the candidate configurations of `desc` (cogconv_autotune); returns
+    dict(variant, variants, avg_us, from_cache, description)."""
+    o = TuneOptionsC(warmup_steps, run_steps, flush_bytes, 1 if force else 0)
+    r = TuneResultC()
+    s = stream.handle if isinstance(stream, Stream) else stream
+    check(lib().cogconv_autotune(ctypes.byref(desc), ctypes.byref(o), s, ctypes.byref(r)))
+    return {"variant": r.variant, "variants": r.variants, "avg_us": r.avg_us, "from_cache": bool(r.from_cache),
+            "description": r.description.decode()}
+
+
+def tune_cache_clear():
+    check(lib().cogconv_tune_cache_clear())
+
+
+def tune_cache_save(path):
+    check(lib().cogconv_tune_cache_save(os.fsencode(path)))
+
+
+def tune_cache_load(path):
+    check(lib().cogconv_tune_cache_load(os.fsencode(path)))
 
 
 def block_reduce_sum(inp, out, points, in_planes, factor, stream=None):

@@ -278,4 +278,17 @@ hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Constant fill of n floats (the autotuner's synthetic inputs).
+__global__ void k_fill_value(float* __restrict__ p, size_t n, float v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+hipError_t launch_fill_value(float* p, size_t n, float v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_fill_value, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
 }  // namespace cogconv

@@ -11,21 +11,22 @@
 //
 // gfx950 mapping: v_mfma_f32_16x16x4_f32; each MFMA consumes four consecutive
 // tap columns of one tap row (the four k-lanes, lane/16).  A block (4 wave64)
+// owns a column group of CBG 16-column blocks and all MB row blocks.  It
 // stages RR tap rows of all K dy planes of one image (stored k-fastest with a
 // row pitch of 16 mod 32 words, so the A-operand reads are bank-conflict free)
-// plus the x rows they touch, for the input planes its column group needs;
-// staging runs one row per wave pass with lanes across columns.  Wave w owns column
-// blocks w, w+4, .. (CBW of them) and all MB row blocks.  Blocks split the
-// (image, tap-row chunk) space; each writes one partial dW, and the partials
-// are added in a fixed order by a two-level reduction, so results do not
-// depend on scheduling.
+// plus the x rows they touch, for the input planes of its column group; staging
+// runs one row per wave pass with lanes across columns.  Wave w reduces tap
+// rows w, w+4, .. for every tile of the group, and the four waves' sums meet in
+// LDS.  Blocks split the (image, tap-row chunk) space; each writes one partial
+// dW, and the partials are added in a fixed order by a two-level reduction, so
+// results do not depend on scheduling.
 #include "internal.h"
 
 namespace cogconv {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-template <int MB, int CBW>
+template <int MB, int CBG>
 __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
                                                       const float* __restrict__ DY, float* __restrict__ ws) {
     // Staged dy rows are k-fastest with a pitch of 16*MB padded to 16 mod 32 words.
@@ -34,6 +35,7 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
     const int MTp = MTP;
     float* sD = smem;                                  // [RR][DWp][MTP]
     float* sX = smem + (size_t)c.RR * c.DWp * MTP;     // [nCnt][xRows][xCols]
+    float* sJunk = sX + (size_t)c.nCntMax * c.xRows * c.xCols;  // 128 scratch slots for masked-off stores
     const int FHW = g.FH * g.FW;                       // 32-bit offsets: the plan checks the sizes
     const int HWx = g.H * g.W;
 
@@ -68,16 +70,16 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
     }
     const int P = g.OH * g.OW;
     const int Ncols = N * P;
-    const int colBase = cg * 4 * CBW * 16;
+    const int colBase = cg * CBG * 16;
     const int n_lo = colBase / P;
-    const int n_hi = min(N - 1, (min(colBase + 4 * CBW * 16, Ncols) - 1) / P);
+    const int n_hi = min(N - 1, (min(colBase + CBG * 16, Ncols) - 1) / P);
     const int nCnt = n_hi - n_lo + 1;
     const int xPlane = c.xRows * c.xCols;
 
-    int xoff[CBW];
+    int xoff[CBG];
 #pragma unroll
-    for (int q = 0; q < CBW; q++) {
-        const int j = colBase + (wave + 4 * q) * 16 + l16;
+    for (int q = 0; q < CBG; q++) {
+        const int j = colBase + q * 16 + l16;
         int off = 0;
         if (j < Ncols) {
             const int n = j / P, rem = j - n * P;
@@ -87,11 +89,11 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         xoff[q] = off + g4;
     }
 
-    floatx4 acc[MB][CBW];
+    floatx4 acc[MB][CBG];
 #pragma unroll
     for (int i = 0; i < MB; i++)
 #pragma unroll
-        for (int q = 0; q < CBW; q++) acc[i][q] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int q = 0; q < CBG; q++) acc[i][q] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     const long items = (long)m.B * c.rowChunks;
     const long it0 = items * s / c.splits, it1 = items * (s + 1) / c.splits;
@@ -100,118 +102,172 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         const int i0 = (int)(it - (long)b * c.rowChunks) * c.RR;
         const int rr = min(c.RR, g.FHe - i0);
         __syncthreads();
-        // dy tap rows and x rows: one row per wave pass, lanes across columns
-        // (at most 128).  Every item loads unconditionally from a clamped,
-        // in-bounds address and masks afterwards, so the U loads of a wave are
-        // all in flight before the first LDS store.
-        // Staging: one row per wave pass; the row of an item is wave-uniform
-        // (scalar work), the lanes' column offsets were computed once per
-        // block; loads are unconditional from in-bounds offsets and all U of a
-        // wave are in flight before the first LDS store.
-        constexpr int U = 24;
-        const int dRows = MTP * c.RR;  // (il, k)
+        // Staging of the dy tap rows and x rows: one row per wave pass, lanes
+        // across columns (at most 128).  Per batch of U rows: the rows' offsets
+        // and masks first (wave-uniform scalar work), then all loads, from
+        // clamped in-bounds offsets and with no branch between them, then the
+        // masked LDS stores; rows past the end and lanes past the row width
+        // store into a scratch slot instead of branching.
+        constexpr int U = 16;
+        constexpr int MT = 16 * MB;    // dy planes the MFMAs read (K padded with zeros)
+        const int dRows = MT * rr;     // (il, k); the pitch padding beyond MT is never read
         const float* DYb = DY + (size_t)b * K * FHW;
         for (int base = wave; base < dRows; base += 4 * U) {
-            float v0[U], v1[U];
+            int offU[U];
+            bool onU[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int row = base + 4 * u;
-                const int il = row / MTP;
-                const int k = row - il * MTP;
+                const int il = row / MT;
+                const int k = row - il * MT;
                 int ti = i0 + il;
                 if (g.flip) ti = g.FHe - 1 - ti;
-                bool on = row < dRows && k < K && il < rr;
+                bool on = row < dRows && k < K;
                 if (g.upFr > 1) { on = on && (ti % g.upFr) == 0; ti /= g.upFr; }
                 ti = min(max(ti, 0), g.FH - 1);
-                const float* src = DYb + min(k, K - 1) * FHW + ti * g.FW;
-                const float a0 = src[dCol0];
-                v0[u] = (on && dIn0) ? a0 : 0.f;
-                if (dTwo) {
-                    const float a1 = src[dCol1];
-                    v1[u] = (on && dIn1) ? a1 : 0.f;
+                offU[u] = min(k, K - 1) * FHW + ti * g.FW;
+                onU[u] = on;
+            }
+            float v0[U], v1[U];
+            if (dTwo) {
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    v0[u] = DYb[offU[u] + dCol0];
+                    v1[u] = DYb[offU[u] + dCol1];
                 }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; u++) v0[u] = DYb[offU[u] + dCol0];
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int row = base + 4 * u;
-                if (row < dRows) {
-                    const int il = row / MTP;
-                    float* dst = sD + (size_t)il * c.DWp * MTP + (row - il * MTP);
-                    if (lane < c.DWp) dst[lane * MTP] = v0[u];
-                    if (dTwo && lane + 64 < c.DWp) dst[(lane + 64) * MTP] = v1[u];
+                const bool live = row < dRows;  // wave-uniform
+                const int il = row / MT;
+                float* dst = sD + (size_t)il * c.DWp * MTP + (row - il * MT);
+                dst = (live && lane < c.DWp) ? dst + lane * MTP : sJunk + lane;
+                *dst = (onU[u] && dIn0) ? v0[u] : 0.f;
+                if (dTwo) {
+                    float* dst1 = sD + (size_t)il * c.DWp * MTP + (row - il * MT);
+                    dst1 = (live && lane + 64 < c.DWp) ? dst1 + (lane + 64) * MTP : sJunk + 64 + lane;
+                    *dst1 = (onU[u] && dIn1) ? v1[u] : 0.f;
                 }
             }
         }
         const int xRowsAll = nCnt * c.xRows;  // (plane, row)
         const float* Xb = X + ((size_t)b * N + n_lo) * HWx;
         for (int base = wave; base < xRowsAll; base += 4 * U) {
-            float v0[U], v1[U];
-            int nlq = base / c.xRows;
-            int lrq = base - nlq * c.xRows;
+            int offU[U], dstU[U];
+            bool onU[U];
+            {
+                int nlq = base / c.xRows;
+                int lrq = base - nlq * c.xRows;
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int gy = i0 + lrq;
-                const bool on = base + 4 * u < xRowsAll && gy < g.H;
-                const float* src = Xb + min(nlq, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W;
-                const float a0 = src[xCol0];
-                v0[u] = (on && xIn0) ? a0 : 0.f;
-                if (xTwo) {
-                    const float a1 = src[xCol1];
-                    v1[u] = (on && xIn1) ? a1 : 0.f;
+                for (int u = 0; u < U; u++) {
+                    const int gy = i0 + lrq;
+                    const bool live = base + 4 * u < xRowsAll;
+                    onU[u] = live && gy < g.H;
+                    offU[u] = min(nlq, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W;
+                    dstU[u] = live ? nlq * xPlane + lrq * c.xCols : -1;
+                    lrq += 4;
+                    while (lrq >= c.xRows) { lrq -= c.xRows; nlq++; }
                 }
-                lrq += 4;
-                while (lrq >= c.xRows) { lrq -= c.xRows; nlq++; }
             }
-            nlq = base / c.xRows;
-            lrq = base - nlq * c.xRows;
+            float v0[U], v1[U];
+            if (xTwo) {
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    v0[u] = Xb[offU[u] + xCol0];
+                    v1[u] = Xb[offU[u] + xCol1];
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; u++) v0[u] = Xb[offU[u] + xCol0];
+            }
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                if (base + 4 * u < xRowsAll) {
-                    float* dst = sX + (size_t)nlq * xPlane + lrq * c.xCols;
-                    if (lane < c.xCols) dst[lane] = v0[u];
-                    if (xTwo && lane + 64 < c.xCols) dst[lane + 64] = v1[u];
+                const bool live = dstU[u] >= 0;  // wave-uniform
+                float* dst = (live && lane < c.xCols) ? sX + dstU[u] + lane : sJunk + lane;
+                *dst = (onU[u] && xIn0) ? v0[u] : 0.f;
+                if (xTwo) {
+                    float* dst1 = (live && lane + 64 < c.xCols) ? sX + dstU[u] + lane + 64 : sJunk + 64 + lane;
+                    *dst1 = (onU[u] && xIn1) ? v1[u] : 0.f;
                 }
-                lrq += 4;
-                while (lrq >= c.xRows) { lrq -= c.xRows; nlq++; }
             }
         }
         __syncthreads();
-        for (int il = 0; il < rr; il++) {
+        // Wave w reduces tap rows w, w+4, ... of the staged chunk for every
+        // output tile of the block's column group.
+        for (int il = wave; il < rr; il += 4) {
             const float* dRow = sD + ((size_t)il * c.DWp + g4) * MTp + l16;
             const float* xRow = sX + il * c.xCols;
             for (int jb = 0; jb < c.DWp; jb += 4) {
                 float av[MB];
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++) av[mb] = dRow[(size_t)jb * MTp + mb * 16];
-                float bv[CBW];
+                float bv[CBG];
 #pragma unroll
-                for (int q = 0; q < CBW; q++) bv[q] = xRow[xoff[q] + jb];
+                for (int q = 0; q < CBG; q++) bv[q] = xRow[xoff[q] + jb];
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++)
 #pragma unroll
-                    for (int q = 0; q < CBW; q++)
+                    for (int q = 0; q < CBG; q++)
                         acc[mb][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mb], bv[q], acc[mb][q], 0, 0, 0);
             }
         }
     }
-    // Partial dW of this split, ws[s][k][j]: staged through LDS as
-    // [16*MB rows][64*CBW columns] and written row by row with coalesced stores.
+    // The four waves' sums meet in LDS in a fixed order, (w0 + w2) + (w1 + w3),
+    // then wave 0 writes the block's partial dW, ws[s][k][j].
     // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
+    constexpr int E = MB * CBG * 4;  // accumulator floats per lane
     __syncthreads();
-    constexpr int GC = 64 * CBW;
-    float* sP = smem;
-#pragma unroll
-    for (int q = 0; q < CBW; q++)
+    float* red = smem;
+    if (wave >= 2) {
 #pragma unroll
         for (int mb = 0; mb < MB; mb++)
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-                sP[(mb * 16 + 4 * g4 + v) * GC + (wave + 4 * q) * 16 + l16] = acc[mb][q][v];
+            for (int q = 0; q < CBG; q++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    red[(size_t)((wave - 2) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
+    }
     __syncthreads();
-    float* out = ws + (size_t)s * K * Ncols + colBase;
-    const int ncol = min(GC, Ncols - colBase);
-    for (int k = wave; k < K; k += 4)
-        for (int jl = lane; jl < ncol; jl += 64) out[(size_t)k * Ncols + jl] = sP[k * GC + jl];
+    if (wave < 2) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int q = 0; q < CBG; q++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    acc[mb][q][v] += red[(size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane];
+    }
+    __syncthreads();
+    if (wave == 1) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int q = 0; q < CBG; q++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) red[(size_t)((mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        float* out = ws + (size_t)s * K * Ncols;
+#pragma unroll
+        for (int q = 0; q < CBG; q++) {
+            const int j = colBase + q * 16 + l16;
+            if (j >= Ncols) continue;
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int k = mb * 16 + 4 * g4 + v;
+                    if (k < K)
+                        out[(size_t)k * Ncols + j] =
+                            acc[mb][q][v] + red[(size_t)((mb * CBG + q) * 4 + v) * 64 + lane];
+                }
+        }
+    }
 }
 
 // Sum `splits` partial arrays of `total` floats in a fixed order.  Groups of
@@ -277,10 +333,11 @@ hipError_t launch_reduce_partials(float* ws, float* out, int splits, long total,
 template <int MB>
 static hipError_t launch_mb(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
                             float* ws, hipStream_t s, dim3 grid) {
-    if (c.CBW == 3)
-        hipLaunchKernelGGL((k_fa_igemm_f32<MB, 3>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws);
-    else
-        hipLaunchKernelGGL((k_fa_igemm_f32<MB, 2>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws);
+    switch (c.CBW) {
+    case 3: hipLaunchKernelGGL((k_fa_igemm_f32<MB, 3>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    case 6: hipLaunchKernelGGL((k_fa_igemm_f32<MB, 6>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    default: hipLaunchKernelGGL((k_fa_igemm_f32<MB, 9>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    }
     return hipGetLastError();
 }
 

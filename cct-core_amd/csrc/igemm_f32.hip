@@ -35,6 +35,7 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
     float* sB = smem;
     float* sA = smem + (size_t)c.RC * c.planeStride;
     const int aStride = c.aStride;  // floats per staged plane of the filter slice (TAPS*MT padded)
+    float* sJunk = sA + (size_t)c.RC * aStride;  // 128 scratch slots for masked-off stores
 
     int bid = blockIdx.x;
     const int tc = bid % c.tilesC;
@@ -104,50 +105,60 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
         __syncthreads();
         // Input tile: one (plane, row) item per wave pass.  The row of an item
         // is the same for all lanes of a wave (scalar work); the lanes' column
-        // offsets and border masks were computed once per block.  Loads are
-        // unconditional from in-bounds offsets and the IU items of a wave are
-        // all in flight before the first LDS store.
+        // offsets and border masks were computed once per block.  Per batch of
+        // IU items: offsets and masks first, then all loads (unconditional,
+        // from in-bounds offsets, no branch between them), then the masked LDS
+        // stores; items past the end and lanes past the tile width store into
+        // a scratch slot instead of branching.
         const int nrows = c.RC * TRp;
         constexpr int IU = 24;
         for (int base = wave; base < nrows; base += 8 * IU) {
-            float v0[IU], v1[IU];
-            int rcq = base / TRp;
-            int tyq = base - rcq * TRp;
+            int offU[IU], dstU[IU];
+            bool onU[IU];
+            {
+                int rcq = base / TRp;
+                int tyq = base - rcq * TRp;
 #pragma unroll
-            for (int u = 0; u < IU; u++) {
-                const int r = r0 + rcq;
-                int y = y0 + tyq;
-                bool ym = (unsigned)y < (unsigned)g.H;
-                if (g.border == COGCONV_BORDER_CYCLIC) {
-                    y %= g.H;
-                    y += y < 0 ? g.H : 0;
-                    ym = true;
-                } else {
-                    ym = ym || g.border == COGCONV_BORDER_CLAMP;
-                    y = min(max(y, 0), g.H - 1);
+                for (int u = 0; u < IU; u++) {
+                    const int r = r0 + rcq;
+                    int y = y0 + tyq;
+                    bool ym = (unsigned)y < (unsigned)g.H;
+                    if (g.border == COGCONV_BORDER_CYCLIC) {
+                        y %= g.H;
+                        y += y < 0 ? g.H : 0;
+                        ym = true;
+                    } else {
+                        ym = ym || g.border == COGCONV_BORDER_CLAMP;
+                        y = min(max(y, 0), g.H - 1);
+                    }
+                    const bool live = base + 8 * u < nrows;
+                    onU[u] = ym && r < m.R && live;
+                    offU[u] = min(r, m.R - 1) * HW + y * g.W;
+                    dstU[u] = live ? rcq * c.planeStride + tyq * TCp : -1;
+                    tyq += 8;
+                    if (tyq >= TRp) { tyq -= TRp; rcq++; }
                 }
-                const bool rowOn = ym && r < m.R && base + 8 * u < nrows;
-                const float* src = Xb + min(r, m.R - 1) * HW + y * g.W;
-                const float a0 = src[colOff0];
-                v0[u] = (rowOn && colIn0) ? a0 : 0.f;
-                if (twoPass) {
-                    const float a1 = src[colOff1];
-                    v1[u] = (rowOn && colIn1) ? a1 : 0.f;
-                }
-                tyq += 8;
-                if (tyq >= TRp) { tyq -= TRp; rcq++; }
             }
-            rcq = base / TRp;
-            tyq = base - rcq * TRp;
+            float v0[IU], v1[IU];
+            if (twoPass) {
+#pragma unroll
+                for (int u = 0; u < IU; u++) {
+                    v0[u] = Xb[offU[u] + colOff0];
+                    v1[u] = Xb[offU[u] + colOff1];
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < IU; u++) v0[u] = Xb[offU[u] + colOff0];
+            }
 #pragma unroll
             for (int u = 0; u < IU; u++) {
-                if (base + 8 * u < nrows) {
-                    float* dst = sB + rcq * c.planeStride + tyq * TCp;
-                    if (lane < TCp) dst[lane] = v0[u];
-                    if (twoPass && lane + 64 < TCp) dst[lane + 64] = v1[u];
+                const bool live = dstU[u] >= 0;  // wave-uniform
+                float* dst = (live && lane < TCp) ? sB + dstU[u] + lane : sJunk + lane;
+                *dst = (onU[u] && colIn0) ? v0[u] : 0.f;
+                if (twoPass) {
+                    float* dst1 = (live && lane + 64 < TCp) ? sB + dstU[u] + lane + 64 : sJunk + 64 + lane;
+                    *dst1 = (onU[u] && colIn1) ? v1[u] : 0.f;
                 }
-                tyq += 8;
-                if (tyq >= TRp) { tyq -= TRp; rcq++; }
             }
         }
         // Filter slice: thread -> (plane rc, filter mm); the F*F taps are

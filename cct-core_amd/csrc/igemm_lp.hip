@@ -101,6 +101,9 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     const int npix = TRp * TCp;
     T* sB = reinterpret_cast<T*>(lp_smem);                    // [TERMS][npix][RCP]
     T* sA = sB + (size_t)TERMS * npix * LP_RCP;               // [TERMS][TAPS][MT][RCP]
+    // Scratch slots (offset from sB) for the stores of lanes beyond the tile
+    // width, after the filter slice: TERMS x 64 lanes x 8 elements.
+    const size_t sJunk = (size_t)TERMS * npix * LP_RCP + (size_t)TERMS * TAPS * MT * LP_RCP;
 
     int bid = blockIdx.x;
     const int tc = bid % c.tilesC;
@@ -160,10 +163,16 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
         __syncthreads();
         // Input tile: items (8-plane group, row); each lane loads the eight
         // planes of its column, rounds them and writes one 16-byte LDS vector.
+        // The loads of IU items are unconditional from clamped, in-bounds
+        // addresses and all in flight before the first is used; masks are
+        // applied afterwards, and lanes beyond the tile width store into a
+        // scratch slot, so no lane-dependent branch separates the loads.
         const int nitems = 4 * TRp;
         constexpr int IU = 4;
         for (int base = wave; base < nitems; base += 4 * IU) {
             float v[IU][8];
+            bool rowOn[IU];
+            int grpU[IU], tyU[IU];
 #pragma unroll
             for (int u = 0; u < IU; u++) {
                 const int item = base + 4 * u;
@@ -179,45 +188,62 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
                     ym = ym || g.border == COGCONV_BORDER_CLAMP;
                     y = min(max(y, 0), g.H - 1);
                 }
-                const bool on = item < nitems && ym && colIn;
+                rowOn[u] = item < nitems && ym;
+                grpU[u] = grp;
+                tyU[u] = ty;
+                const float* src = Xb + y * g.W + colOff;
 #pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    const int r = r0 + 8 * grp + e;
-                    const float a = Xb[min(r, m.R - 1) * HW + y * g.W + colOff];
-                    v[u][e] = (on && r < m.R) ? a : 0.f;
-                }
+                for (int e = 0; e < 8; e++) v[u][e] = src[min(r0 + 8 * grp + e, m.R - 1) * HW];
             }
 #pragma unroll
             for (int u = 0; u < IU; u++) {
-                const int item = base + 4 * u;
-                if (item < nitems && lane < TCp) {
-                    const int grp = item / TRp;
-                    const int ty = item - grp * TRp;
+                if (base + 4 * u < nitems) {  // wave-uniform
+                    const bool keep = rowOn[u] && colIn;
                     V hv, lv;
 #pragma unroll
                     for (int e = 0; e < 8; e++) {
-                        const T h = (T)v[u][e];
+                        const float a = (keep && r0 + 8 * grpU[u] + e < m.R) ? v[u][e] : 0.f;
+                        const T h = (T)a;
                         hv[e] = h;
-                        if (SPLIT) lv[e] = (T)(v[u][e] - (float)h);
+                        if (SPLIT) lv[e] = (T)(a - (float)h);
                     }
-                    const size_t o = (size_t)(ty * TCp + lane) * LP_RCP + 8 * grp;
-                    *reinterpret_cast<V*>(sB + o) = hv;
-                    if (SPLIT) *reinterpret_cast<V*>(sB + (size_t)npix * LP_RCP + o) = lv;
+                    const bool inTile = lane < TCp;
+                    const size_t o = (size_t)(tyU[u] * TCp + lane) * LP_RCP + 8 * grpU[u];
+                    *reinterpret_cast<V*>(sB + (inTile ? o : sJunk + 8 * lane)) = hv;
+                    if (SPLIT) *reinterpret_cast<V*>(sB + (inTile ? (size_t)npix * LP_RCP + o : sJunk + 512 + 8 * lane)) = lv;
                 }
             }
         }
-        // Filter slice from the pre-packed bank: 16-byte units (term, t, m, q).
-        const int units = TERMS * TAPS * MT * 4;
-        for (int q = tid; q < units; q += 256) {
-            const int qq = q & 3;
-            const int rest = q >> 2;
-            const int mm = rest % MT;
-            const int rest2 = rest / MT;
-            const int t = rest2 % TAPS;
-            const int term = rest2 / TAPS;
-            const V val = *reinterpret_cast<const V*>(Wp + term * wTerm + ((long)t * Mpad + m0 + mm) * Rpad + r0 +
-                                                      8 * qq);
-            *reinterpret_cast<V*>(sA + ((size_t)(term * TAPS + t) * MT + mm) * LP_RCP + 8 * qq) = val;
+        // Filter slice from the pre-packed bank: 16-byte units (term, t, m, q),
+        // loaded in unconditional batches of WU (clamped unit index).
+        constexpr int UNITS = TERMS * TAPS * MT * 4;
+        constexpr int WU = 8;
+        for (int q0 = tid; q0 < UNITS; q0 += 256 * WU) {
+            V wv[WU];
+#pragma unroll
+            for (int i = 0; i < WU; i++) {
+                const int q = min(q0 + 256 * i, UNITS - 1);
+                const int qq = q & 3;
+                const int rest = q >> 2;
+                const int mm = rest % MT;
+                const int rest2 = rest / MT;
+                const int t = rest2 % TAPS;
+                const int term = rest2 / TAPS;
+                wv[i] = *reinterpret_cast<const V*>(Wp + term * wTerm + ((long)t * Mpad + m0 + mm) * Rpad + r0 + 8 * qq);
+            }
+#pragma unroll
+            for (int i = 0; i < WU; i++) {
+                const int q = q0 + 256 * i;
+                if (q < UNITS) {
+                    const int qq = q & 3;
+                    const int rest = q >> 2;
+                    const int mm = rest % MT;
+                    const int rest2 = rest / MT;
+                    const int t = rest2 % TAPS;
+                    const int term = rest2 / TAPS;
+                    *reinterpret_cast<V*>(sA + ((size_t)(term * TAPS + t) * MT + mm) * LP_RCP + 8 * qq) = wv[i];
+                }
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -281,7 +307,8 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
 
 size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision) {
     const int terms = precision == COGCONV_SPLIT_BF16 ? 2 : 1;
-    const size_t stage = (size_t)terms * ((size_t)c.tileRows * c.tileCols + (size_t)F * F * 16 * c.MB) * LP_RCP * 2;
+    const size_t stage = (size_t)terms * ((size_t)c.tileRows * c.tileCols + (size_t)F * F * 16 * c.MB) * LP_RCP * 2 +
+                         (size_t)terms * 64 * 8 * 2;  // scratch slots of the lanes beyond the tile width
     const size_t epi = (size_t)16 * c.MB * 260 * sizeof(float);
     return stage > epi ? stage : epi;
 }

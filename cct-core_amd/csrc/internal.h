@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "cogconv.h"
 
 namespace cogconv {
@@ -145,13 +147,13 @@ struct IgemmCfg {
 struct FaCfg {
     int K;          // dy planes per image (GEMM rows)
     int MB;         // 16-row blocks (K <= 16*MB <= 64)
-    int CBW;        // 16-column blocks per wave
+    int CBW;        // 16-column blocks per block (column group width), 3, 6 or 9
     int RR;         // tap rows staged per item
     int DWp;        // tap columns rounded up to 4
     int MTp;        // staged dy row pitch: 16*MB padded to 16 mod 32
     int xRows, xCols;
     int nCntMax;    // input planes staged per column group
-    int colGroups;  // column groups of 64*CBW columns
+    int colGroups;  // column groups of 16*CBW columns
     int rowChunks;  // ceil(FHe / RR)
     int splits;     // partial results
     size_t lds_bytes;
@@ -180,6 +182,13 @@ hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, cons
 hipError_t launch_block_reduce_sum(const float* in, float* out, int64_t points, int out_planes, int factor,
                                    hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_fill_value(float* p, size_t n, float v, hipStream_t s);
+
+// Candidate configurations of a plan (plan.cpp) and the autotuning cache
+// (tune.cpp): the tuned variant index for `desc` on the current device and the
+// description it had when it was timed.
+int plan_variant_count(const cogconv_desc* desc, int* count);
+bool tune_cache_lookup(const cogconv_desc* desc, int* variant, std::string* name);
 
 }  // namespace cogconv
 
@@ -196,4 +205,5 @@ struct cogconv_plan_s {
     size_t workspace;
     double flops;
     double bytes;
+    int variant;  // index into the plan's candidate list (autotuning)
 };

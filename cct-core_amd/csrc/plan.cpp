@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <string>
+#include <vector>
 
 #include "internal.h"
 
@@ -191,9 +193,9 @@ int analyse(const cogconv_desc* d, cogconv_field_type* out, Geom* gout, PlaneMap
     return COGCONV_OK;
 }
 
-DirectCfg choose_direct(const Geom& g, int nOut) {
+DirectCfg choose_direct(const Geom& g, int rgMax) {
     DirectCfg c;
-    c.RG = 4;
+    c.RG = rgMax;
     c.FCR = std::min(g.FHe, 16);
     c.FCC = std::min(g.FWe, 32);
     const size_t budget = 48 * 1024;
@@ -213,7 +215,6 @@ DirectCfg choose_direct(const Geom& g, int nOut) {
     }
     c.tilesR = (g.OH + 4 * c.RG - 1) / (4 * c.RG);
     c.tilesC = (g.OW + 63) / 64;
-    (void)nOut;
     return c;
 }
 
@@ -228,7 +229,8 @@ bool igemm_frame_eligible(const cogconv_desc* d, const Geom& g) {
     return igemm_f32_supported(g.FH);
 }
 
-IgemmCfg choose_igemm_frame(const Geom& g, const PlaneMap& m) {
+// mbOnly / rcOnly (0 = any) restrict the search to one output block or plane chunk.
+IgemmCfg choose_igemm_frame(const Geom& g, const PlaneMap& m, int mbOnly = 0, int rcOnly = 0) {
     const size_t limit = 80 * 1024;  // two blocks per CU (160 KB of LDS)
     IgemmCfg c;
     std::memset(&c, 0, sizeof c);
@@ -246,11 +248,16 @@ IgemmCfg choose_igemm_frame(const Geom& g, const PlaneMap& m) {
     // Largest output block (MB) and plane chunk (RC, a multiple of 8: two
     // plane groups of whole MFMA k-quads) whose staging fits in LDS.
     for (int MB = c.Mout <= 16 ? 1 : (c.Mout <= 32 ? 2 : 4); MB >= 1; MB = MB == 4 ? 2 : MB - 1) {
+        if (mbOnly && MB != mbOnly) {
+            if (MB == 1) break;
+            continue;
+        }
         int as = taps * 16 * MB;
         as += ((16 - as % 32) + 32) % 32;
         for (int rc = maxRC; rc >= 8; rc -= 8) {
             const int RC = rc;
-            const size_t stage = (size_t)RC * (ps + as) * sizeof(float);
+            if (rcOnly && RC != rcOnly) continue;
+            const size_t stage = ((size_t)RC * (ps + as) + 128) * sizeof(float);  // + scratch slots
             const size_t xch = igemm_f32_exchange_floats(MB) * sizeof(float);
             const size_t bytes = std::max(stage, xch);
             if (bytes <= limit) {
@@ -269,7 +276,10 @@ IgemmCfg choose_igemm_frame(const Geom& g, const PlaneMap& m) {
 }
 
 // FP32 MFMA split-reduction GEMM for FilterAdjointBlockReduceSum, BorderValid.
-bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out) {
+// cbw (0 = by size) fixes the column-group width; partials is the target
+// number of partial results (blocks) of the split reduction.
+bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out, int cbw = 0,
+                     int partials = 256) {
     if (d->mode != COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM || d->border != COGCONV_BORDER_VALID || d->in0.dims != 2)
         return false;
     FaCfg c;
@@ -279,35 +289,37 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     c.MB = (c.K + 15) / 16;
     const int P = g.OH * g.OW;
     const long ncols = (long)m.N * P;
-    c.CBW = ncols <= 192 ? 3 : 2;
     const long colBlocks = (ncols + 15) / 16;
-    c.colGroups = (int)((colBlocks + 4 * c.CBW - 1) / (4 * c.CBW));
-    c.nCntMax = std::min(m.N, (64 * c.CBW + P - 1) / P + 1);
+    c.CBW = cbw ? cbw : (colBlocks <= 3 ? 3 : (colBlocks <= 6 ? 6 : 9));
+    c.colGroups = (int)((colBlocks + c.CBW - 1) / c.CBW);
+    c.nCntMax = std::min(m.N, (16 * c.CBW + P - 1) / P + 1);
     c.DWp = (g.FWe + 3) / 4 * 4;
     c.xCols = (g.OW - 1) * g.dnC + c.DWp;
     if (c.DWp > 128 || c.xCols > 128) return false;  // staged rows are at most two 64-lane passes
     c.MTp = c.MB == 1 ? 16 : (c.MB == 4 ? 80 : 48);  // 16*MB padded to 16 mod 32 (k_fa_igemm_f32's MTP)
     if ((long)m.B * m.N * g.H * g.W >= (1L << 31) || (long)m.B * c.K * g.FH * g.FW >= (1L << 31)) return false;
+    const size_t red = (size_t)2 * c.MB * c.CBW * 4 * 64 * sizeof(float);  // cross-wave reduction buffer
     c.RR = 0;
-    for (int rr = std::min(g.FHe, 16); rr >= 1; rr--) {
+    for (int rr = std::min(g.FHe, 8); rr >= 1; rr--) {
         const int xRows = rr + (g.OH - 1) * g.dnR;
-        const size_t bytes = ((size_t)c.MTp * rr * c.DWp + (size_t)c.nCntMax * xRows * c.xCols) * sizeof(float);
-        if (bytes <= 48 * 1024) { c.RR = rr; break; }
+        const size_t bytes = ((size_t)c.MTp * rr * c.DWp + (size_t)c.nCntMax * xRows * c.xCols + 128) * sizeof(float);
+        if (std::max(bytes, red) <= 80 * 1024) { c.RR = rr; break; }
     }
     if (c.RR == 0) return false;
     c.xRows = c.RR + (g.OH - 1) * g.dnR;
-    c.lds_bytes = ((size_t)c.MTp * c.RR * c.DWp + (size_t)c.nCntMax * c.xRows * c.xCols) * sizeof(float);
-    c.lds_bytes = std::max(c.lds_bytes, (size_t)16 * c.MB * 64 * c.CBW * sizeof(float));  // epilogue tile
+    // dy rows, x rows and 128 scratch slots for masked-off stores
+    c.lds_bytes = ((size_t)c.MTp * c.RR * c.DWp + (size_t)c.nCntMax * c.xRows * c.xCols + 128) * sizeof(float);
+    c.lds_bytes = std::max(c.lds_bytes, red);
     c.rowChunks = (g.FHe + c.RR - 1) / c.RR;
     const long items = (long)m.B * c.rowChunks;
-    long want = (768 + c.colGroups - 1) / c.colGroups;
+    long want = (partials + c.colGroups - 1) / c.colGroups;
     c.splits = (int)std::max(1L, std::min(items, want));
     *out = c;
     return true;
 }
 
 // BF16 / FP16 / split-BF16 MFMA implicit GEMM for the frame block-reduce modes.
-bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, IgemmCfg* out) {
+bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, IgemmCfg* out, int mbOnly = 0) {
     if (d->mode != COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM && d->mode != COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM)
         return false;
     if (d->in0.dims != 2 || g.FH != g.FW || !igemm_lp_supported(g.FH) || g.upAr != 1 || g.upAc != 1 ||
@@ -327,7 +339,7 @@ bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ig
         c.MB = MB;
         c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
         c.lds_bytes = igemm_lp_lds_bytes(c, g.FH, d->precision);
-        if (c.lds_bytes <= 160 * 1024) {
+        if (c.lds_bytes <= 160 * 1024 && (!mbOnly || MB == mbOnly)) {
             *out = c;
             return true;
         }
@@ -336,27 +348,130 @@ bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ig
     return false;
 }
 
-SmallCfg choose_small(const Geom& g, const PlaneMap& m) {
+SmallCfg choose_small(const Geom& g, const PlaneMap& m, long blocks = 4096) {
     SmallCfg c;
     c.qchunks = (g.OH * g.OW + 15) / 16;
     const long rowsTotal = (long)m.R * g.FHe;
     // Aim for a few thousand blocks so that all 256 CUs are busy, with at least
     // ~8 filter rows of reduction per block.
-    long want = (4096 + (long)m.nOut * c.qchunks - 1) / ((long)m.nOut * c.qchunks);
+    long want = (blocks + (long)m.nOut * c.qchunks - 1) / ((long)m.nOut * c.qchunks);
     long maxs = std::max(1L, rowsTotal / 8);
     c.splits = (int)std::max(1L, std::min(want, maxs));
     return c;
 }
 
-}  // namespace
+// One candidate kernel configuration of a plan.
+struct Variant {
+    int kind;
+    DirectCfg d;
+    SmallCfg s;
+    IgemmCfg i;
+    FaCfg f;
+    size_t workspace;
+};
 
-extern "C" int cogconv_output_type(const cogconv_desc* desc, cogconv_field_type* out) {
-    clear_error();
-    return analyse(desc, out, nullptr, nullptr);
+void describe_into(const cogconv_plan_s* plan, char* buf, size_t len);
+
+// The candidate configurations for `desc`, the heuristic choice first.  The
+// list depends on the descriptor only, so an index into it is a stable
+// autotuning result (the tuner also keeps the description to check it).
+void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap& m, std::vector<Variant>& out) {
+    out.clear();
+    const long outPts = (long)g.OH * g.OW;
+    std::vector<std::string> seen;
+    auto add = [&](int kind, const DirectCfg* dc, const SmallCfg* sc, const IgemmCfg* ic, const FaCfg* fc) {
+        Variant v;
+        std::memset(&v, 0, sizeof v);
+        v.kind = kind;
+        if (dc) v.d = *dc;
+        if (sc) v.s = *sc;
+        if (ic) v.i = *ic;
+        if (fc) v.f = *fc;
+        switch (kind) {
+        case KK_IGEMM_LP: v.workspace = igemm_lp_workspace_bytes(v.i, g.FH, desc->precision); break;
+        case KK_FA_IGEMM: v.workspace = (size_t)v.f.splits * v.f.K * m.N * outPts * sizeof(float); break;
+        case KK_SMALL_OUTPUT: v.workspace = (size_t)v.s.splits * m.nOut * outPts * sizeof(float); break;
+        default: v.workspace = 0; break;
+        }
+        cogconv_plan_s tmp;
+        std::memset(&tmp, 0, sizeof tmp);
+        tmp.desc = *desc;
+        tmp.g = g;
+        tmp.m = m;
+        tmp.kind = v.kind;
+        tmp.dcfg = v.d;
+        tmp.scfg = v.s;
+        tmp.icfg = v.i;
+        tmp.fcfg = v.f;
+        char name[192];
+        describe_into(&tmp, name, sizeof name);
+        for (const std::string& n : seen)
+            if (n == name) return;
+        seen.emplace_back(name);
+        out.push_back(v);
+    };
+    if (desc->precision != COGCONV_FP32) {
+        IgemmCfg lp;
+        if (!choose_igemm_lp(desc, g, m, &lp)) return;
+        add(KK_IGEMM_LP, nullptr, nullptr, &lp, nullptr);
+        for (int mb : {4, 2, 1}) {
+            if (mb < lp.MB && choose_igemm_lp(desc, g, m, &lp, mb)) add(KK_IGEMM_LP, nullptr, nullptr, &lp, nullptr);
+        }
+        return;
+    }
+    const bool small_legal = outPts <= 4096;
+    const bool small_out = g.border == COGCONV_BORDER_VALID && is_fa(desc->mode) && outPts <= 256 &&
+                           (long)g.FHe * g.FWe >= 64;
+    const IgemmCfg ig = igemm_frame_eligible(desc, g) ? choose_igemm_frame(g, m) : IgemmCfg{};
+    const bool igemm_ok = ig.MB > 0;
+    FaCfg fa;
+    const bool fa_ok = choose_fa_igemm(desc, g, m, &fa);
+    const SmallCfg sm = choose_small(g, m);
+    const DirectCfg dr = choose_direct(g, 4);
+    // Heuristic default.
+    if (igemm_ok) add(KK_IGEMM, nullptr, nullptr, &ig, nullptr);
+    else if (fa_ok) add(KK_FA_IGEMM, nullptr, nullptr, nullptr, &fa);
+    else if (small_out) add(KK_SMALL_OUTPUT, nullptr, &sm, nullptr, nullptr);
+    else add(KK_DIRECT, &dr, nullptr, nullptr, nullptr);
+    // Alternatives.
+    if (igemm_ok) {
+        for (int mb : {4, 2, 1})
+            for (int rc : {32, 16, 8}) {
+                const IgemmCfg c = choose_igemm_frame(g, m, mb, rc);
+                if (c.MB > 0) add(KK_IGEMM, nullptr, nullptr, &c, nullptr);
+            }
+    }
+    if (fa_ok) {
+        for (int cbw : {3, 6, 9})
+            for (int parts : {128, 256, 512}) {
+                FaCfg c;
+                if (choose_fa_igemm(desc, g, m, &c, cbw, parts)) add(KK_FA_IGEMM, nullptr, nullptr, nullptr, &c);
+            }
+    }
+    if (small_legal && (long)m.R * g.FHe * g.FWe >= 64) {
+        for (long blocks : {2048L, 4096L, 8192L}) {
+            const SmallCfg c = choose_small(g, m, blocks);
+            add(KK_SMALL_OUTPUT, nullptr, &c, nullptr, nullptr);
+        }
+    }
+    for (int rg : {4, 2, 1}) {
+        const DirectCfg c = choose_direct(g, rg);
+        add(KK_DIRECT, &c, nullptr, nullptr, nullptr);
+    }
 }
 
-extern "C" int cogconv_plan_create(const cogconv_desc* desc, cogconv_plan* plan) {
-    clear_error();
+bool kind_matches_algo(int kind, int algo) {
+    switch (algo) {
+    case COGCONV_ALGO_DIRECT: return kind == KK_DIRECT;
+    case COGCONV_ALGO_SMALL_OUTPUT: return kind == KK_SMALL_OUTPUT;
+    case COGCONV_ALGO_IMPLICIT_GEMM: return kind == KK_IGEMM || kind == KK_FA_IGEMM || kind == KK_IGEMM_LP;
+    default: return true;
+    }
+}
+
+// Builds the plan for variant `vi` of `desc` (vi < 0: by algorithm hint,
+// then by the autotuning cache, then the heuristic default).
+int make_plan(const cogconv_desc* desc, int vi, cogconv_plan* plan) {
     if (!plan) return fail(COGCONV_ERR_INVALID_ARG, "null plan pointer");
     *plan = nullptr;
     cogconv_field_type out;
@@ -364,13 +479,51 @@ extern "C" int cogconv_plan_create(const cogconv_desc* desc, cogconv_plan* plan)
     PlaneMap m;
     int e = analyse(desc, &out, &g, &m);
     if (e) return e;
-    IgemmCfg lp;
-    if (desc->precision != COGCONV_FP32 && !choose_igemm_lp(desc, g, m, &lp))
+    std::vector<Variant> vars;
+    enumerate_variants(desc, g, m, vars);
+    if (vars.empty())
         return fail(COGCONV_ERR_UNSUPPORTED,
                     "precision %d is offered for ProjectFrame/BackProjectFrame with block reduction on 2-D fields with "
                     "a square 1/3/5/7 filter, no input up-sampling and a 32-column tile that fits one wavefront; "
                     "not for this %s",
                     desc->precision, mode_name(desc->mode));
+    const long outPts = (long)g.OH * g.OW;
+    int pick = -1;
+    if (vi >= 0) {
+        if (vi >= (int)vars.size())
+            return fail(COGCONV_ERR_INVALID_ARG, "variant %d out of range (%d variants)", vi, (int)vars.size());
+        pick = vi;
+    } else if (desc->algo != COGCONV_ALGO_AUTO) {
+        if (desc->algo == COGCONV_ALGO_SMALL_OUTPUT && outPts > 4096)
+            return fail(COGCONV_ERR_UNSUPPORTED, "small-output kernel needs an output plane of at most 4096 points");
+        for (int i = 0; i < (int)vars.size() && pick < 0; i++)
+            if (kind_matches_algo(vars[i].kind, desc->algo)) pick = i;
+        if (pick < 0)
+            return fail(COGCONV_ERR_UNSUPPORTED,
+                        "implicit GEMM needs a block-reduce frame mode with a square 1/3/5/7/9 filter on 2-D fields "
+                        "and no input up-sampling, or FilterAdjoint with block reduction, BorderValid and at most 64 "
+                        "planes per image in dy");
+    } else {
+        pick = 0;
+        int tuned;
+        std::string tunedName;
+        if (tune_cache_lookup(desc, &tuned, &tunedName) && tuned < (int)vars.size()) {
+            // Accept the cached index only if it still names the same configuration.
+            cogconv_plan_s tmp;
+            std::memset(&tmp, 0, sizeof tmp);
+            tmp.desc = *desc;
+            tmp.g = g;
+            tmp.m = m;
+            tmp.kind = vars[tuned].kind;
+            tmp.dcfg = vars[tuned].d;
+            tmp.scfg = vars[tuned].s;
+            tmp.icfg = vars[tuned].i;
+            tmp.fcfg = vars[tuned].f;
+            char name[192];
+            describe_into(&tmp, name, sizeof name);
+            if (tunedName == name) pick = tuned;
+        }
+    }
     cogconv_plan p = new (std::nothrow) cogconv_plan_s;
     if (!p) return fail(COGCONV_ERR_OOM, "host allocation failed");
     std::memset(p, 0, sizeof *p);
@@ -378,48 +531,14 @@ extern "C" int cogconv_plan_create(const cogconv_desc* desc, cogconv_plan* plan)
     p->out = out;
     p->g = g;
     p->m = m;
-
-    const long outPts = (long)g.OH * g.OW;
-    const bool small_out = g.border == COGCONV_BORDER_VALID && is_fa(desc->mode) && outPts <= 256 &&
-                           (long)g.FHe * g.FWe >= 64;
-    const int algo = desc->algo;
-    if (algo == COGCONV_ALGO_SMALL_OUTPUT && outPts > 4096) {
-        delete p;
-        return fail(COGCONV_ERR_UNSUPPORTED, "small-output kernel needs an output plane of at most 4096 points");
-    }
-    if (desc->precision != COGCONV_FP32) {
-        p->kind = KK_IGEMM_LP;
-        p->icfg = lp;
-        p->workspace = igemm_lp_workspace_bytes(lp, g.FH, desc->precision);
-    }
-    const bool igemm_ok = igemm_frame_eligible(desc, g) && choose_igemm_frame(g, m).MB > 0;
-    FaCfg probe_fa;
-    if (p->kind != KK_IGEMM_LP && algo == COGCONV_ALGO_IMPLICIT_GEMM && !igemm_ok &&
-        !choose_fa_igemm(desc, g, m, &probe_fa)) {
-        delete p;
-        return fail(COGCONV_ERR_UNSUPPORTED, "implicit GEMM needs a block-reduce frame mode with a square 1/3/5/7/9 "
-                                             "filter on 2-D fields and no input up-sampling, or FilterAdjoint with "
-                                             "block reduction, BorderValid and at most 64 planes per image in dy");
-    }
-    if (p->kind == KK_IGEMM_LP) {
-        // precision variant chosen above
-    } else if ((algo == COGCONV_ALGO_AUTO || algo == COGCONV_ALGO_IMPLICIT_GEMM) && igemm_ok) {
-        p->kind = KK_IGEMM;
-        p->icfg = choose_igemm_frame(g, m);
-        p->workspace = 0;
-    } else if ((algo == COGCONV_ALGO_AUTO || algo == COGCONV_ALGO_IMPLICIT_GEMM) &&
-               choose_fa_igemm(desc, g, m, &p->fcfg)) {
-        p->kind = KK_FA_IGEMM;
-        p->workspace = (size_t)p->fcfg.splits * p->fcfg.K * m.N * outPts * sizeof(float);
-    } else if (algo == COGCONV_ALGO_SMALL_OUTPUT || (algo == COGCONV_ALGO_AUTO && small_out)) {
-        p->kind = KK_SMALL_OUTPUT;
-        p->scfg = choose_small(g, m);
-        p->workspace = (size_t)p->scfg.splits * m.nOut * outPts * sizeof(float);
-    } else {
-        p->kind = KK_DIRECT;
-        p->dcfg = choose_direct(g, m.nOut);
-        p->workspace = 0;
-    }
+    const Variant& v = vars[pick];
+    p->kind = v.kind;
+    p->dcfg = v.d;
+    p->scfg = v.s;
+    p->icfg = v.i;
+    p->fcfg = v.f;
+    p->workspace = v.workspace;
+    p->variant = pick;
     // Algorithmic work: every multiply-add whose tap multiplies a stored
     // (non-inserted) sample.  Up-sampled operand 1 leaves FH*FW/(upAr*upAc)
     // such taps per output on average; FilterAdjoint up-sampling of operand 2
@@ -429,6 +548,53 @@ extern "C" int cogconv_plan_create(const cogconv_desc* desc, cogconv_plan* plan)
     p->bytes = 4.0 * ((double)desc->in0.planes * desc->in0.rows * desc->in0.cols +
                       (double)desc->in1.planes * desc->in1.rows * desc->in1.cols + (double)m.nOut * outPts);
     *plan = p;
+    return COGCONV_OK;
+}
+
+}  // namespace
+
+int cogconv::plan_variant_count(const cogconv_desc* desc, int* count) {
+    cogconv_field_type out;
+    Geom g;
+    PlaneMap m;
+    int e = analyse(desc, &out, &g, &m);
+    if (e) return e;
+    std::vector<Variant> vars;
+    enumerate_variants(desc, g, m, vars);
+    *count = (int)vars.size();
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_output_type(const cogconv_desc* desc, cogconv_field_type* out) {
+    clear_error();
+    return analyse(desc, out, nullptr, nullptr);
+}
+
+extern "C" int cogconv_plan_create(const cogconv_desc* desc, cogconv_plan* plan) {
+    clear_error();
+    return make_plan(desc, -1, plan);
+}
+
+extern "C" int cogconv_plan_variant_count(const cogconv_desc* desc, int32_t* count) {
+    clear_error();
+    if (!count) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    int n = 0;
+    const int e = plan_variant_count(desc, &n);
+    if (e) return e;
+    *count = n;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_plan_create_variant(const cogconv_desc* desc, int32_t variant, cogconv_plan* plan) {
+    clear_error();
+    if (variant < 0) return fail(COGCONV_ERR_INVALID_ARG, "negative variant index");
+    return make_plan(desc, variant, plan);
+}
+
+extern "C" int cogconv_plan_variant(cogconv_plan plan, int32_t* variant) {
+    clear_error();
+    if (!plan || !variant) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    *variant = plan->variant;
     return COGCONV_OK;
 }
 
@@ -466,9 +632,8 @@ extern "C" int cogconv_plan_bytes(cogconv_plan plan, double* bytes) {
     return COGCONV_OK;
 }
 
-extern "C" int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len) {
-    clear_error();
-    if (!plan || !buf || len == 0) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+namespace {
+void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
     const Geom& g = plan->g;
     switch (plan->kind) {
     case KK_DIRECT:
@@ -495,6 +660,13 @@ extern "C" int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len) {
                  plan->m.nOut, g.OH, g.OW, plan->icfg.MB, plan->icfg.RC, plan->icfg.lds_bytes);
         break;
     }
+}
+}  // namespace
+
+extern "C" int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len) {
+    clear_error();
+    if (!plan || !buf || len == 0) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    describe_into(plan, buf, len);
     return COGCONV_OK;
 }
 

@@ -218,6 +218,44 @@ int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len);
 int cogconv_launch(cogconv_plan plan, const float* in0, const float* in1, float* out, void* workspace,
                    cogconv_stream stream);
 
+/* ---- autotuning ----
+ * Counterpart of Profiler.bestVariant ($M/runtime/execution/Profiler.scala:131-190) and
+ * ProfilerCache ($M/runtime/execution/ProfilerCache.scala:39-122).  Every plan has a list of
+ * candidate kernel configurations (kernel family, output block, plane chunk, split count),
+ * the heuristic choice first.  cogconv_autotune times each candidate on synthetic inputs
+ * (input i of n filled with (i+1)/n, as the reference's profiler drives its circuits),
+ * writing a flush buffer before every timed step, picks the lowest average step time and
+ * records it per device and descriptor; plans built with COGCONV_ALGO_AUTO then use it.
+ * The cache persists to a text file (save / load).  Tuning allocates device memory and
+ * synchronizes `stream`: call it at graph-build time, not in a serving loop. */
+typedef struct cogconv_tune_options {
+    int32_t warmup_steps; /* untimed launches per candidate (Cog.profilerWarmupSteps); default 1   */
+    int32_t run_steps;    /* timed launches per candidate (Cog.profilerSteps); default 5            */
+    int64_t flush_bytes;  /* bytes written before each timed launch (Cog.profilerCacheFlushSizeMB);
+                             default 512 MiB, beyond the last-level cache                          */
+    int32_t force;        /* time again even if the cache holds a usable sample (Cog.forceProfiling) */
+} cogconv_tune_options;
+
+typedef struct cogconv_tune_result {
+    int32_t variant;       /* chosen candidate */
+    int32_t variants;      /* candidates considered */
+    double avg_us;         /* the chosen candidate's average launch time */
+    int32_t from_cache;    /* 1: an earlier sample was reused */
+    char description[192]; /* cogconv_plan_describe of the chosen candidate */
+} cogconv_tune_result;
+
+int cogconv_plan_variant_count(const cogconv_desc* desc, int32_t* count);
+/* A plan fixed to candidate `variant` (0 = the heuristic choice). */
+int cogconv_plan_create_variant(const cogconv_desc* desc, int32_t variant, cogconv_plan* plan);
+int cogconv_plan_variant(cogconv_plan plan, int32_t* variant);
+/* `options` may be NULL (defaults); `stream` may be NULL (default stream). */
+int cogconv_autotune(const cogconv_desc* desc, const cogconv_tune_options* options, cogconv_stream stream,
+                     cogconv_tune_result* result);
+int cogconv_tune_cache_clear(void);
+int cogconv_tune_cache_save(const char* path);
+/* Merges a saved cache; a missing file is not an error. */
+int cogconv_tune_cache_load(const char* path);
+
 /* blockReduceSum(field, factor) ($M/api/CogFunctionAPI.scala:755-756,
  * $HK/TensorReduceHyperKernel.scala:39-207): out plane p = sum_{j<factor} in plane p*factor+j.
  * `points` = rows*cols of one plane. */

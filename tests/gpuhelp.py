@@ -26,8 +26,8 @@ def desc_pair(mode, border, orient, sampling, step, batch, a_shape, k_shape, pre
     return od, cd
 
 
-def run_gpu(cdesc, a, k, stream=None):
-    plan = L.Plan(cdesc)
+def run_gpu(cdesc, a, k, stream=None, variant=None):
+    plan = L.Plan(cdesc, variant)
     ba = L.DeviceBuffer.from_array(a.astype(np.float32))
     bk = L.DeviceBuffer.from_array(k.astype(np.float32))
     bo = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
