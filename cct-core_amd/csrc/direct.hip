@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_direct(Geom g, PlaneMap m, DirectCfg c,
     extern __shared__ float smem[];
     float* st = smem;
     float* sf = smem + (size_t)c.ldsRows * c.ldsCols;
-    int bid = blockIdx.x;
+    int bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tc = bid % c.tilesC;
     bid /= c.tilesC;
     const int tr = bid % c.tilesR;

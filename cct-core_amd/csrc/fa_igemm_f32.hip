@@ -20,7 +20,7 @@
 // LDS.  Blocks split the (image, tap-row chunk) space; each writes one partial
 // dW, and the partials are added in a fixed order by a two-level reduction, so
 // results do not depend on scheduling.
-#include "internal.h"
+#include "stage.h"
 
 namespace cogconv {
 
@@ -39,8 +39,9 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
     const int FHW = g.FH * g.FW;                       // 32-bit offsets: the plan checks the sizes
     const int HWx = g.H * g.W;
 
-    const int cg = blockIdx.x % c.colGroups;
-    const int s = blockIdx.x / c.colGroups;
+    const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int cg = bid % c.colGroups;
+    const int s = bid / c.colGroups;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -112,47 +113,37 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         constexpr int MT = 16 * MB;    // dy planes the MFMAs read (K padded with zeros)
         const int dRows = MT * rr;     // (il, k); the pitch padding beyond MT is never read
         const float* DYb = DY + (size_t)b * K * FHW;
+        // Per-lane table (lane = staged tap row il < RR <= 8) of the dy row
+        // offsets, flip and up-sampling lattice applied, and the rows' mask.
+        int tiOffLane;
+        uint64_t tiMask;
+        {
+            int ti = i0 + lane;
+            if (g.flip) ti = g.FHe - 1 - ti;
+            bool on = lane < rr;
+            if (g.upFr > 1) {
+                on = on && (ti % g.upFr) == 0;
+                ti /= g.upFr;
+            }
+            tiOffLane = min(max(ti, 0), g.FH - 1) * g.FW;
+            tiMask = __ballot(on);
+        }
         for (int base = wave; base < dRows; base += 4 * U) {
-            int offU[U];
+            int offU[U], dstU[U];
             bool onU[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const int row = base + 4 * u;
-                const int il = row / MT;
-                const int k = row - il * MT;
-                int ti = i0 + il;
-                if (g.flip) ti = g.FHe - 1 - ti;
-                bool on = row < dRows && k < K;
-                if (g.upFr > 1) { on = on && (ti % g.upFr) == 0; ti /= g.upFr; }
-                ti = min(max(ti, 0), g.FH - 1);
-                offU[u] = min(k, K - 1) * FHW + ti * g.FW;
-                onU[u] = on;
+                const int il = min(row / MT, 63);
+                const int k = row - (row / MT) * MT;
+                offU[u] = min(k, K - 1) * FHW + __builtin_amdgcn_readlane(tiOffLane, il);
+                onU[u] = row < dRows && k < K && ((tiMask >> il) & 1);
+                dstU[u] = row < dRows ? il * c.DWp * MTP + k : -1;
             }
-            float v0[U], v1[U];
-            if (dTwo) {
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    v0[u] = DYb[offU[u] + dCol0];
-                    v1[u] = DYb[offU[u] + dCol1];
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < U; u++) v0[u] = DYb[offU[u] + dCol0];
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int row = base + 4 * u;
-                const bool live = row < dRows;  // wave-uniform
-                const int il = row / MT;
-                float* dst = sD + (size_t)il * c.DWp * MTP + (row - il * MT);
-                dst = (live && lane < c.DWp) ? dst + lane * MTP : sJunk + lane;
-                *dst = (onU[u] && dIn0) ? v0[u] : 0.f;
-                if (dTwo) {
-                    float* dst1 = sD + (size_t)il * c.DWp * MTP + (row - il * MT);
-                    dst1 = (live && lane + 64 < c.DWp) ? dst1 + (lane + 64) * MTP : sJunk + 64 + lane;
-                    *dst1 = (onU[u] && dIn1) ? v1[u] : 0.f;
-                }
-            }
+            if (dTwo)
+                stage_rows<true, U>(DYb, offU, onU, dstU, sD, sJunk, lane, c.DWp, MTP, dCol0, dCol1, dIn0, dIn1);
+            else
+                stage_rows<false, U>(DYb, offU, onU, dstU, sD, sJunk, lane, c.DWp, MTP, dCol0, dCol1, dIn0, dIn1);
         }
         const int xRowsAll = nCnt * c.xRows;  // (plane, row)
         const float* Xb = X + ((size_t)b * N + n_lo) * HWx;
@@ -173,27 +164,10 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
                     while (lrq >= c.xRows) { lrq -= c.xRows; nlq++; }
                 }
             }
-            float v0[U], v1[U];
-            if (xTwo) {
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    v0[u] = Xb[offU[u] + xCol0];
-                    v1[u] = Xb[offU[u] + xCol1];
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < U; u++) v0[u] = Xb[offU[u] + xCol0];
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const bool live = dstU[u] >= 0;  // wave-uniform
-                float* dst = (live && lane < c.xCols) ? sX + dstU[u] + lane : sJunk + lane;
-                *dst = (onU[u] && xIn0) ? v0[u] : 0.f;
-                if (xTwo) {
-                    float* dst1 = (live && lane + 64 < c.xCols) ? sX + dstU[u] + lane + 64 : sJunk + 64 + lane;
-                    *dst1 = (onU[u] && xIn1) ? v1[u] : 0.f;
-                }
-            }
+            if (xTwo)
+                stage_rows<true, U>(Xb, offU, onU, dstU, sX, sJunk, lane, c.xCols, 1, xCol0, xCol1, xIn0, xIn1);
+            else
+                stage_rows<false, U>(Xb, offU, onU, dstU, sX, sJunk, lane, c.xCols, 1, xCol0, xCol1, xIn0, xIn1);
         }
         __syncthreads();
         // Wave w reduces tap rows w, w+4, ... of the staged chunk for every

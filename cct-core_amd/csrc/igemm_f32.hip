@@ -20,7 +20,7 @@
 // filter slice [RC][F*F][16*MB] (each thread copies the F*F contiguous taps of
 // one (plane, filter) pair).  Both staged plane strides are 16 mod 32 words so
 // that the four k-lane groups of an MFMA read different LDS banks.
-#include "internal.h"
+#include "stage.h"
 
 namespace cogconv {
 
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
     const int aStride = c.aStride;  // floats per staged plane of the filter slice (TAPS*MT padded)
     float* sJunk = sA + (size_t)c.RC * aStride;  // 128 scratch slots for masked-off stores
 
-    int bid = blockIdx.x;
+    int bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tc = bid % c.tilesC;
     bid /= c.tilesC;
     const int tr = bid % c.tilesR;
@@ -85,6 +85,25 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
         colOff0 = xs[0]; colOff1 = xs[1];
         colIn0 = ins[0]; colIn1 = ins[1];
     }
+    // Per-lane table of the staged tile's row offsets (lane = tile row, at
+    // most 64 rows) with the border applied, and the rows' in-field mask: the
+    // staging loop reads them with one readlane / bit test per row.
+    int yOffLane;
+    uint64_t yMask;
+    {
+        int y = y0 + lane;
+        bool in = (unsigned)y < (unsigned)g.H;
+        if (g.border == COGCONV_BORDER_CYCLIC) {
+            y %= g.H;
+            y += y < 0 ? g.H : 0;
+            in = true;
+        } else {
+            in = in || g.border == COGCONV_BORDER_CLAMP;
+            y = min(max(y, 0), g.H - 1);
+        }
+        yOffLane = y * g.W;
+        yMask = __ballot(in);
+    }
     const int half = c.RC >> 1;
 
     floatx4 acc[MB][4];
@@ -121,45 +140,19 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
 #pragma unroll
                 for (int u = 0; u < IU; u++) {
                     const int r = r0 + rcq;
-                    int y = y0 + tyq;
-                    bool ym = (unsigned)y < (unsigned)g.H;
-                    if (g.border == COGCONV_BORDER_CYCLIC) {
-                        y %= g.H;
-                        y += y < 0 ? g.H : 0;
-                        ym = true;
-                    } else {
-                        ym = ym || g.border == COGCONV_BORDER_CLAMP;
-                        y = min(max(y, 0), g.H - 1);
-                    }
                     const bool live = base + 8 * u < nrows;
-                    onU[u] = ym && r < m.R && live;
-                    offU[u] = min(r, m.R - 1) * HW + y * g.W;
+                    const int ty = min(tyq, 63);  // tyq < TRp <= 64 for live items
+                    onU[u] = ((yMask >> ty) & 1) && r < m.R && live;
+                    offU[u] = min(r, m.R - 1) * HW + __builtin_amdgcn_readlane(yOffLane, ty);
                     dstU[u] = live ? rcq * c.planeStride + tyq * TCp : -1;
                     tyq += 8;
                     if (tyq >= TRp) { tyq -= TRp; rcq++; }
                 }
             }
-            float v0[IU], v1[IU];
-            if (twoPass) {
-#pragma unroll
-                for (int u = 0; u < IU; u++) {
-                    v0[u] = Xb[offU[u] + colOff0];
-                    v1[u] = Xb[offU[u] + colOff1];
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < IU; u++) v0[u] = Xb[offU[u] + colOff0];
-            }
-#pragma unroll
-            for (int u = 0; u < IU; u++) {
-                const bool live = dstU[u] >= 0;  // wave-uniform
-                float* dst = (live && lane < TCp) ? sB + dstU[u] + lane : sJunk + lane;
-                *dst = (onU[u] && colIn0) ? v0[u] : 0.f;
-                if (twoPass) {
-                    float* dst1 = (live && lane + 64 < TCp) ? sB + dstU[u] + lane + 64 : sJunk + 64 + lane;
-                    *dst1 = (onU[u] && colIn1) ? v1[u] : 0.f;
-                }
-            }
+            if (twoPass)
+                stage_rows<true, IU>(Xb, offU, onU, dstU, sB, sJunk, lane, TCp, 1, colOff0, colOff1, colIn0, colIn1);
+            else
+                stage_rows<false, IU>(Xb, offU, onU, dstU, sB, sJunk, lane, TCp, 1, colOff0, colOff1, colIn0, colIn1);
         }
         // Filter slice: thread -> (plane rc, filter mm); the F*F taps are
         // contiguous in W.  Loads are unconditional from a clamped pair.

@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     // width, after the filter slice: TERMS x 64 lanes x 8 elements.
     const size_t sJunk = (size_t)TERMS * npix * LP_RCP + (size_t)TERMS * TAPS * MT * LP_RCP;
 
-    int bid = blockIdx.x;
+    int bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tc = bid % c.tilesC;
     bid /= c.tilesC;
     const int tr = bid % c.tilesR;
@@ -145,6 +145,25 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
         colIn = in && lane < TCp;
     }
 
+    // Per-lane table of the staged tile's row offsets (lane = tile row, at
+    // most 64 rows) with the border applied, and the rows' in-field mask.
+    int yOffLane;
+    uint64_t yMask;
+    {
+        int y = y0 + lane;
+        bool in = (unsigned)y < (unsigned)g.H;
+        if (g.border == COGCONV_BORDER_CYCLIC) {
+            y %= g.H;
+            y += y < 0 ? g.H : 0;
+            in = true;
+        } else {
+            in = in || g.border == COGCONV_BORDER_CLAMP;
+            y = min(max(y, 0), g.H - 1);
+        }
+        yOffLane = y * g.W;
+        yMask = __ballot(in);
+    }
+
     floatx4 acc[MB][4];
 #pragma unroll
     for (int i = 0; i < MB; i++)
@@ -173,28 +192,22 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
             float v[IU][8];
             bool rowOn[IU];
             int grpU[IU], tyU[IU];
+            int grp = base / TRp;  // wave-uniform; TRp >= 8, so one wrap per step of 4
+            int ty = base - grp * TRp;
 #pragma unroll
             for (int u = 0; u < IU; u++) {
                 const int item = base + 4 * u;
-                const int grp = item / TRp;  // wave-uniform
-                const int ty = item - grp * TRp;
-                int y = y0 + ty;
-                bool ym = (unsigned)y < (unsigned)g.H;
-                if (g.border == COGCONV_BORDER_CYCLIC) {
-                    y %= g.H;
-                    y += y < 0 ? g.H : 0;
-                    ym = true;
-                } else {
-                    ym = ym || g.border == COGCONV_BORDER_CLAMP;
-                    y = min(max(y, 0), g.H - 1);
-                }
-                rowOn[u] = item < nitems && ym;
+                const int tyc = min(ty, 63);
+                rowOn[u] = item < nitems && ((yMask >> tyc) & 1);
                 grpU[u] = grp;
                 tyU[u] = ty;
-                const float* src = Xb + y * g.W + colOff;
+                const float* src = Xb + __builtin_amdgcn_readlane(yOffLane, tyc) + colOff;
 #pragma unroll
                 for (int e = 0; e < 8; e++) v[u][e] = src[min(r0 + 8 * grp + e, m.R - 1) * HW];
+                ty += 4;
+                if (ty >= TRp) { ty -= TRp; grp++; }
             }
+            __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll
             for (int u = 0; u < IU; u++) {
                 if (base + 4 * u < nitems) {  // wave-uniform

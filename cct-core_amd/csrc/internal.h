@@ -93,6 +93,16 @@ __host__ __device__ inline void plane_pair(const PlaneMap& m, int o, int r, int&
     }
 }
 
+// Block index remapped so that each of the 8 XCDs (which take launched blocks
+// round-robin: block i runs on XCD i % 8) works on a contiguous range of tiles,
+// whose shared halo rows and filter slices then hit the same L2.  Bijective on
+// [0, n): the first 8*floor(n/8) blocks are permuted, the tail keeps its index.
+__device__ __forceinline__ int xcd_swizzle(int bid, int n) {
+    const int per = n >> 3;
+    if (bid >= per * 8) return bid;
+    return (bid & 7) * per + (bid >> 3);
+}
+
 // n / d for 0 <= n < 2^22 and 1 <= d, from a float reciprocal plus one
 // correction step each way (exact in that range).  Used by the staging loops
 // in place of an integer division.

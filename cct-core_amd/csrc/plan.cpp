@@ -225,6 +225,7 @@ bool igemm_frame_eligible(const cogconv_desc* d, const Geom& g) {
         return false;
     if (d->in0.dims != 2 || g.FH != g.FW || g.upAr != 1 || g.upAc != 1 || g.dnR != g.dnC) return false;
     if (31 * g.dnC + g.FW > 128) return false;  // staged tile rows are at most two 64-lane passes
+    if (7 * g.dnR + g.FH > 64) return false;    // one lane per staged tile row in the row table
     if ((long)d->in0.planes * g.H * g.W >= (1L << 31)) return false;  // 32-bit staging offsets
     return igemm_f32_supported(g.FH);
 }
@@ -326,6 +327,7 @@ bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ig
         g.dnR != g.dnC)
         return false;
     if (31 * g.dnC + g.FW > 64) return false;  // one 64-lane pass per staged row
+    if (7 * g.dnR + g.FH > 64) return false;   // one lane per staged tile row in the row table
     if ((long)d->in0.planes * g.H * g.W >= (1L << 31)) return false;
     IgemmCfg c;
     std::memset(&c, 0, sizeof c);
