@@ -244,63 +244,46 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
     }
 }
 
-// Sum `splits` partial arrays of `total` floats in a fixed order.  Groups of
-// 16 partials are first added in place (thread (group, idx) owns those 16
-// slots), then the group sums are added into `out`.
-__global__ void k_reduce_groups(float* __restrict__ ws, int splits, long total) {
-    const int groups = (splits + 15) / 16;
-    const long n = (long)groups * total;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
-        const int grp = (int)(t / total);
-        const long idx = t - (long)grp * total;
-        const int s0 = grp * 16;
-        const int cnt = min(16, splits - s0);
+// Sum `splits` partial arrays of `total` floats in a fixed order, in one
+// launch.  A block owns 64 consecutive outputs; its G = 16 waves each add a
+// contiguous run of ceil(splits / 16) partials (loads batched 16 at a time),
+// and the 16 wave sums are then added in wave order.  The order depends on
+// `splits` only, so results do not depend on scheduling.
+constexpr int RED_G = 16;
+
+__global__ __launch_bounds__(64 * RED_G) void k_reduce_partials(const float* __restrict__ ws, float* __restrict__ out,
+                                                                  int splits, long total) {
+    __shared__ float part[RED_G][64];
+    const int lane = threadIdx.x & 63;
+    const int grp = threadIdx.x >> 6;
+    const long idx = (long)blockIdx.x * 64 + lane;
+    const long safe = idx < total ? idx : total - 1;
+    const int per = (splits + RED_G - 1) / RED_G;
+    const int s0 = grp * per;
+    const int s1 = min(splits, s0 + per);
+    float acc = 0.f;
+    for (int sb = s0; sb < s1; sb += 16) {
         float v[16];
 #pragma unroll
-        for (int i = 0; i < 16; i++) v[i] = i < cnt ? ws[(size_t)(s0 + i) * total + idx] : 0.f;
-        float acc = 0.f;
+        for (int i = 0; i < 16; i++) v[i] = ws[(size_t)min(sb + i, splits - 1) * total + safe];
 #pragma unroll
-        for (int i = 0; i < 16; i++) acc += v[i];
-        ws[(size_t)s0 * total + idx] = acc;
+        for (int i = 0; i < 16; i++)
+            if (sb + i < s1) acc += v[i];
+    }
+    part[grp][lane] = acc;
+    __syncthreads();
+    if (grp == 0 && idx < total) {
+        float t = part[0][lane];
+#pragma unroll
+        for (int gi = 1; gi < RED_G; gi++) t += part[gi][lane];
+        out[idx] = t;
     }
 }
 
-__global__ void k_reduce_final(const float* __restrict__ ws, float* __restrict__ out, int splits, long total) {
-    for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-        float acc = 0.f;
-        for (int s0 = 0; s0 < splits; s0 += 256) {
-            float v[16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int s = s0 + 16 * i;
-                v[i] = s < splits ? ws[(size_t)s * total + idx] : 0.f;
-            }
-#pragma unroll
-            for (int i = 0; i < 16; i++) acc += v[i];
-        }
-        out[idx] = acc;
-    }
-}
-
-hipError_t launch_reduce_partials(float* ws, float* out, int splits, long total, hipStream_t s) {
-    if (splits > 16) {
-        const long n = (long)((splits + 15) / 16) * total;
-        const long blocks = std::min((n + 255) / 256, 65536L);
-        hipLaunchKernelGGL(k_reduce_groups, dim3((unsigned)blocks), dim3(256), 0, s, ws, splits, total);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    const long blocks = std::min((total + 255) / 256, 65536L);
-    if (splits > 16) {
-        hipLaunchKernelGGL(k_reduce_final, dim3((unsigned)blocks), dim3(256), 0, s, ws, out, splits, total);
-    } else {
-        // 16 or fewer partials: one group, summed directly.
-        hipLaunchKernelGGL(k_reduce_groups, dim3((unsigned)std::min((total + 255) / 256, 65536L)), dim3(256), 0, s,
-                           ws, splits, total);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_reduce_final, dim3((unsigned)blocks), dim3(256), 0, s, ws, out, 1, total);
-    }
+hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long total, hipStream_t s) {
+    const long blocks = (total + 63) / 64;
+    if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)blocks), dim3(64 * RED_G), 0, s, ws, out, splits, total);
     return hipGetLastError();
 }
 
