@@ -85,10 +85,11 @@ def cpu_baseline(sample_images=16, budget_s=10.0):
 
 
 def precision_workload(args):
-    """B32 128x128x64 planes, 64 filters 3x3x64, BorderZero: ProjectFrame+BRS and
-    BackProjectFrame+BRS in FP32, BF16, FP16 and split-BF16; GFLOP/s from HIP
-    events, errors against the FP64 oracle on a 2-image slice (uniform[-1,1)
-    and wide-range 10^U(-3,3) with random signs)."""
+    """B32 128x128x64 planes, 64 filters 3x3x64: ProjectFrame+BRS and
+    BackProjectFrame+BRS (BorderZero) and FilterAdjoint+BRS (BorderValid, dy
+    126x126x64) in FP32, BF16, FP16 and split-BF16; GFLOP/s from HIP events,
+    errors against the FP64 oracle on a 2-image slice (uniform[-1,1) and
+    wide-range 10^U(-3,3) with random signs)."""
     import cogconv._lib as L
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -98,27 +99,30 @@ def precision_workload(args):
     x = rng.uniform(-1, 1, (B * N, H, H)).astype(np.float32)
     w = rng.uniform(-1, 1, (K * N, 3, 3)).astype(np.float32)
     dy = rng.uniform(-1, 1, (B * K, H, H)).astype(np.float32)
-    bx, bw, bdy = (L.DeviceBuffer.from_array(a) for a in (x, w, dy))
-    f = lambda planes: (2, H, H, planes)  # noqa: E731
+    dyv = rng.uniform(-1, 1, (B * K, H - 2, H - 2)).astype(np.float32)  # output gradient of a Valid layer
+    bx, bw, bdy, bdyv = (L.DeviceBuffer.from_array(a) for a in (x, w, dy, dyv))
+    f = lambda planes, side=H: (2, side, side, planes)  # noqa: E731
     stream = L.Stream()
     names = {L.FP32: "fp32", L.BF16: "bf16", L.FP16: "fp16", L.SPLIT_BF16: "split_bf16"}
     res = {}
     for prec in (L.FP32, L.BF16, L.FP16, L.SPLIT_BF16):
-        for opname, mode, orient, a, ab in (("projectFrame+BRS", L.PROJECT_FRAME_BLOCK_REDUCE_SUM,
-                                             L.CROSS_CORRELATION, x, bx),
-                                            ("backProjectFrame+BRS", L.BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM,
-                                             L.CONVOLUTION, dy, bdy)):
-            plan = L.Plan(L.make_desc(mode, L.BORDER_ZERO, orient, L.NO_SAMPLING, 1, B, f(a.shape[0]),
-                                      (2, 3, 3, K * N), prec))
+        for opname, mode, border, orient, a, ab, t1, b1 in (
+                ("projectFrame+BRS", L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CROSS_CORRELATION, x, bx,
+                 (2, 3, 3, K * N), bw),
+                ("backProjectFrame+BRS", L.BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CONVOLUTION, dy, bdy,
+                 (2, 3, 3, K * N), bw),
+                ("filterAdjoint+BRS", L.FILTER_ADJOINT_BLOCK_REDUCE_SUM, L.BORDER_VALID, L.CROSS_CORRELATION, x, bx,
+                 f(B * K, H - 2), bdyv)):
+            plan = L.Plan(L.make_desc(mode, border, orient, L.NO_SAMPLING, 1, B, f(a.shape[0]), t1, prec))
             out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
             ws = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
             for _ in range(3):
-                plan.launch(ab, bw, out, ws, stream)
+                plan.launch(ab, b1, out, ws, stream)
             e0, e1 = L.Event(), L.Event()
             reps = 20
             e0.record(stream)
             for _ in range(reps):
-                plan.launch(ab, bw, out, ws, stream)
+                plan.launch(ab, b1, out, ws, stream)
             e1.record(stream)
             e1.synchronize()
             ms = e0.elapsed_ms(e1) / reps
@@ -134,26 +138,38 @@ def precision_workload(args):
                        ("wide", lambda sh: 10.0 ** rng.uniform(-3, 3, sh) * rng.choice([-1.0, 1.0], sh))):
         xs = gen((2 * N, H, H)).astype(np.float32)
         ws_ = gen((K * N, 3, 3)).astype(np.float32)
+        dys = gen((2 * K, H - 2, H - 2)).astype(np.float32)
         od = O.make_desc(O.PF_BRS, O.ZERO, O.XCORR, O.NONE, 1, 2, xs.shape, ws_.shape)
         ref = O.conv(od, xs, ws_, threads=16)
+        odf = O.make_desc(O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, 2, xs.shape, dys.shape)
+        reff = O.conv(odf, xs, dys, threads=16)
         for prec in (L.FP32, L.BF16, L.FP16, L.SPLIT_BF16):
-            plan = L.Plan(L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CROSS_CORRELATION,
-                                      L.NO_SAMPLING, 1, 2, f(2 * N), (2, 3, 3, K * N), prec))
-            a, k = L.DeviceBuffer.from_array(xs), L.DeviceBuffer.from_array(ws_)
-            out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
-            wsb = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
-            plan.launch(a, k, out, wsb, None)
-            L.synchronize()
-            got = out.download(plan.out_shape).astype(np.float64)
-            d = np.abs(got - ref)
-            errs.setdefault(names[prec], {})[label] = {
-                "max_abs": float(d.max()), "max_rel_normwise": float(d.max() / np.abs(ref).max()),
-                "rel_l2": float(np.sqrt((d ** 2).sum() / (ref ** 2).sum()))}
+            for opname, mode, border, in1, r in (
+                    ("projectFrame+BRS", L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, ws_, ref),
+                    ("filterAdjoint+BRS", L.FILTER_ADJOINT_BLOCK_REDUCE_SUM, L.BORDER_VALID, dys, reff)):
+                plan = L.Plan(L.make_desc(mode, border, L.CROSS_CORRELATION, L.NO_SAMPLING, 1, 2, f(2 * N),
+                                          (2, in1.shape[1], in1.shape[2], in1.shape[0]), prec))
+                a, k = L.DeviceBuffer.from_array(xs), L.DeviceBuffer.from_array(in1)
+                out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
+                wsb = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
+                plan.launch(a, k, out, wsb, None)
+                L.synchronize()
+                got = out.download(plan.out_shape).astype(np.float64)
+                d = np.abs(got - r)
+                errs.setdefault(names[prec], {}).setdefault(label, {})[opname] = {
+                    "max_abs": float(d.max()), "max_rel_normwise": float(d.max() / np.abs(r).max()),
+                    "rel_l2": float(np.sqrt((d ** 2).sum() / (r ** 2).sum()))}
+                for b_ in (a, k, out, wsb):
+                    if b_ is not None:
+                        b_.close()
     fp32 = res["fp32"]["projectFrame+BRS"]["gflops"]
     line = {"metric": "conv GFLOP/s & % roofline (FP32 vs BF16-MFMA); p50/p99 request latency",
-            "workload": "precision variants: batch 32 images 128x128x64, 64 filters 3x3x64, BorderZero",
+            "workload": "precision variants: batch 32 images 128x128x64, 64 filters 3x3x64; PF/BPF BorderZero, "
+                        "FA BorderValid (dy 126x126x64)",
             "results": res, "errors_vs_fp64": errs,
             "bf16_over_fp32_projectFrame": round(res["bf16"]["projectFrame+BRS"]["gflops"] / fp32, 2),
+            "bf16_over_fp32_filterAdjoint": round(res["bf16"]["filterAdjoint+BRS"]["gflops"] /
+                                                  res["fp32"]["filterAdjoint+BRS"]["gflops"], 2),
             "peaks": {"fp32_tflops": PEAK_FP32_TFLOPS, "bf16_tflops": PEAK_BF16_TFLOPS, "hbm_gbs": PEAK_HBM_GBS}}
     print(json.dumps(line), flush=True)
 

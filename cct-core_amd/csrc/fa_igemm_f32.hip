@@ -48,27 +48,12 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
     const int g4 = lane >> 4, l16 = lane & 15;
     const int N = m.N, K = c.K;
     // Per-lane column offsets (two 64-lane passes at most) of the staged dy
-    // rows (flip and up-sampling lattice applied) and x rows.
+    // rows (flip and up-sampling lattice applied) and x rows; they depend on
+    // the item's tap-column chunk and are set at the start of each item.
     const bool dTwo = c.DWp > 64, xTwo = c.xCols > 64;
-    int dCol0, dCol1, xCol0, xCol1;
-    bool dIn0, dIn1, xIn0, xIn1;
-    {
-        int dc[2], xc[2];
-        bool di[2], xi[2];
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            const int jj = lane + 64 * p;
-            int tj = g.flip ? g.FWe - 1 - jj : jj;
-            bool on = jj < g.FWe;
-            if (g.upFc > 1) { on = on && (tj % g.upFc) == 0; tj /= g.upFc; }
-            dc[p] = min(max(tj, 0), g.FW - 1);
-            di[p] = on;
-            xc[p] = min(jj, g.W - 1);
-            xi[p] = jj < g.W;
-        }
-        dCol0 = dc[0]; dCol1 = dc[1]; dIn0 = di[0]; dIn1 = di[1];
-        xCol0 = xc[0]; xCol1 = xc[1]; xIn0 = xi[0]; xIn1 = xi[1];
-    }
+    int dCol0 = 0, dCol1 = 0, xCol0 = 0, xCol1 = 0;
+    bool dIn0 = false, dIn1 = false, xIn0 = false, xIn1 = false;
+    int jChunk = -1;  // tap-column chunk the offsets are set for
     const int P = g.OH * g.OW;
     const int Ncols = N * P;
     const int colBase = cg * CBG * 16;
@@ -96,12 +81,35 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
 #pragma unroll
         for (int q = 0; q < CBG; q++) acc[i][q] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    const long items = (long)m.B * c.rowChunks;
+    // Items: (image b, tap-column chunk, tap-row chunk), row chunks fastest.
+    const long items = (long)m.B * c.colChunks * c.rowChunks;
     const long it0 = items * s / c.splits, it1 = items * (s + 1) / c.splits;
     for (long it = it0; it < it1; it++) {
-        const int b = (int)(it / c.rowChunks);
-        const int i0 = (int)(it - (long)b * c.rowChunks) * c.RR;
+        const int b = (int)(it / ((long)c.colChunks * c.rowChunks));
+        const int rest = (int)(it - (long)b * c.colChunks * c.rowChunks);
+        const int cc = rest / c.rowChunks;
+        const int i0 = (rest - cc * c.rowChunks) * c.RR;
         const int rr = min(c.RR, g.FHe - i0);
+        if (cc != jChunk) {  // wave-uniform
+            jChunk = cc;
+            const int j0 = cc * c.CW;
+            const int jEnd = min(j0 + c.CW, g.FWe);
+            int dc[2], xc[2];
+            bool di[2], xi[2];
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int jj = j0 + lane + 64 * p;
+                int tj = g.flip ? g.FWe - 1 - jj : jj;
+                bool on = jj < jEnd;
+                if (g.upFc > 1) { on = on && (tj % g.upFc) == 0; tj /= g.upFc; }
+                dc[p] = min(max(tj, 0), g.FW - 1);
+                di[p] = on;
+                xc[p] = min(jj, g.W - 1);
+                xi[p] = jj < g.W;
+            }
+            dCol0 = dc[0]; dCol1 = dc[1]; dIn0 = di[0]; dIn1 = di[1];
+            xCol0 = xc[0]; xCol1 = xc[1]; xIn0 = xi[0]; xIn1 = xi[1];
+        }
         __syncthreads();
         // Staging of the dy tap rows and x rows: one row per wave pass, lanes
         // across columns (at most 128).  Per batch of U rows: the rows' offsets

@@ -1,0 +1,350 @@
+// Precision variants of FilterAdjointBlockReduceSum with BorderValid (the
+// weight gradient of a layer): BF16 / FP16 MFMA with FP32 accumulation, and
+// split-BF16 (three-term compensated products).  Same contraction as
+// fa_igemm_f32.hip:
+//
+//   dW[k*N + n](R, C) = sum_b sum_{i,j} tap_{b,k}(i, j) * X_{b*N + n}(R*dn + i, C*dn + j)
+//
+// (ConvolveHyperKernel.scala:1310-1336; ConvolveToSmallFieldPipelinedTiledHyperKernel.scala:45-127.)
+//
+// gfx950 mapping: v_mfma_f32_16x16x32_{bf16,f16}.  The 32-deep k slice is 32
+// consecutive tap columns of one tap row; lane l holds A[l%16][8*(l/16) .. +7]
+// (dy plane l%16, eight tap columns) and B[8*(l/16) .. +7][l%16] (the x
+// samples under those taps for output column l%16).  Both are 16-byte LDS
+// reads: dy rows are staged [tap row][plane][column]; x rows are staged once
+// per output column C, shifted left by C*dn, so that the x samples of output
+// (n, R, C) under tap column t sit at column t of copy C.  Operands arrive in
+// FP32 and are rounded while staged.  A block (4 wave64) owns a column group
+// of CBG 16-column blocks and all MB row blocks; wave w reduces tap rows w,
+// w+4, .. of each item (image, tap-column chunk, tap-row chunk); the four
+// waves' sums meet in LDS and one partial dW per block is written, added
+// afterwards in a fixed order (launch_reduce_partials).
+#include "lp.h"
+
+namespace cogconv {
+
+hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long total, hipStream_t s);
+
+template <int MB, int CBG, int PREC>
+__global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
+                                                     const float* __restrict__ DY, float* __restrict__ ws) {
+    typedef typename LpType<PREC>::T T;
+    typedef typename LpType<PREC>::V V;
+    constexpr bool SPLIT = PREC == COGCONV_SPLIT_BF16;
+    constexpr int TERMS = SPLIT ? 2 : 1;
+    constexpr int MT = 16 * MB;
+    extern __shared__ __attribute__((aligned(16))) unsigned char fa_lp_smem[];
+    const int OW = g.OW;
+    const int DP = c.DWp + 8;  // LDS row pitch in elements (DWp is 32, 64 or 128)
+    T* sD = reinterpret_cast<T*>(fa_lp_smem);                  // [TERMS][RR][MT][DP]
+    const size_t dTerm = (size_t)c.RR * MT * DP;
+    T* sX = sD + TERMS * dTerm;                                  // [TERMS][nCntMax][xRows][OW][DP]
+    const size_t xTerm = (size_t)c.nCntMax * c.xRows * OW * DP;
+    T* sJunk = sX + TERMS * xTerm;                               // [TERMS][64 lanes][8]
+    const int FHW = g.FH * g.FW;
+    const int HWx = g.H * g.W;
+
+    const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int cg = bid % c.colGroups;
+    const int s = bid / c.colGroups;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g4 = lane >> 4, l16 = lane & 15;
+    const int N = m.N, K = c.K;
+    const int P = g.OH * OW;
+    const int Ncols = N * P;
+    const int colBase = cg * CBG * 16;
+    const int n_lo = colBase / P;
+    const int n_hi = min(N - 1, (min(colBase + CBG * 16, Ncols) - 1) / P);
+    const int nCnt = n_hi - n_lo + 1;
+
+    // B-operand offset of each column block: copy C of x row R*dn (+ tap row).
+    int xoff[CBG];
+#pragma unroll
+    for (int q = 0; q < CBG; q++) {
+        const int j = colBase + q * 16 + l16;
+        int off = 0;
+        if (j < Ncols) {
+            const int n = j / P, rem = j - n * P;
+            const int R = rem / OW, C = rem - R * OW;
+            off = (((n - n_lo) * c.xRows + R * g.dnR) * OW + C) * DP;
+        }
+        xoff[q] = off + 8 * g4;
+    }
+
+    floatx4 acc[MB][CBG];
+#pragma unroll
+    for (int i = 0; i < MB; i++)
+#pragma unroll
+        for (int q = 0; q < CBG; q++) acc[i][q] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    // Staging lanes: LPR lanes per row, each holding 8 consecutive columns.
+    const int LPR = c.DWp >> 3;
+    const int RPP = 64 / LPR;  // rows per wave pass
+    const int myRow = lane / LPR, cgrp = lane - myRow * LPR;
+    const float invOW = 1.0f / OW, invXR = 1.0f / c.xRows;
+    int dColE[8];
+    unsigned dInMask = 0;
+    int jChunk = -1, j0 = 0;
+
+    const long items = (long)m.B * c.colChunks * c.rowChunks;
+    const long it0 = items * s / c.splits, it1 = items * (s + 1) / c.splits;
+    for (long it = it0; it < it1; it++) {
+        const int b = (int)(it / ((long)c.colChunks * c.rowChunks));
+        const int rest = (int)(it - (long)b * c.colChunks * c.rowChunks);
+        const int cc = rest / c.rowChunks;
+        const int i0 = (rest - cc * c.rowChunks) * c.RR;
+        const int rr = min(c.RR, g.FHe - i0);
+        if (cc != jChunk) {  // wave-uniform
+            jChunk = cc;
+            j0 = cc * c.CW;
+            const int jEnd = min(j0 + c.CW, g.FWe);
+            dInMask = 0;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int jj = j0 + 8 * cgrp + e;
+                int tj = g.flip ? g.FWe - 1 - jj : jj;
+                bool on = jj < jEnd;
+                if (g.upFc > 1) {
+                    on = on && (tj % g.upFc) == 0;
+                    tj /= g.upFc;
+                }
+                dColE[e] = min(max(tj, 0), g.FW - 1);
+                dInMask |= on ? 1u << e : 0u;
+            }
+        }
+        // Per-lane table (lane = staged tap row il < RR <= 8) of the dy row
+        // offsets, flip and up-sampling lattice applied, and the rows' mask.
+        int tiOffLane;
+        uint64_t tiMask;
+        {
+            int ti = i0 + lane;
+            if (g.flip) ti = g.FHe - 1 - ti;
+            bool on = lane < rr;
+            if (g.upFr > 1) {
+                on = on && (ti % g.upFr) == 0;
+                ti /= g.upFr;
+            }
+            tiOffLane = min(max(ti, 0), g.FH - 1) * g.FW;
+            tiMask = __ballot(on);
+        }
+        __syncthreads();
+        constexpr int DU = 4;  // wave passes per batch: 32 loads in flight per lane
+        // dy rows (il, k), k fastest; a pass's rows share il (RPP divides MT).
+        const int dRows = MT * rr;
+        const float* DYb = DY + (size_t)b * K * FHW;
+        for (int pb = wave * RPP; pb < dRows; pb += 4 * RPP * DU) {
+            float v[DU][8];
+            bool onD[DU];
+            int dstD[DU];
+#pragma unroll
+            for (int d = 0; d < DU; d++) {
+                const int p0 = pb + d * 4 * RPP;            // wave-uniform
+                const int il = min(p0 / MT, 7);
+                const int row = p0 + myRow;
+                const int k = row - (p0 / MT) * MT;
+                const bool live = p0 < dRows;
+                onD[d] = live && k < K && ((tiMask >> il) & 1);
+                dstD[d] = live ? (il * MT + k) * DP + 8 * cgrp : -1;
+                const float* src = DYb + min(k, K - 1) * FHW + __builtin_amdgcn_readlane(tiOffLane, il);
+#pragma unroll
+                for (int e = 0; e < 8; e++) v[d][e] = src[dColE[e]];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
+#pragma unroll
+            for (int d = 0; d < DU; d++) {
+                V hv, lv;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const float a = (onD[d] && ((dInMask >> e) & 1)) ? v[d][e] : 0.f;
+                    const T h = (T)a;
+                    hv[e] = h;
+                    if (SPLIT) lv[e] = (T)(a - (float)h);
+                }
+                const bool live = dstD[d] >= 0;
+                *reinterpret_cast<V*>(live ? sD + dstD[d] : sJunk + 8 * lane) = hv;
+                if (SPLIT) *reinterpret_cast<V*>(live ? sD + dTerm + dstD[d] : sJunk + 512 + 8 * lane) = lv;
+            }
+        }
+        // x rows (nl, r, C), C fastest: copy C holds x row i0 + r of plane
+        // n_lo + nl from column j0 + C*dn on.
+        const int xRowsAll = nCnt * c.xRows * OW;
+        const float* Xb = X + ((size_t)b * N + n_lo) * HWx;
+        for (int pb = wave * RPP; pb < xRowsAll; pb += 4 * RPP * DU) {
+            float v[DU][8];
+            bool onX[DU];
+            int dstX[DU];
+            unsigned colMask[DU];
+#pragma unroll
+            for (int d = 0; d < DU; d++) {
+                const int row = pb + d * 4 * RPP + myRow;
+                const bool live = row < xRowsAll;
+                const int t = fast_div(row, OW, invOW);
+                const int C = row - t * OW;
+                const int nl = fast_div(t, c.xRows, invXR);
+                const int r = t - nl * c.xRows;
+                const int gy = i0 + r;
+                onX[d] = live && gy < g.H;
+                dstX[d] = live ? row * DP + 8 * cgrp : -1;
+                const int col0 = j0 + C * g.dnC + 8 * cgrp;
+                const float* src = Xb + min(nl, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W;
+                unsigned cm = 0;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    v[d][e] = src[min(col0 + e, g.W - 1)];
+                    cm |= (col0 + e < g.W) ? 1u << e : 0u;
+                }
+                colMask[d] = cm;
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
+#pragma unroll
+            for (int d = 0; d < DU; d++) {
+                V hv, lv;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const float a = (onX[d] && ((colMask[d] >> e) & 1)) ? v[d][e] : 0.f;
+                    const T h = (T)a;
+                    hv[e] = h;
+                    if (SPLIT) lv[e] = (T)(a - (float)h);
+                }
+                const bool live = dstX[d] >= 0;
+                *reinterpret_cast<V*>(live ? sX + dstX[d] : sJunk + 8 * lane) = hv;
+                if (SPLIT) *reinterpret_cast<V*>(live ? sX + xTerm + dstX[d] : sJunk + 512 + 8 * lane) = lv;
+            }
+        }
+        __syncthreads();
+        for (int il = wave; il < rr; il += 4) {
+            const T* dRow = sD + ((size_t)il * MT + l16) * DP + 8 * g4;
+            const T* xRow = sX + (size_t)il * OW * DP;
+            for (int jb = 0; jb < c.DWp; jb += 32) {
+                V av[MB], bv[CBG];
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++) av[mb] = *reinterpret_cast<const V*>(dRow + mb * 16 * DP + jb);
+#pragma unroll
+                for (int q = 0; q < CBG; q++) bv[q] = *reinterpret_cast<const V*>(xRow + xoff[q] + jb);
+                if (SPLIT) {
+                    V al[MB], bl[CBG];
+#pragma unroll
+                    for (int mb = 0; mb < MB; mb++)
+                        al[mb] = *reinterpret_cast<const V*>(dRow + dTerm + mb * 16 * DP + jb);
+#pragma unroll
+                    for (int q = 0; q < CBG; q++) bl[q] = *reinterpret_cast<const V*>(xRow + xTerm + xoff[q] + jb);
+#pragma unroll
+                    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                        for (int q = 0; q < CBG; q++) {
+                            acc[mb][q] = lp_mfma<PREC>(al[mb], bv[q], acc[mb][q]);
+                            acc[mb][q] = lp_mfma<PREC>(av[mb], bl[q], acc[mb][q]);
+                        }
+                }
+#pragma unroll
+                for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                    for (int q = 0; q < CBG; q++) acc[mb][q] = lp_mfma<PREC>(av[mb], bv[q], acc[mb][q]);
+            }
+        }
+    }
+    // The four waves' sums meet in LDS in a fixed order, (w0 + w2) + (w1 + w3),
+    // then wave 0 writes the block's partial dW, ws[s][k][j].
+    // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
+    constexpr int E = MB * CBG * 4;
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(fa_lp_smem);
+    if (wave >= 2) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int q = 0; q < CBG; q++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    red[(size_t)((wave - 2) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
+    }
+    __syncthreads();
+    if (wave < 2) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int q = 0; q < CBG; q++)
+#pragma unroll
+                for (int v = 0; v < 4; v++)
+                    acc[mb][q][v] += red[(size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane];
+    }
+    __syncthreads();
+    if (wave == 1) {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int q = 0; q < CBG; q++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) red[(size_t)((mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        float* out = ws + (size_t)s * K * Ncols;
+#pragma unroll
+        for (int q = 0; q < CBG; q++) {
+            const int j = colBase + q * 16 + l16;
+            if (j >= Ncols) continue;
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int k = mb * 16 + 4 * g4 + v;
+                    if (k < K)
+                        out[(size_t)k * Ncols + j] =
+                            acc[mb][q][v] + red[(size_t)((mb * CBG + q) * 4 + v) * 64 + lane];
+                }
+        }
+    }
+}
+
+size_t fa_igemm_lp_lds_bytes(const FaCfg& c, int OW, int precision) {
+    const int terms = precision == COGCONV_SPLIT_BF16 ? 2 : 1;
+    const size_t dp = (size_t)c.DWp + 8;
+    const size_t stage = (size_t)terms * 2 *
+                         ((size_t)c.RR * 16 * c.MB * dp + (size_t)c.nCntMax * c.xRows * OW * dp + 64 * 8);
+    const size_t red = (size_t)2 * c.MB * c.CBW * 4 * 64 * sizeof(float);
+    return stage > red ? stage : red;
+}
+
+template <int MB, int PREC>
+static hipError_t fa_lp_launch2(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
+                                float* ws, hipStream_t s, dim3 grid) {
+    switch (c.CBW) {
+    case 3: hipLaunchKernelGGL((k_fa_igemm_lp<MB, 3, PREC>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    case 6: hipLaunchKernelGGL((k_fa_igemm_lp<MB, 6, PREC>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    default: hipLaunchKernelGGL((k_fa_igemm_lp<MB, 9, PREC>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    }
+    return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t fa_lp_launch1(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
+                                float* ws, hipStream_t s, dim3 grid) {
+    switch (c.MB) {
+    case 1: return fa_lp_launch2<1, PREC>(g, m, c, x, dy, ws, s, grid);
+    case 2: return fa_lp_launch2<2, PREC>(g, m, c, x, dy, ws, s, grid);
+    case 3: return fa_lp_launch2<3, PREC>(g, m, c, x, dy, ws, s, grid);
+    default: return fa_lp_launch2<4, PREC>(g, m, c, x, dy, ws, s, grid);
+    }
+}
+
+hipError_t launch_fa_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x,
+                              const float* dy, float* out, float* ws, hipStream_t s) {
+    const long blocks = (long)c.colGroups * c.splits;
+    if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    const dim3 grid((unsigned)blocks);
+    hipError_t e;
+    switch (precision) {
+    case COGCONV_BF16: e = fa_lp_launch1<COGCONV_BF16>(g, m, c, x, dy, ws, s, grid); break;
+    case COGCONV_FP16: e = fa_lp_launch1<COGCONV_FP16>(g, m, c, x, dy, ws, s, grid); break;
+    case COGCONV_SPLIT_BF16: e = fa_lp_launch1<COGCONV_SPLIT_BF16>(g, m, c, x, dy, ws, s, grid); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    const long total = (long)c.K * m.N * g.OH * g.OW;
+    return launch_reduce_partials(ws, out, c.splits, total, s);
+}
+
+}  // namespace cogconv

@@ -22,42 +22,12 @@
 // filter); x*w ~ xh*wh + xh*wl + xl*wh, each term on the MFMA, all into the
 // same FP32 accumulator.  BF16 has FP32's exponent range, so the split needs
 // no rescaling except for values below ~2^-118 whose low part underflows.
-#include "internal.h"
+#include "lp.h"
 
 namespace cogconv {
 
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
-
 constexpr int LP_RC = 32;   // planes per staged chunk (one MFMA k slice)
 constexpr int LP_RCP = 40;  // LDS pitch of a staged pixel / filter row, in 16-bit elements
-
-template <int PREC>
-struct LpType;
-template <>
-struct LpType<COGCONV_BF16> {
-    typedef __bf16 T;
-    typedef bf16x8 V;
-};
-template <>
-struct LpType<COGCONV_SPLIT_BF16> {
-    typedef __bf16 T;
-    typedef bf16x8 V;
-};
-template <>
-struct LpType<COGCONV_FP16> {
-    typedef _Float16 T;
-    typedef halfx8 V;
-};
-
-template <int PREC>
-__device__ __forceinline__ floatx4 lp_mfma(typename LpType<PREC>::V a, typename LpType<PREC>::V b, floatx4 c) {
-    if constexpr (PREC == COGCONV_FP16)
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
-    else
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
 
 // Filter pre-pack: Wp[term][t][m][r] in the MFMA input type, m padded to
 // Mpad, r padded to Rpad (zeros), taps flipped for the convolution

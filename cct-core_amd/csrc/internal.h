@@ -120,6 +120,7 @@ enum KernelKind {
     KK_IGEMM = 3,
     KK_FA_IGEMM = 4,
     KK_IGEMM_LP = 5,
+    KK_FA_IGEMM_LP = 6,
 };
 
 // LDS-tiled direct kernel configuration.
@@ -159,7 +160,9 @@ struct FaCfg {
     int MB;         // 16-row blocks (K <= 16*MB <= 64)
     int CBW;        // 16-column blocks per block (column group width), 3, 6 or 9
     int RR;         // tap rows staged per item
-    int DWp;        // tap columns rounded up to 4
+    int CW;         // tap columns per column chunk
+    int colChunks;  // ceil(FWe / CW)
+    int DWp;        // CW rounded up to 4
     int MTp;        // staged dy row pitch: 16*MB padded to 16 mod 32
     int xRows, xCols;
     int nCntMax;    // input planes staged per column group
@@ -184,6 +187,9 @@ hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c,
                             float* out, hipStream_t s);
 hipError_t launch_fa_igemm_f32(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
                                float* out, float* ws, hipStream_t s);
+size_t fa_igemm_lp_lds_bytes(const FaCfg& c, int OW, int precision);
+hipError_t launch_fa_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x,
+                              const float* dy, float* out, float* ws, hipStream_t s);
 bool igemm_lp_supported(int F);
 size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision);
 size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision);

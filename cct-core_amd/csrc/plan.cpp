@@ -277,10 +277,10 @@ IgemmCfg choose_igemm_frame(const Geom& g, const PlaneMap& m, int mbOnly = 0, in
 }
 
 // FP32 MFMA split-reduction GEMM for FilterAdjointBlockReduceSum, BorderValid.
-// cbw (0 = by size) fixes the column-group width; partials is the target
-// number of partial results (blocks) of the split reduction.
+// cbw (0 = by size) fixes the column-group width; partials (0 = by size) is
+// the target number of blocks, each of which writes one partial result.
 bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out, int cbw = 0,
-                     int partials = 256) {
+                     long partials = 0) {
     if (d->mode != COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM || d->border != COGCONV_BORDER_VALID || d->in0.dims != 2)
         return false;
     FaCfg c;
@@ -294,25 +294,90 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     c.CBW = cbw ? cbw : (colBlocks <= 3 ? 3 : (colBlocks <= 6 ? 6 : 9));
     c.colGroups = (int)((colBlocks + c.CBW - 1) / c.CBW);
     c.nCntMax = std::min(m.N, (16 * c.CBW + P - 1) / P + 1);
-    c.DWp = (g.FWe + 3) / 4 * 4;
-    c.xCols = (g.OW - 1) * g.dnC + c.DWp;
-    if (c.DWp > 128 || c.xCols > 128) return false;  // staged rows are at most two 64-lane passes
+    // Tap columns in chunks whose staged x rows fit two 64-lane passes.
+    if ((g.OW - 1) * g.dnC > 64) return false;
+    c.colChunks = 1;
+    for (;;) {
+        c.CW = (g.FWe + c.colChunks - 1) / c.colChunks;
+        c.DWp = (c.CW + 3) / 4 * 4;
+        c.xCols = (g.OW - 1) * g.dnC + c.DWp;
+        if (c.xCols <= 128) break;
+        c.colChunks++;
+    }
+    c.CW = c.DWp;  // chunk starts on multiples of 4; the last chunk is masked at FWe
+    c.colChunks = (g.FWe + c.CW - 1) / c.CW;
     c.MTp = c.MB == 1 ? 16 : (c.MB == 4 ? 80 : 48);  // 16*MB padded to 16 mod 32 (k_fa_igemm_f32's MTP)
     if ((long)m.B * m.N * g.H * g.W >= (1L << 31) || (long)m.B * c.K * g.FH * g.FW >= (1L << 31)) return false;
     const size_t red = (size_t)2 * c.MB * c.CBW * 4 * 64 * sizeof(float);  // cross-wave reduction buffer
-    c.RR = 0;
-    for (int rr = std::min(g.FHe, 8); rr >= 1; rr--) {
+    // Tap rows per item: as many as fit 80 KB (two blocks per CU); but at
+    // least 4, one per wave, when the tap rows allow it and 160 KB holds them.
+    auto lds_for = [&](int rr) {
         const int xRows = rr + (g.OH - 1) * g.dnR;
         const size_t bytes = ((size_t)c.MTp * rr * c.DWp + (size_t)c.nCntMax * xRows * c.xCols + 128) * sizeof(float);
-        if (std::max(bytes, red) <= 80 * 1024) { c.RR = rr; break; }
+        return std::max(bytes, red);
+    };
+    c.RR = 0;
+    for (int rr = std::min(g.FHe, 8); rr >= 1; rr--) {
+        if (lds_for(rr) <= 80 * 1024) { c.RR = rr; break; }
     }
+    if (c.RR < 4 && g.FHe >= 4 && lds_for(4) <= 160 * 1024) c.RR = 4;
     if (c.RR == 0) return false;
     c.xRows = c.RR + (g.OH - 1) * g.dnR;
     // dy rows, x rows and 128 scratch slots for masked-off stores
     c.lds_bytes = ((size_t)c.MTp * c.RR * c.DWp + (size_t)c.nCntMax * c.xRows * c.xCols + 128) * sizeof(float);
     c.lds_bytes = std::max(c.lds_bytes, red);
     c.rowChunks = (g.FHe + c.RR - 1) / c.RR;
-    const long items = (long)m.B * c.rowChunks;
+    const long items = (long)m.B * c.colChunks * c.rowChunks;
+    if (partials <= 0) partials = items * c.colGroups >= 8192 ? 512 : 256;  // blocks: 1-2 per CU
+    long want = (partials + c.colGroups - 1) / c.colGroups;
+    c.splits = (int)std::max(1L, std::min(items, want));
+    *out = c;
+    return true;
+}
+
+// BF16 / FP16 / split-BF16 MFMA split-reduction GEMM for
+// FilterAdjointBlockReduceSum, BorderValid (k_fa_igemm_lp).
+bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out, int cbw = 0,
+                        long partials = 0) {
+    if (d->mode != COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM || d->border != COGCONV_BORDER_VALID || d->in0.dims != 2)
+        return false;
+    FaCfg c;
+    std::memset(&c, 0, sizeof c);
+    c.K = m.T / m.B;
+    if (c.K > 64) return false;
+    c.MB = (c.K + 15) / 16;
+    const int P = g.OH * g.OW;
+    const long ncols = (long)m.N * P;
+    const long colBlocks = (ncols + 15) / 16;
+    c.CBW = cbw ? cbw : (colBlocks <= 3 ? 3 : (colBlocks <= 6 ? 6 : 9));
+    c.colGroups = (int)((colBlocks + c.CBW - 1) / c.CBW);
+    c.nCntMax = std::min(m.N, (16 * c.CBW + P - 1) / P + 1);
+    // Tap columns per chunk: 32, 64 or 128 (whole 32-deep MFMA slices; 8
+    // columns per staging lane).
+    c.CW = g.FWe <= 32 ? 32 : (g.FWe <= 64 ? 64 : 128);
+    c.DWp = c.CW;
+    c.colChunks = (g.FWe + c.CW - 1) / c.CW;
+    c.MTp = 16 * c.MB;
+    if ((long)m.B * m.N * g.H * g.W >= (1L << 31) || (long)m.B * c.K * g.FH * g.FW >= (1L << 31)) return false;
+    auto lds_for = [&](int rr) {
+        FaCfg t = c;
+        t.RR = rr;
+        t.xRows = rr + (g.OH - 1) * g.dnR;
+        return fa_igemm_lp_lds_bytes(t, g.OW, d->precision);
+    };
+    c.RR = 0;
+    for (int rr = std::min(g.FHe, 8); rr >= 1; rr--) {
+        if (lds_for(rr) <= 80 * 1024) { c.RR = rr; break; }
+    }
+    if (c.RR < 4 && g.FHe >= 4 && lds_for(4) <= 160 * 1024) c.RR = 4;
+    if (c.RR == 0 && lds_for(1) <= 160 * 1024) c.RR = 1;
+    if (c.RR == 0) return false;
+    c.xRows = c.RR + (g.OH - 1) * g.dnR;
+    if ((long)m.N * c.xRows * g.OW >= (1L << 22)) return false;  // staging row index (fast_div range)
+    c.lds_bytes = lds_for(c.RR);
+    c.rowChunks = (g.FHe + c.RR - 1) / c.RR;
+    const long items = (long)m.B * c.colChunks * c.rowChunks;
+    if (partials <= 0) partials = items * c.colGroups >= 8192 ? 512 : 256;
     long want = (partials + c.colGroups - 1) / c.colGroups;
     c.splits = (int)std::max(1L, std::min(items, want));
     *out = c;
@@ -391,7 +456,8 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         if (fc) v.f = *fc;
         switch (kind) {
         case KK_IGEMM_LP: v.workspace = igemm_lp_workspace_bytes(v.i, g.FH, desc->precision); break;
-        case KK_FA_IGEMM: v.workspace = (size_t)v.f.splits * v.f.K * m.N * outPts * sizeof(float); break;
+        case KK_FA_IGEMM:
+        case KK_FA_IGEMM_LP: v.workspace = (size_t)v.f.splits * v.f.K * m.N * outPts * sizeof(float); break;
         case KK_SMALL_OUTPUT: v.workspace = (size_t)v.s.splits * m.nOut * outPts * sizeof(float); break;
         default: v.workspace = 0; break;
         }
@@ -413,6 +479,17 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         out.push_back(v);
     };
     if (desc->precision != COGCONV_FP32) {
+        FaCfg flp;
+        if (choose_fa_igemm_lp(desc, g, m, &flp)) {
+            add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &flp);
+            for (int cbw : {3, 6, 9})
+                for (long parts : {128L, 256L, 512L}) {
+                    FaCfg c2;
+                    if (choose_fa_igemm_lp(desc, g, m, &c2, cbw, parts))
+                        add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &c2);
+                }
+            return;
+        }
         IgemmCfg lp;
         if (!choose_igemm_lp(desc, g, m, &lp)) return;
         add(KK_IGEMM_LP, nullptr, nullptr, &lp, nullptr);
@@ -466,7 +543,8 @@ bool kind_matches_algo(int kind, int algo) {
     switch (algo) {
     case COGCONV_ALGO_DIRECT: return kind == KK_DIRECT;
     case COGCONV_ALGO_SMALL_OUTPUT: return kind == KK_SMALL_OUTPUT;
-    case COGCONV_ALGO_IMPLICIT_GEMM: return kind == KK_IGEMM || kind == KK_FA_IGEMM || kind == KK_IGEMM_LP;
+    case COGCONV_ALGO_IMPLICIT_GEMM:
+        return kind == KK_IGEMM || kind == KK_FA_IGEMM || kind == KK_IGEMM_LP || kind == KK_FA_IGEMM_LP;
     default: return true;
     }
 }
@@ -486,7 +564,8 @@ int make_plan(const cogconv_desc* desc, int vi, cogconv_plan* plan) {
     if (vars.empty())
         return fail(COGCONV_ERR_UNSUPPORTED,
                     "precision %d is offered for ProjectFrame/BackProjectFrame with block reduction on 2-D fields with "
-                    "a square 1/3/5/7 filter, no input up-sampling and a 32-column tile that fits one wavefront; "
+                    "a square 1/3/5/7 filter, no input up-sampling and a 32-column tile that fits one wavefront, and "
+                    "for FilterAdjoint with block reduction, BorderValid and at most 64 planes per image in dy; "
                     "not for this %s",
                     desc->precision, mode_name(desc->mode));
     const long outPts = (long)g.OH * g.OW;
@@ -652,10 +731,18 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
                  mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->icfg.MB, plan->icfg.lds_bytes);
         break;
     }
+    case KK_FA_IGEMM_LP: {
+        static const char* pn[] = {"fp32", "bf16", "fp16", "split_bf16"};
+        snprintf(buf, len, "fa_igemm_%s %s out=%dx%dx%d MB=%d CBW=%d RR=%d CW=%d groups=%d splits=%d lds=%zu",
+                 pn[plan->desc.precision], mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->fcfg.MB,
+                 plan->fcfg.CBW, plan->fcfg.RR, plan->fcfg.CW, plan->fcfg.colGroups, plan->fcfg.splits,
+                 plan->fcfg.lds_bytes);
+        break;
+    }
     case KK_FA_IGEMM:
-        snprintf(buf, len, "fa_igemm_f32 %s out=%dx%dx%d MB=%d CBW=%d RR=%d groups=%d splits=%d lds=%zu",
+        snprintf(buf, len, "fa_igemm_f32 %s out=%dx%dx%d MB=%d CBW=%d RR=%d CW=%d groups=%d splits=%d lds=%zu",
                  mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->fcfg.MB, plan->fcfg.CBW, plan->fcfg.RR,
-                 plan->fcfg.colGroups, plan->fcfg.splits, plan->fcfg.lds_bytes);
+                 plan->fcfg.CW, plan->fcfg.colGroups, plan->fcfg.splits, plan->fcfg.lds_bytes);
         break;
     default:
         snprintf(buf, len, "igemm_f32 %s out=%dx%dx%d MB=%d RC=%d tile=8x32 threads=512 lds=%zu", mode_name(plan->desc.mode),
@@ -695,6 +782,10 @@ extern "C" int cogconv_launch(cogconv_plan plan, const float* in0, const float* 
         break;
     case KK_IGEMM_LP:
         e = launch_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->icfg, in0, in1, out, workspace, s);
+        break;
+    case KK_FA_IGEMM_LP:
+        e = launch_fa_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->fcfg, in0, in1, out,
+                               static_cast<float*>(workspace), s);
         break;
     default:
         return fail(COGCONV_ERR_RUNTIME, "internal: unknown kernel kind %d", plan->kind);

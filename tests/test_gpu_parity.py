@@ -266,6 +266,45 @@ def test_c2_filter_adjoint_full_size():
     assert_parity(got, ref, mag, what="c2 FA")
 
 
+@pytest.mark.parametrize("shape,orient,sampling,step", [
+    (((2, 3, 40, 150), (2, 4, 38, 148)), O.XCORR, O.NONE, 1),   # two tap-column chunks
+    (((1, 2, 30, 300), (1, 5, 26, 292)), O.CONV, O.NONE, 1),    # three chunks, 5x9 output
+    (((2, 2, 41, 181), (2, 3, 18, 88)), O.XCORR, O.UP, 2),      # up-sampled dy, 175 tap columns
+])
+def test_filter_adjoint_column_chunks_gpu(shape, orient, sampling, step):
+    """FilterAdjoint+BRS, BorderValid, with dy wider than one staged row: the
+    tap columns are split into chunks whose partial sums meet in the split
+    reduction."""
+    (B, N, H, W), (_, K, dh, dw) = shape
+    rng = np.random.default_rng(zlib.crc32(repr(shape).encode()))
+    x = rng.uniform(-1, 1, (B * N, H, W)).astype(np.float32)
+    dy = rng.uniform(-1, 1, (B * K, dh, dw)).astype(np.float32)
+    od, cd = desc_pair(O.FA_BRS, O.VALID, orient, sampling, step, B, x.shape, dy.shape, algo=L.ALGO_IMPLICIT_GEMM)
+    plan = L.Plan(cd)
+    announce("FA chunks", shape, plan.description)
+    assert plan.description.startswith("fa_igemm_f32")
+    plan.close()
+    got = run_gpu(cd, x, dy)
+    ref, mag = oracle_ref(od, x, dy, threads=16)
+    assert_parity(got, ref, mag, what="FA chunks")
+
+
+def test_c3_filter_adjoint_fp32_full_size():
+    """Batch 32, x 128x128x64, dy 126x126x64 (BorderValid): dW 64x64x3x3, FP32."""
+    rng = np.random.default_rng(33)
+    B, N, K = 32, 64, 64
+    x = rng.uniform(-1, 1, (B * N, 128, 128)).astype(np.float32)
+    dy = rng.uniform(-1, 1, (B * K, 126, 126)).astype(np.float32)
+    od, cd = desc_pair(O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, B, x.shape, dy.shape)
+    plan = L.Plan(cd)
+    announce("c3 FA fp32", plan.description)
+    plan.close()
+    got = run_gpu(cd, x, dy)
+    ref, mag = oracle_ref(od, x, dy, threads=16)
+    assert got.shape == (K * N, 3, 3)
+    assert_parity(got, ref, mag, what="c3 FA")
+
+
 def test_fused_equals_unfused():
     """The reduce fusion of ProjectFrameTensorReduceSumOptimizer changes nothing
     beyond rounding (StaticConvolutionSpec.scala:1742-1768)."""

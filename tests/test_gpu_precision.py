@@ -86,3 +86,55 @@ def test_precision_rejects_unsupported_modes():
     with pytest.raises(L.CogconvError) as e:
         L.Plan(cd)
     assert e.value.status == L.ERR_UNSUPPORTED
+
+
+FA_CASES = [
+    # (orientation, sampling, step, (B, N, H, W), (K, dh, dw))
+    (O.XCORR, O.NONE, 1, (4, 16, 34, 34), (32, 32, 32)),      # 3x3 dW, c2-like planes
+    (O.CONV, O.NONE, 1, (2, 5, 30, 70), (3, 26, 64)),        # 5x7 dW, ragged planes
+    (O.XCORR, O.DOWN, 2, (2, 6, 37, 41), (20, 18, 20)),      # down-sampled x taps: dW 10x11
+    (O.XCORR, O.UP, 2, (2, 4, 41, 181), (12, 18, 88)),       # up-sampled dy, two tap-column chunks
+    (O.XCORR, O.NONE, 1, (1, 3, 20, 300), (40, 18, 294)),    # three chunks, MB 3
+]
+
+
+@pytest.mark.parametrize("precision", (L.BF16, L.FP16, L.SPLIT_BF16))
+@pytest.mark.parametrize("orient,sampling,step,xs,ys", FA_CASES)
+def test_filter_adjoint_precision_accuracy(precision, orient, sampling, step, xs, ys):
+    """FilterAdjoint + blockReduceSum(batch), BorderValid, in the precision
+    variants: same tolerances as the frame modes."""
+    B, N, H, W = xs
+    K, dh, dw = ys
+    rng = np.random.default_rng(7000 + 100 * precision + dh + dw)
+    for wide in (False, True):
+        x = inputs(rng, (B * N, H, W), wide)
+        dy = inputs(rng, (B * K, dh, dw), wide)
+        od, cd = desc_pair(O.FA_BRS, O.VALID, orient, sampling, step, B, x.shape, dy.shape, precision=precision)
+        plan = L.Plan(cd)
+        print("CASE fa precision", NAMES[precision], xs, ys, "wide" if wide else "uniform", plan.description,
+              flush=True)
+        assert plan.description.startswith("fa_igemm_" + NAMES[precision])
+        plan.close()
+        got = run_gpu(cd, x, dy)
+        ref = O.conv(od, x, dy, threads=16)
+        err = normwise(got, ref)
+        print("  normwise error %.3g" % err)
+        assert got.shape == ref.shape
+        assert err <= TOL[precision], (NAMES[precision], wide, err)
+
+
+def test_filter_adjoint_precision_c3_slice():
+    """B32 128x128x64 / dy 126x126x64 on a 4-image slice, BF16 / FP16 / split."""
+    rng = np.random.default_rng(34)
+    B, N, K = 4, 64, 64
+    x = rng.uniform(-1, 1, (B * N, 128, 128)).astype(np.float32)
+    dy = rng.uniform(-1, 1, (B * K, 126, 126)).astype(np.float32)
+    od = O.make_desc(O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, B, x.shape, dy.shape)
+    ref = O.conv(od, x, dy, threads=16)
+    for precision in (L.BF16, L.FP16, L.SPLIT_BF16):
+        _, cd = desc_pair(O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, B, x.shape, dy.shape, precision=precision)
+        print("CASE c3 FA slice", NAMES[precision], flush=True)
+        got = run_gpu(cd, x, dy)
+        err = normwise(got, ref)
+        print("  normwise error %.3g" % err)
+        assert err <= TOL[precision]
