@@ -84,6 +84,20 @@ def cpu_baseline(sample_images=16, budget_s=10.0):
                       % (B, flops / 1e9, len(times))}
 
 
+def tuned_plan(L, desc, args):
+    """The plan for `desc`, with its kernel configuration chosen by the
+    autotuner (cogconv_autotune) unless --no-tune."""
+    if not args.no_tune:
+        if args.tune_cache:
+            L.tune_cache_load(args.tune_cache)
+        r = L.autotune(desc, warmup_steps=1, run_steps=5, flush_bytes=512 << 20)
+        log("tuned:", r["description"], "%.2f us" % r["avg_us"], "(cached)" if r["from_cache"] else
+            "(%d candidates)" % r["variants"])
+        if args.tune_cache:
+            L.tune_cache_save(args.tune_cache)
+    return L.Plan(desc)
+
+
 def precision_workload(args):
     """B32 128x128x64 planes, 64 filters 3x3x64: ProjectFrame+BRS and
     BackProjectFrame+BRS (BorderZero) and FilterAdjoint+BRS (BorderValid, dy
@@ -113,7 +127,8 @@ def precision_workload(args):
                  (2, 3, 3, K * N), bw),
                 ("filterAdjoint+BRS", L.FILTER_ADJOINT_BLOCK_REDUCE_SUM, L.BORDER_VALID, L.CROSS_CORRELATION, x, bx,
                  f(B * K, H - 2), bdyv)):
-            plan = L.Plan(L.make_desc(mode, border, orient, L.NO_SAMPLING, 1, B, f(a.shape[0]), t1, prec))
+            plan = tuned_plan(L, L.make_desc(mode, border, orient, L.NO_SAMPLING, 1, B, f(a.shape[0]), t1, prec),
+                              args)
             out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
             ws = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
             for _ in range(3):
@@ -280,6 +295,10 @@ def main():
                     help="fork the three independent ops of a step onto three streams (default: back to back on one)")
     ap.add_argument("--workload", default="cnn-triple", choices=("cnn-triple", "precision", "serving"))
     ap.add_argument("--requests", type=int, default=3000, help="requests of the serving workload")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the heuristic kernel configuration instead of autotuning each op's descriptor")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuning cache file: loaded before planning, saved after (cached samples are not re-timed)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run three 64 MiB device-to-device copies (a known-size reference for HBM counters)")
     args = ap.parse_args()
@@ -316,7 +335,7 @@ def main():
     bx, bw, bdy = (L.DeviceBuffer.from_array(a) for a in (x, w, dy))
     ops = []
     for name, desc in triple_descs(L):
-        plan = L.Plan(desc)
+        plan = tuned_plan(L, desc, args)
         out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
         ws = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
         ins = {"projectFrame+BRS": (bx, bw), "backProjectFrame+BRS": (bdy, bw), "filterAdjoint+BRS": (bx, bdy)}[name]
@@ -422,6 +441,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded uniform[-1,1) x and dy, He-normal filters)",
+        "kernel_choice": "heuristic (--no-tune)" if args.no_tune else "autotuned per op (cogconv_autotune)",
         "config": {"workload": "FP32 CNN-layer triple: PF+BRS Valid, BPF+BRS Full, FA+BRS Valid; batch 16 "
                                "images 64x64x16, 32 filters 3x3x16", "batch": 16, "image": [64, 64, 16],
                    "filters": [32, 3, 3, 16], "flops_per_step": flops_step,

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         constexpr int U = 16;
         constexpr int MT = 16 * MB;    // dy planes the MFMAs read (K padded with zeros)
         const int dRows = MT * rr;     // (il, k); the pitch padding beyond MT is never read
-        const float* DYb = DY + (size_t)b * K * FHW;
+        const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * FHW, (uint32_t)K * FHW * 4u);
         // Per-lane table (lane = staged tap row il < RR <= 8) of the dy row
         // offsets, flip and up-sampling lattice applied, and the rows' mask.
         int tiOffLane;
@@ -149,12 +149,12 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
                 dstU[u] = row < dRows ? il * c.DWp * MTP + k : -1;
             }
             if (dTwo)
-                stage_rows<true, U>(DYb, offU, onU, dstU, sD, sJunk, lane, c.DWp, MTP, dCol0, dCol1, dIn0, dIn1);
+                stage_rows<true, U>(dyr, offU, onU, dstU, sD, sJunk, lane, c.DWp, MTP, dCol0, dCol1, dIn0, dIn1);
             else
-                stage_rows<false, U>(DYb, offU, onU, dstU, sD, sJunk, lane, c.DWp, MTP, dCol0, dCol1, dIn0, dIn1);
+                stage_rows<false, U>(dyr, offU, onU, dstU, sD, sJunk, lane, c.DWp, MTP, dCol0, dCol1, dIn0, dIn1);
         }
         const int xRowsAll = nCnt * c.xRows;  // (plane, row)
-        const float* Xb = X + ((size_t)b * N + n_lo) * HWx;
+        const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + ((size_t)b * N + n_lo) * HWx, (uint32_t)nCnt * HWx * 4u);
         for (int base = wave; base < xRowsAll; base += 4 * U) {
             int offU[U], dstU[U];
             bool onU[U];
@@ -173,9 +173,9 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
                 }
             }
             if (xTwo)
-                stage_rows<true, U>(Xb, offU, onU, dstU, sX, sJunk, lane, c.xCols, 1, xCol0, xCol1, xIn0, xIn1);
+                stage_rows<true, U>(xr, offU, onU, dstU, sX, sJunk, lane, c.xCols, 1, xCol0, xCol1, xIn0, xIn1);
             else
-                stage_rows<false, U>(Xb, offU, onU, dstU, sX, sJunk, lane, c.xCols, 1, xCol0, xCol1, xIn0, xIn1);
+                stage_rows<false, U>(xr, offU, onU, dstU, sX, sJunk, lane, c.xCols, 1, xCol0, xCol1, xIn0, xIn1);
         }
         __syncthreads();
         // Wave w reduces tap rows w, w+4, ... of the staged chunk for every

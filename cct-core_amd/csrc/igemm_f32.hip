@@ -59,6 +59,7 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
     const int L = m.T / m.N;
     const int HW = g.H * g.W;                       // 32-bit offsets: the plan checks the sizes
     const float* Xb = X + (size_t)b * m.N * HW;
+    const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(Xb, (uint32_t)m.N * HW * 4u);  // the plan bounds this by 2^31
     // Per-lane column offsets of the staged tile (two 64-lane passes at most)
     // with the border applied: the same for every staged row and plane.
     const bool twoPass = TCp > 64;
@@ -150,9 +151,9 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
                 }
             }
             if (twoPass)
-                stage_rows<true, IU>(Xb, offU, onU, dstU, sB, sJunk, lane, TCp, 1, colOff0, colOff1, colIn0, colIn1);
+                stage_rows<true, IU>(xr, offU, onU, dstU, sB, sJunk, lane, TCp, 1, colOff0, colOff1, colIn0, colIn1);
             else
-                stage_rows<false, IU>(Xb, offU, onU, dstU, sB, sJunk, lane, TCp, 1, colOff0, colOff1, colIn0, colIn1);
+                stage_rows<false, IU>(xr, offU, onU, dstU, sB, sJunk, lane, TCp, 1, colOff0, colOff1, colIn0, colIn1);
         }
         // Filter slice: thread -> (plane rc, filter mm); the F*F taps are
         // contiguous in W.  Loads are unconditional from a clamped pair.

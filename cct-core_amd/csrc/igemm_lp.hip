@@ -23,10 +23,14 @@
 // same FP32 accumulator.  BF16 has FP32's exponent range, so the split needs
 // no rescaling except for values below ~2^-118 whose low part underflows.
 #include "lp.h"
+#include "stage.h"
 
 namespace cogconv {
 
 constexpr int LP_RC = 32;   // planes per staged chunk (one MFMA k slice)
+// Lane byte offset with bit 31 set: past every staged buffer (each is under
+// 2^31 bytes), so the buffer unit's range check answers the load with 0.
+constexpr int LP_OOB = (int)0x80000000u;
 constexpr int LP_RCP = 40;  // LDS pitch of a staged pixel / filter row, in 16-bit elements
 
 // Filter pre-pack: Wp[term][t][m][r] in the MFMA input type, m padded to
@@ -92,10 +96,11 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     const int y0 = tr * 8 * g.dnR - g.hR;
     const int x0 = tc * 32 * g.dnC - g.hC;
     const int HW = g.H * g.W;
-    const float* Xb = X + (size_t)b * m.N * HW;
+    const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + (size_t)b * m.N * HW, (uint32_t)m.N * HW * 4u);
     const int Mpad = c.mTiles * MT;
     const int Rpad = c.RC;  // planes padded to a multiple of 32
     const long wTerm = (long)TAPS * Mpad * Rpad;
+    const __amdgpu_buffer_rsrc_t wr = buffer_rsrc(Wp, (uint32_t)(TERMS * wTerm * 2));  // the plan bounds this
 
     // Per-lane column offset of the staged tile (TCp <= 64) with the border applied.
     int colOff;
@@ -113,6 +118,32 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
         }
         colOff = x;
         colIn = in && lane < TCp;
+    }
+    const int colByte = colIn ? 4 * colOff : LP_OOB;
+
+    // Filter-slice units (term, tap, output plane, 8-plane quarter) of this
+    // thread: byte offsets in the packed bank (the chunk adds 2*r0 as the
+    // uniform offset) and LDS destinations (elements from sB; units past the
+    // end go to the scratch slot), computed once when there are few per thread.
+    constexpr int UNITS = TERMS * TAPS * MT * 4;
+    constexpr int WPT = (UNITS + 255) / 256;
+    constexpr bool WPRE = WPT <= 12;
+    int wOff[WPRE ? WPT : 1], wDst[WPRE ? WPT : 1];
+    if (WPRE) {
+#pragma unroll
+        for (int i = 0; i < (WPRE ? WPT : 1); i++) {
+            const int q = tid + 256 * i;
+            const int qc = min(q, UNITS - 1);
+            const int qq = qc & 3;
+            const int rest = qc >> 2;
+            const int mm = rest % MT;
+            const int rest2 = rest / MT;
+            const int t = rest2 % TAPS;
+            const int term = rest2 / TAPS;
+            wOff[i] = 2 * (int)(term * wTerm + ((long)t * Mpad + m0 + mm) * Rpad + 8 * qq);
+            wDst[i] = q < UNITS ? (int)((size_t)TERMS * npix * LP_RCP + ((size_t)(term * TAPS + t) * MT + mm) * LP_RCP + 8 * qq)
+                                : (int)(sJunk + 8 * lane);
+        }
     }
 
     // Per-lane table of the staged tile's row offsets (lane = tile row, at
@@ -150,17 +181,25 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
 
     for (int r0 = 0; r0 < m.R; r0 += LP_RC) {
         __syncthreads();
+        V wv[WPRE ? WPT : 1];
+        if (WPRE) {
+#pragma unroll
+            for (int i = 0; i < (WPRE ? WPT : 1); i++)
+                wv[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, wOff[i], 2 * r0, 0));
+        }
         // Input tile: items (8-plane group, row); each lane loads the eight
         // planes of its column, rounds them and writes one 16-byte LDS vector.
         // The loads of IU items are unconditional from clamped, in-bounds
         // addresses and all in flight before the first is used; masks are
         // applied afterwards, and lanes beyond the tile width store into a
         // scratch slot, so no lane-dependent branch separates the loads.
+        // Masked samples (rows or columns outside the field for Zero/Full/Valid
+        // borders) read through an out-of-range lane offset, which the buffer
+        // unit answers with 0: one select per item instead of one per sample.
         const int nitems = 4 * TRp;
         constexpr int IU = 4;
         for (int base = wave; base < nitems; base += 4 * IU) {
             float v[IU][8];
-            bool rowOn[IU];
             int grpU[IU], tyU[IU];
             int grp = base / TRp;  // wave-uniform; TRp >= 8, so one wrap per step of 4
             int ty = base - grp * TRp;
@@ -168,12 +207,14 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
             for (int u = 0; u < IU; u++) {
                 const int item = base + 4 * u;
                 const int tyc = min(ty, 63);
-                rowOn[u] = item < nitems && ((yMask >> tyc) & 1);
+                const bool rowOn = item < nitems && ((yMask >> tyc) & 1);
                 grpU[u] = grp;
                 tyU[u] = ty;
-                const float* src = Xb + __builtin_amdgcn_readlane(yOffLane, tyc) + colOff;
+                const int rowOff = __builtin_amdgcn_readlane(yOffLane, tyc);
+                const int vo = rowOn ? colByte : LP_OOB;
 #pragma unroll
-                for (int e = 0; e < 8; e++) v[u][e] = src[min(r0 + 8 * grp + e, m.R - 1) * HW];
+                for (int e = 0; e < 8; e++)
+                    v[u][e] = buffer_load_f32(xr, vo, 4 * (min(r0 + 8 * grp + e, m.R - 1) * HW + rowOff));
                 ty += 4;
                 if (ty >= TRp) { ty -= TRp; grp++; }
             }
@@ -181,14 +222,17 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
 #pragma unroll
             for (int u = 0; u < IU; u++) {
                 if (base + 4 * u < nitems) {  // wave-uniform
-                    const bool keep = rowOn[u] && colIn;
+                    if (r0 + 8 * grpU[u] + 8 > m.R) {  // wave-uniform: planes past R in this group
+#pragma unroll
+                        for (int e = 0; e < 8; e++)
+                            if (r0 + 8 * grpU[u] + e >= m.R) v[u][e] = 0.f;
+                    }
                     V hv, lv;
 #pragma unroll
                     for (int e = 0; e < 8; e++) {
-                        const float a = (keep && r0 + 8 * grpU[u] + e < m.R) ? v[u][e] : 0.f;
-                        const T h = (T)a;
+                        const T h = (T)v[u][e];
                         hv[e] = h;
-                        if (SPLIT) lv[e] = (T)(a - (float)h);
+                        if (SPLIT) lv[e] = (T)(v[u][e] - (float)h);
                     }
                     const bool inTile = lane < TCp;
                     const size_t o = (size_t)(tyU[u] * TCp + lane) * LP_RCP + 8 * grpU[u];
@@ -197,34 +241,38 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
                 }
             }
         }
-        // Filter slice from the pre-packed bank: 16-byte units (term, t, m, q),
-        // loaded in unconditional batches of WU (clamped unit index).
-        constexpr int UNITS = TERMS * TAPS * MT * 4;
-        constexpr int WU = 8;
-        for (int q0 = tid; q0 < UNITS; q0 += 256 * WU) {
-            V wv[WU];
+        // Filter slice from the pre-packed bank: 16-byte units (term, t, m, q).
+        if (WPRE) {
 #pragma unroll
-            for (int i = 0; i < WU; i++) {
-                const int q = min(q0 + 256 * i, UNITS - 1);
-                const int qq = q & 3;
-                const int rest = q >> 2;
-                const int mm = rest % MT;
-                const int rest2 = rest / MT;
-                const int t = rest2 % TAPS;
-                const int term = rest2 / TAPS;
-                wv[i] = *reinterpret_cast<const V*>(Wp + term * wTerm + ((long)t * Mpad + m0 + mm) * Rpad + r0 + 8 * qq);
-            }
+            for (int i = 0; i < (WPRE ? WPT : 1); i++) *reinterpret_cast<V*>(sB + wDst[i]) = wv[i];
+        } else {
+            constexpr int WU = 8;
+            for (int q0 = tid; q0 < UNITS; q0 += 256 * WU) {
+                V wb[WU];
 #pragma unroll
-            for (int i = 0; i < WU; i++) {
-                const int q = q0 + 256 * i;
-                if (q < UNITS) {
+                for (int i = 0; i < WU; i++) {
+                    const int q = min(q0 + 256 * i, UNITS - 1);
                     const int qq = q & 3;
                     const int rest = q >> 2;
                     const int mm = rest % MT;
                     const int rest2 = rest / MT;
                     const int t = rest2 % TAPS;
                     const int term = rest2 / TAPS;
-                    *reinterpret_cast<V*>(sA + ((size_t)(term * TAPS + t) * MT + mm) * LP_RCP + 8 * qq) = wv[i];
+                    const int eoff = (int)(term * wTerm + ((long)t * Mpad + m0 + mm) * Rpad + 8 * qq);
+                    wb[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, 2 * eoff, 2 * r0, 0));
+                }
+#pragma unroll
+                for (int i = 0; i < WU; i++) {
+                    const int q = q0 + 256 * i;
+                    if (q < UNITS) {
+                        const int qq = q & 3;
+                        const int rest = q >> 2;
+                        const int mm = rest % MT;
+                        const int rest2 = rest / MT;
+                        const int t = rest2 % TAPS;
+                        const int term = rest2 / TAPS;
+                        *reinterpret_cast<V*>(sA + ((size_t)(term * TAPS + t) * MT + mm) * LP_RCP + 8 * qq) = wb[i];
+                    }
                 }
             }
         }

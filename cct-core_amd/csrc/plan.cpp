@@ -227,6 +227,7 @@ bool igemm_frame_eligible(const cogconv_desc* d, const Geom& g) {
     if (31 * g.dnC + g.FW > 128) return false;  // staged tile rows are at most two 64-lane passes
     if (7 * g.dnR + g.FH > 64) return false;    // one lane per staged tile row in the row table
     if ((long)d->in0.planes * g.H * g.W >= (1L << 31)) return false;  // 32-bit staging offsets
+    if ((long)(d->in0.planes / d->batch) * g.H * g.W * 4 >= (1L << 31)) return false;  // buffer byte offsets
     return igemm_f32_supported(g.FH);
 }
 
@@ -308,6 +309,7 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     c.colChunks = (g.FWe + c.CW - 1) / c.CW;
     c.MTp = c.MB == 1 ? 16 : (c.MB == 4 ? 80 : 48);  // 16*MB padded to 16 mod 32 (k_fa_igemm_f32's MTP)
     if ((long)m.B * m.N * g.H * g.W >= (1L << 31) || (long)m.B * c.K * g.FH * g.FW >= (1L << 31)) return false;
+    if ((long)m.N * g.H * g.W * 4 >= (1L << 31) || (long)c.K * g.FH * g.FW * 4 >= (1L << 31)) return false;
     const size_t red = (size_t)2 * c.MB * c.CBW * 4 * 64 * sizeof(float);  // cross-wave reduction buffer
     // Tap rows per item: as many as fit 80 KB (two blocks per CU); but at
     // least 4, one per wave, when the tap rows allow it and 160 KB holds them.
@@ -359,6 +361,7 @@ bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m,
     c.colChunks = (g.FWe + c.CW - 1) / c.CW;
     c.MTp = 16 * c.MB;
     if ((long)m.B * m.N * g.H * g.W >= (1L << 31) || (long)m.B * c.K * g.FH * g.FW >= (1L << 31)) return false;
+    if ((long)m.N * g.H * g.W * 4 >= (1L << 31) || (long)c.K * g.FH * g.FW * 4 >= (1L << 31)) return false;
     auto lds_for = [&](int rr) {
         FaCfg t = c;
         t.RR = rr;
@@ -394,6 +397,7 @@ bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ig
     if (31 * g.dnC + g.FW > 64) return false;  // one 64-lane pass per staged row
     if (7 * g.dnR + g.FH > 64) return false;   // one lane per staged tile row in the row table
     if ((long)d->in0.planes * g.H * g.W >= (1L << 31)) return false;
+    if ((long)(d->in0.planes / d->batch) * g.H * g.W * 4 >= (1L << 31)) return false;  // buffer byte offsets
     IgemmCfg c;
     std::memset(&c, 0, sizeof c);
     c.Mout = m.T / m.N;
@@ -406,7 +410,8 @@ bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ig
         c.MB = MB;
         c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
         c.lds_bytes = igemm_lp_lds_bytes(c, g.FH, d->precision);
-        if (c.lds_bytes <= 160 * 1024 && (!mbOnly || MB == mbOnly)) {
+        if (c.lds_bytes <= 160 * 1024 && (!mbOnly || MB == mbOnly) &&
+            igemm_lp_workspace_bytes(c, g.FH, d->precision) < (1UL << 31)) {
             *out = c;
             return true;
         }
