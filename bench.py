@@ -90,7 +90,8 @@ def tuned_plan(L, desc, args):
     if not args.no_tune:
         if args.tune_cache:
             L.tune_cache_load(args.tune_cache)
-        r = L.autotune(desc, warmup_steps=1, run_steps=5, flush_bytes=512 << 20)
+        # Warm caches, as in the timed steps (the inputs are reused every step).
+        r = L.autotune(desc, warmup_steps=2, run_steps=20, flush_bytes=0)
         log("tuned:", r["description"], "%.2f us" % r["avg_us"], "(cached)" if r["from_cache"] else
             "(%d candidates)" % r["variants"])
         if args.tune_cache:
@@ -185,6 +186,89 @@ def precision_workload(args):
             "bf16_over_fp32_projectFrame": round(res["bf16"]["projectFrame+BRS"]["gflops"] / fp32, 2),
             "bf16_over_fp32_filterAdjoint": round(res["bf16"]["filterAdjoint+BRS"]["gflops"] /
                                                   res["fp32"]["filterAdjoint+BRS"]["gflops"], 2),
+            "peaks": {"fp32_tflops": PEAK_FP32_TFLOPS, "bf16_tflops": PEAK_BF16_TFLOPS, "hbm_gbs": PEAK_HBM_GBS}}
+    print(json.dumps(line), flush=True)
+
+
+def large_workload(args):
+    """Large fields: batch 8 images 256x256x32, 32 filters 7x7x32 and 9x9x32,
+    ProjectFrame+BRS with BorderClamp in FP32 and BF16; the same 7x7 bank on
+    the 2x up-sampled input (UpsampleInputConvolution(2), BorderZero, the
+    reference rejects Up + Clamp); PlaneByPlane 1024x1024 scalar field with
+    an 11x11 Gaussian-plus-noise filter, BorderClamp.  GFLOP/s from HIP
+    events; BF16 errors against the FP64 oracle on a one-image slice."""
+    import cogconv._lib as L
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    L.init(0)
+    rng = np.random.default_rng(4)
+    B, N, K, H = 8, 32, 32, 256
+    x = rng.uniform(-1, 1, (B * N, H, H)).astype(np.float32)
+    banks = {F: (rng.standard_normal((K * N, F, F)) * np.sqrt(2.0 / (N * F * F))).astype(np.float32) for F in (7, 9)}
+    yy, xx = np.mgrid[-5:6, -5:6]
+    gauss = np.exp(-(yy ** 2 + xx ** 2) / (2 * 2.0 ** 2))
+    pf = (gauss / gauss.sum() + rng.normal(0, 0.01, gauss.shape)).astype(np.float32)[None]
+    img = rng.uniform(0, 1, (1, 1024, 1024)).astype(np.float32)
+    bx, bimg, bpf = (L.DeviceBuffer.from_array(a) for a in (x, img, pf))
+    bb = {F: L.DeviceBuffer.from_array(w) for F, w in banks.items()}
+    stream = L.Stream()
+    ops = []
+    for F in (7, 9):
+        for prec, pn in ((L.FP32, "fp32"), (L.BF16, "bf16")):
+            ops.append(("projectFrame+BRS clamp %dx%d %s" % (F, F, pn),
+                        L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_CLAMP, L.CROSS_CORRELATION,
+                                    L.NO_SAMPLING, 1, B, (2, H, H, B * N), (2, F, F, K * N), prec), bx, bb[F]))
+    ops.append(("projectFrame+BRS up2 zero 7x7 fp32",
+                L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CROSS_CORRELATION,
+                            L.UPSAMPLE_INPUT, 2, B, (2, H, H, B * N), (2, 7, 7, K * N)), bx, bb[7]))
+    ops.append(("planeByPlane clamp 1024x1024 11x11 fp32",
+                L.make_desc(L.PLANE_BY_PLANE, L.BORDER_CLAMP, L.CONVOLUTION, L.NO_SAMPLING, 1, 1,
+                            (2, 1024, 1024, 1), (2, 11, 11, 1)), bimg, bpf))
+    res = {}
+    for name, desc, a, k in ops:
+        plan = tuned_plan(L, desc, args)
+        out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
+        ws = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
+        for _ in range(2):
+            plan.launch(a, k, out, ws, stream)
+        e0, e1 = L.Event(), L.Event()
+        reps = 10
+        e0.record(stream)
+        for _ in range(reps):
+            plan.launch(a, k, out, ws, stream)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        res[name] = {"ms": round(ms, 4), "gflops": round(plan.flops / (ms * 1e-3) / 1e9, 1),
+                     "gflop": round(plan.flops / 1e9, 3), "kernel": plan.description}
+        log(name, plan.description, "%.4f ms" % ms)
+        out.close()
+        if ws:
+            ws.close()
+    errs = {}
+    xs = x[:N]
+    for F in (7, 9):
+        od = O.make_desc(O.PF_BRS, O.CLAMP, O.XCORR, O.NONE, 1, 1, xs.shape, banks[F].shape)
+        ref = O.conv(od, xs, banks[F], threads=16)
+        for prec, pn in ((L.FP32, "fp32"), (L.BF16, "bf16")):
+            plan = L.Plan(L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_CLAMP, L.CROSS_CORRELATION,
+                                      L.NO_SAMPLING, 1, 1, (2, H, H, N), (2, F, F, K * N), prec))
+            a = L.DeviceBuffer.from_array(xs)
+            out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
+            wsb = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
+            plan.launch(a, bb[F], out, wsb, None)
+            L.synchronize()
+            got = out.download(plan.out_shape).astype(np.float64)
+            d = np.abs(got - ref)
+            errs["%dx%d %s" % (F, F, pn)] = {"max_rel_normwise": float(d.max() / np.abs(ref).max()),
+                                           "rel_l2": float(np.sqrt((d ** 2).sum() / (ref ** 2).sum()))}
+            for b_ in (a, out, wsb):
+                if b_ is not None:
+                    b_.close()
+    line = {"metric": "conv GFLOP/s & % roofline (FP32 vs BF16-MFMA); p50/p99 request latency",
+            "workload": "large fields: batch 8 images 256x256x32, 32 filters 7x7x32 / 9x9x32 (BorderClamp; "
+                        "Up2 with BorderZero), PlaneByPlane 1024x1024 11x11 BorderClamp",
+            "results": res, "errors_vs_fp64_one_image": errs,
             "peaks": {"fp32_tflops": PEAK_FP32_TFLOPS, "bf16_tflops": PEAK_BF16_TFLOPS, "hbm_gbs": PEAK_HBM_GBS}}
     print(json.dumps(line), flush=True)
 
@@ -293,7 +377,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--concurrent", action="store_true",
                     help="fork the three independent ops of a step onto three streams (default: back to back on one)")
-    ap.add_argument("--workload", default="cnn-triple", choices=("cnn-triple", "precision", "serving"))
+    ap.add_argument("--workload", default="cnn-triple", choices=("cnn-triple", "precision", "serving", "large"))
     ap.add_argument("--requests", type=int, default=3000, help="requests of the serving workload")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the heuristic kernel configuration instead of autotuning each op's descriptor")
@@ -311,6 +395,9 @@ def main():
         return
     if args.workload == "serving":
         serving_workload(args)
+        return
+    if args.workload == "large":
+        large_workload(args)
         return
 
     import cogconv._lib as L

@@ -384,11 +384,12 @@ static hipError_t launch_lp1(const Geom& g, const PlaneMap& m, const IgemmCfg& c
     case 3: return launch_lp2<3, PREC>(g, m, c, x, w, out, ws, s);
     case 5: return launch_lp2<5, PREC>(g, m, c, x, w, out, ws, s);
     case 7: return launch_lp2<7, PREC>(g, m, c, x, w, out, ws, s);
+    case 9: return launch_lp2<9, PREC>(g, m, c, x, w, out, ws, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-bool igemm_lp_supported(int F) { return F == 1 || F == 3 || F == 5 || F == 7; }
+bool igemm_lp_supported(int F) { return F == 1 || F == 3 || F == 5 || F == 7 || F == 9; }
 
 hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x,
                            const float* w, float* out, void* ws, hipStream_t s) {

@@ -305,6 +305,47 @@ def test_c3_filter_adjoint_fp32_full_size():
     assert_parity(got, ref, mag, what="c3 FA")
 
 
+def test_c4_plane_by_plane_full_size():
+    """PlaneByPlane 1024x1024 scalar field, 11x11 Gaussian-plus-noise filter, BorderClamp."""
+    rng = np.random.default_rng(44)
+    img = rng.uniform(0, 1, (1, 1024, 1024)).astype(np.float32)
+    yy, xx = np.mgrid[-5:6, -5:6]
+    gauss = np.exp(-(yy ** 2 + xx ** 2) / 8.0)
+    k = (gauss / gauss.sum() + rng.normal(0, 0.01, gauss.shape)).astype(np.float32)[None]
+    announce("c4 PBP")
+    od, cd = desc_pair(O.PBP, O.CLAMP, O.CONV, O.NONE, 1, 1, img.shape, k.shape)
+    got = run_gpu(cd, img, k)
+    ref, mag = oracle_ref(od, img, k, threads=16)
+    assert_parity(got, ref, mag, what="c4 PBP")
+
+
+@pytest.mark.parametrize("F", (7, 9))
+def test_c4_upsampled_project_frame_slice(F):
+    """UpsampleInputConvolution(2), BorderZero, ProjectFrame+BRS with the
+    large-field filter sizes, on a one-image 96x96x8 slice."""
+    rng = np.random.default_rng(45 + F)
+    x = rng.uniform(-1, 1, (8, 96, 96)).astype(np.float32)
+    w = rng.uniform(-1, 1, (6 * 8, F, F)).astype(np.float32)
+    od, cd = desc_pair(O.PF_BRS, O.ZERO, O.XCORR, O.UP, 2, 1, x.shape, w.shape)
+    announce("c4 up2", F)
+    got = run_gpu(cd, x, w)
+    ref, mag = oracle_ref(od, x, w, threads=16)
+    assert_parity(got, ref, mag, what="c4 up2")
+
+
+@pytest.mark.parametrize("F", (7, 9))
+def test_c4_clamp_project_frame_slice(F):
+    """BorderClamp ProjectFrame+BRS, 7x7 / 9x9 banks of 32 filters on a one-image 256x256x32 slice."""
+    rng = np.random.default_rng(47 + F)
+    x = rng.uniform(-1, 1, (32, 256, 256)).astype(np.float32)
+    w = (rng.standard_normal((32 * 32, F, F)) * np.sqrt(2.0 / (32 * F * F))).astype(np.float32)
+    od, cd = desc_pair(O.PF_BRS, O.CLAMP, O.XCORR, O.NONE, 1, 1, x.shape, w.shape)
+    announce("c4 clamp", F, L.Plan(cd).description)
+    got = run_gpu(cd, x, w)
+    ref, mag = oracle_ref(od, x, w, threads=16)
+    assert_parity(got, ref, mag, what="c4 clamp")
+
+
 def test_fused_equals_unfused():
     """The reduce fusion of ProjectFrameTensorReduceSumOptimizer changes nothing
     beyond rounding (StaticConvolutionSpec.scala:1742-1768)."""

@@ -138,3 +138,29 @@ def test_filter_adjoint_precision_c3_slice():
         err = normwise(got, ref)
         print("  normwise error %.3g" % err)
         assert err <= TOL[precision]
+
+
+@pytest.mark.parametrize("precision", (L.BF16, L.FP16, L.SPLIT_BF16))
+@pytest.mark.parametrize("mode,border,F", [(O.PF_BRS, O.CLAMP, 7), (O.PF_BRS, O.CLAMP, 9), (O.BPF_BRS, O.ZERO, 9),
+                                           (O.PF_BRS, O.CYCLIC, 7)])
+def test_precision_large_filters(precision, mode, border, F):
+    """7x7 and 9x9 filters (the large-field filter banks); split-BF16 is not
+    offered where its two-term staging does not fit LDS."""
+    rng = np.random.default_rng(500 + 10 * F + border + precision)
+    B, N, K, H, W = 2, 24, 20, 30, 41
+    a = inputs(rng, (B * N, H, W), False)
+    k = inputs(rng, (K * N, F, F), False)
+    orient = O.CONV if mode == O.BPF_BRS else O.XCORR
+    od, cd = desc_pair(mode, border, orient, O.NONE, 1, B, a.shape, k.shape, precision=precision)
+    try:
+        plan = L.Plan(cd)
+    except L.CogconvError as e:
+        assert precision == L.SPLIT_BF16 and e.status == L.ERR_UNSUPPORTED
+        return
+    print("CASE large filter", NAMES[precision], mode, border, F, plan.description, flush=True)
+    plan.close()
+    got = run_gpu(cd, a, k)
+    ref = O.conv(od, a, k)
+    err = normwise(got, ref)
+    print("  normwise error %.3g" % err)
+    assert err <= TOL[precision]
