@@ -26,10 +26,11 @@ namespace cogconv {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-template <int F, int MB>
-__global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg c, const float* __restrict__ X,
+template <int FR, int FC, int MB>
+__global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg c, Phase ph, const float* __restrict__ X,
                                                    const float* __restrict__ Wt, float* __restrict__ O) {
-    constexpr int TAPS = F * F;
+    constexpr int TAPS = FR * FC;
+    const int TAPSF = ph.FHf * ph.FWf;  // taps of a full filter in the bank
     constexpr int MT = 16 * MB;
     extern __shared__ float smem[];
     float* sB = smem;
@@ -162,23 +163,37 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
             const int r = r0 + rc, mg = m0 + mm;
             const bool ok = r < m.R && mg < Mout;
             const int rr = min(r, m.R - 1), mgc = min(mg, Mout - 1);
-            const float* src = Wt + (size_t)(bpf ? rr * L + mgc : mgc * m.N + rr) * TAPS;
+            const float* src = Wt + (size_t)(bpf ? rr * L + mgc : mgc * m.N + rr) * TAPSF;
+            // Tap (a, b) of this launch's (sub-)filter is tap (i0 + step*a,
+            // j0 + step*b) of the full filter, flipped for the convolution
+            // orientation; contiguous when there is one phase.
             float t[TAPS];
-#pragma unroll
-            for (int i = 0; i < TAPS; i++) t[i] = src[i];
             float* dst = sA + (size_t)rc * aStride + mm;
+            if (ph.step == 1) {  // wave-uniform: one phase, the taps are contiguous
 #pragma unroll
-            for (int i = 0; i < TAPS; i++) dst[i * MT] = ok ? (g.flip ? t[TAPS - 1 - i] : t[i]) : 0.f;
+                for (int i = 0; i < TAPS; i++) t[i] = src[i];
+#pragma unroll
+                for (int i = 0; i < TAPS; i++) dst[i * MT] = ok ? (ph.flip ? t[TAPS - 1 - i] : t[i]) : 0.f;
+            } else {
+#pragma unroll
+                for (int i = 0; i < TAPS; i++) {
+                    const int a = i / FC, bcol = i - (i / FC) * FC;
+                    const int fi = (ph.i0 + ph.step * a) * ph.FWf + ph.j0 + ph.step * bcol;
+                    t[i] = src[ph.flip ? TAPSF - 1 - fi : fi];
+                }
+#pragma unroll
+                for (int i = 0; i < TAPS; i++) dst[i * MT] = ok ? t[i] : 0.f;
+            }
         }
         __syncthreads();
         for (int p = pg * half; p < (pg + 1) * half; p += 4) {
             const float* bb = sB + (p + g4) * c.planeStride;
             const float* ab = sA + (size_t)(p + g4) * aStride + l16;
 #pragma unroll
-            for (int ti = 0; ti < F; ti++) {
+            for (int ti = 0; ti < FR; ti++) {
 #pragma unroll
-                for (int tj = 0; tj < F; tj++) {
-                    const int t = ti * F + tj;
+                for (int tj = 0; tj < FC; tj++) {
+                    const int t = ti * FC + tj;
                     float av[MB];
 #pragma unroll
                     for (int mb = 0; mb < MB; mb++) av[mb] = ab[t * MT + mb * 16];
@@ -231,39 +246,76 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
         const int mm = item >> 3, row = item & 7;
         const int orow = tr * 8 + row;
         if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
-            O[((size_t)(b * Mout + m0 + mm) * g.OH + orow) * g.OW + ocol] = sO[mm * OP + row * 32 + col];
+            O[((size_t)(b * Mout + m0 + mm) * ph.OHf + ph.oRow0 + ph.oStride * orow) * ph.OWf + ph.oCol0 +
+              ph.oStride * ocol] = sO[mm * OP + row * 32 + col];
     }
 }
 
-template <int F>
-static hipError_t launch_f(int MB, const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x,
-                           const float* w, float* out, hipStream_t s, dim3 grid) {
-    switch (MB) {
-    case 1: hipLaunchKernelGGL((k_igemm_f32<F, 1>), grid, dim3(512), c.lds_bytes, s, g, m, c, x, w, out); break;
-    case 2: hipLaunchKernelGGL((k_igemm_f32<F, 2>), grid, dim3(512), c.lds_bytes, s, g, m, c, x, w, out); break;
-    default: hipLaunchKernelGGL((k_igemm_f32<F, 4>), grid, dim3(512), c.lds_bytes, s, g, m, c, x, w, out); break;
+template <int FR, int FC>
+static hipError_t launch_rc(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const Phase& ph, const float* x,
+                            const float* w, float* out, hipStream_t s, dim3 grid) {
+    switch (c.MB) {
+    case 1: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 1>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
+    case 2: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 2>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
+    default: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 4>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
     }
     return hipGetLastError();
 }
 
-bool igemm_f32_supported(int F) { return F == 1 || F == 3 || F == 5 || F == 7 || F == 9; }
+// Square filters, and the rectangular phase filters of 2x up-sampling (odd F
+// up to 9 splits into ceil(F/2) and floor(F/2) taps per dimension).
+bool igemm_f32_supported_rc(int FR, int FC) {
+    if (FR == FC) return FR == 1 || FR == 3 || FR == 5 || FR == 7 || FR == 9 || FR == 2 || FR == 4;
+    const int lo = FR < FC ? FR : FC, hi = FR < FC ? FC : FR;
+    return hi == lo + 1 && lo >= 1 && hi <= 5;
+}
+
+bool igemm_f32_supported(int F) { return igemm_f32_supported_rc(F, F); }
 
 // LDS floats the output tile of k_igemm_f32's epilogue needs.
 size_t igemm_f32_exchange_floats(int MB) { return (size_t)16 * MB * 260; }
 
-hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
-                            float* out, hipStream_t s) {
+hipError_t launch_igemm_f32_phase(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const Phase& ph, const float* x,
+                                  const float* w, float* out, hipStream_t s) {
     const long blocks = (long)c.tilesC * c.tilesR * c.mTiles * m.B;
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)blocks);
-    switch (g.FH) {
-    case 1: return launch_f<1>(c.MB, g, m, c, x, w, out, s, grid);
-    case 3: return launch_f<3>(c.MB, g, m, c, x, w, out, s, grid);
-    case 5: return launch_f<5>(c.MB, g, m, c, x, w, out, s, grid);
-    case 7: return launch_f<7>(c.MB, g, m, c, x, w, out, s, grid);
-    case 9: return launch_f<9>(c.MB, g, m, c, x, w, out, s, grid);
+    const int key = g.FH * 10 + g.FW;
+    switch (key) {
+    case 11: return launch_rc<1, 1>(g, m, c, ph, x, w, out, s, grid);
+    case 33: return launch_rc<3, 3>(g, m, c, ph, x, w, out, s, grid);
+    case 55: return launch_rc<5, 5>(g, m, c, ph, x, w, out, s, grid);
+    case 77: return launch_rc<7, 7>(g, m, c, ph, x, w, out, s, grid);
+    case 99: return launch_rc<9, 9>(g, m, c, ph, x, w, out, s, grid);
+    case 22: return launch_rc<2, 2>(g, m, c, ph, x, w, out, s, grid);
+    case 44: return launch_rc<4, 4>(g, m, c, ph, x, w, out, s, grid);
+    case 12: return launch_rc<1, 2>(g, m, c, ph, x, w, out, s, grid);
+    case 21: return launch_rc<2, 1>(g, m, c, ph, x, w, out, s, grid);
+    case 23: return launch_rc<2, 3>(g, m, c, ph, x, w, out, s, grid);
+    case 32: return launch_rc<3, 2>(g, m, c, ph, x, w, out, s, grid);
+    case 34: return launch_rc<3, 4>(g, m, c, ph, x, w, out, s, grid);
+    case 43: return launch_rc<4, 3>(g, m, c, ph, x, w, out, s, grid);
+    case 45: return launch_rc<4, 5>(g, m, c, ph, x, w, out, s, grid);
+    case 54: return launch_rc<5, 4>(g, m, c, ph, x, w, out, s, grid);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
+                            float* out, hipStream_t s) {
+    Phase ph;
+    ph.i0 = 0;
+    ph.j0 = 0;
+    ph.step = 1;
+    ph.FHf = g.FH;
+    ph.FWf = g.FW;
+    ph.flip = g.flip;
+    ph.oRow0 = 0;
+    ph.oCol0 = 0;
+    ph.oStride = 1;
+    ph.OHf = g.OH;
+    ph.OWf = g.OW;
+    return launch_igemm_f32_phase(g, m, c, ph, x, w, out, s);
 }
 
 }  // namespace cogconv

@@ -121,6 +121,7 @@ enum KernelKind {
     KK_FA_IGEMM = 4,
     KK_IGEMM_LP = 5,
     KK_FA_IGEMM_LP = 6,
+    KK_IGEMM_UP = 7,
 };
 
 // LDS-tiled direct kernel configuration.
@@ -172,6 +173,27 @@ struct FaCfg {
     size_t lds_bytes;
 };
 
+// One launch of the FP32 implicit GEMM: its (sub-)filter is taps (i0 +
+// step*a, j0 + step*b) of the full FHf x FWf filter (flipped for the
+// convolution orientation), and its output (row, col) lands at (oRow0 +
+// oStride*row, oCol0 + oStride*col) of the full OHf x OWf output.  Up-sampled
+// input runs as step^2 such phases, each a dense convolution of the stored
+// samples; without up-sampling there is one phase with the identity mapping.
+struct Phase {
+    int i0, j0, step;
+    int FHf, FWf, flip;
+    int oRow0, oCol0, oStride;
+    int OHf, OWf;
+};
+
+// The phase launches of an up-sampled frame-mode plan (2x: four phases).
+struct PhaseSet {
+    int n;
+    Geom g[4];
+    IgemmCfg c[4];
+    Phase p[4];
+};
+
 enum IgemmVariant {
     IV_F32_FRAME = 1,  // FP32 MFMA, ProjectFrame/BackProjectFrame block-reduce modes
 };
@@ -183,6 +205,9 @@ hipError_t launch_small_output(const Geom& g, const PlaneMap& m, const SmallCfg&
                                float* out, float* workspace, hipStream_t s);
 bool igemm_f32_supported(int F);
 size_t igemm_f32_exchange_floats(int MB);
+bool igemm_f32_supported_rc(int FR, int FC);
+hipError_t launch_igemm_f32_phase(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const Phase& ph, const float* x,
+                                  const float* w, float* out, hipStream_t s);
 hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
                             float* out, hipStream_t s);
 hipError_t launch_fa_igemm_f32(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
@@ -218,6 +243,7 @@ struct cogconv_plan_s {
     cogconv::SmallCfg scfg;
     cogconv::IgemmCfg icfg;
     cogconv::FaCfg fcfg;
+    cogconv::PhaseSet ps;
     size_t workspace;
     double flops;
     double bytes;

@@ -439,8 +439,64 @@ struct Variant {
     SmallCfg s;
     IgemmCfg i;
     FaCfg f;
+    PhaseSet ps;
     size_t workspace;
 };
+
+// FP32 implicit GEMM for the frame block-reduce modes on 2x up-sampled input:
+// the zero-inserted input makes output pixel (2R+pr, 2C+pc) see only the taps
+// i = i0 + 2a, j = j0 + 2b whose sample is stored (i0 = (h - pr) mod 2), so
+// each of the four output phases is a dense convolution of the stored samples
+// with a ceil/floor(F/2)-tap sub-filter, at input row R + a + (pr - h + i0)/2.
+bool choose_igemm_up(const cogconv_desc* d, const Geom& g, const PlaneMap& m, PhaseSet* out) {
+    if (d->mode != COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM && d->mode != COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM)
+        return false;
+    if (d->in0.dims != 2 || g.upAr != 2 || g.upAc != 2 || g.dnR != 1 || g.dnC != 1 || g.FH != g.FW || g.FH < 3 ||
+        g.FH > 9 || (g.FH & 1) == 0)
+        return false;
+    if ((long)d->in0.planes * g.H * g.W >= (1L << 31)) return false;
+    if ((long)(d->in0.planes / d->batch) * g.H * g.W * 4 >= (1L << 31)) return false;
+    PhaseSet ps;
+    std::memset(&ps, 0, sizeof ps);
+    for (int pr = 0; pr < 2; pr++)
+        for (int pc = 0; pc < 2; pc++) {
+            const int OHp = (g.OH - pr + 1) / 2, OWp = (g.OW - pc + 1) / 2;
+            if (OHp <= 0 || OWp <= 0) continue;
+            const int i0 = ((g.hR - pr) % 2 + 2) % 2, j0 = ((g.hC - pc) % 2 + 2) % 2;
+            const int FR = (g.FH - i0 + 1) / 2, FC = (g.FW - j0 + 1) / 2;
+            if (!igemm_f32_supported_rc(FR, FC)) return false;
+            Geom q = g;
+            q.FH = q.FHe = FR;
+            q.FW = q.FWe = FC;
+            q.upAr = q.upAc = q.upFr = q.upFc = 1;
+            q.hR = -((pr - g.hR + i0) / 2);
+            q.hC = -((pc - g.hC + j0) / 2);
+            q.OH = OHp;
+            q.OW = OWp;
+            q.flip = 0;
+            const IgemmCfg c = choose_igemm_frame(q, m);
+            if (c.MB == 0) return false;
+            Phase f;
+            f.i0 = i0;
+            f.j0 = j0;
+            f.step = 2;
+            f.FHf = g.FH;
+            f.FWf = g.FW;
+            f.flip = g.flip;
+            f.oRow0 = pr;
+            f.oCol0 = pc;
+            f.oStride = 2;
+            f.OHf = g.OH;
+            f.OWf = g.OW;
+            ps.g[ps.n] = q;
+            ps.c[ps.n] = c;
+            ps.p[ps.n] = f;
+            ps.n++;
+        }
+    if (ps.n == 0) return false;
+    *out = ps;
+    return true;
+}
 
 void describe_into(const cogconv_plan_s* plan, char* buf, size_t len);
 
@@ -476,11 +532,20 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         tmp.scfg = v.s;
         tmp.icfg = v.i;
         tmp.fcfg = v.f;
+        tmp.ps = v.ps;
         char name[192];
         describe_into(&tmp, name, sizeof name);
         for (const std::string& n : seen)
             if (n == name) return;
         seen.emplace_back(name);
+        out.push_back(v);
+    };
+    auto add_up = [&](const PhaseSet& ps) {
+        Variant v;
+        std::memset(&v, 0, sizeof v);
+        v.kind = KK_IGEMM_UP;
+        v.ps = ps;
+        v.workspace = 0;
         out.push_back(v);
     };
     if (desc->precision != COGCONV_FP32) {
@@ -512,8 +577,11 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
     const bool fa_ok = choose_fa_igemm(desc, g, m, &fa);
     const SmallCfg sm = choose_small(g, m);
     const DirectCfg dr = choose_direct(g, 4);
+    PhaseSet up;
+    const bool up_ok = choose_igemm_up(desc, g, m, &up);
     // Heuristic default.
-    if (igemm_ok) add(KK_IGEMM, nullptr, nullptr, &ig, nullptr);
+    if (up_ok) add_up(up);
+    else if (igemm_ok) add(KK_IGEMM, nullptr, nullptr, &ig, nullptr);
     else if (fa_ok) add(KK_FA_IGEMM, nullptr, nullptr, nullptr, &fa);
     else if (small_out) add(KK_SMALL_OUTPUT, nullptr, &sm, nullptr, nullptr);
     else add(KK_DIRECT, &dr, nullptr, nullptr, nullptr);
@@ -549,7 +617,8 @@ bool kind_matches_algo(int kind, int algo) {
     case COGCONV_ALGO_DIRECT: return kind == KK_DIRECT;
     case COGCONV_ALGO_SMALL_OUTPUT: return kind == KK_SMALL_OUTPUT;
     case COGCONV_ALGO_IMPLICIT_GEMM:
-        return kind == KK_IGEMM || kind == KK_FA_IGEMM || kind == KK_IGEMM_LP || kind == KK_FA_IGEMM_LP;
+        return kind == KK_IGEMM || kind == KK_FA_IGEMM || kind == KK_IGEMM_LP || kind == KK_FA_IGEMM_LP ||
+               kind == KK_IGEMM_UP;
     default: return true;
     }
 }
@@ -605,6 +674,7 @@ int make_plan(const cogconv_desc* desc, int vi, cogconv_plan* plan) {
             tmp.scfg = vars[tuned].s;
             tmp.icfg = vars[tuned].i;
             tmp.fcfg = vars[tuned].f;
+            tmp.ps = vars[tuned].ps;
             char name[192];
             describe_into(&tmp, name, sizeof name);
             if (tunedName == name) pick = tuned;
@@ -623,6 +693,7 @@ int make_plan(const cogconv_desc* desc, int vi, cogconv_plan* plan) {
     p->scfg = v.s;
     p->icfg = v.i;
     p->fcfg = v.f;
+    p->ps = v.ps;
     p->workspace = v.workspace;
     p->variant = pick;
     // Algorithmic work: every multiply-add whose tap multiplies a stored
@@ -736,6 +807,14 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
                  mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->icfg.MB, plan->icfg.lds_bytes);
         break;
     }
+    case KK_IGEMM_UP: {
+        int n = snprintf(buf, len, "igemm_f32_up2 %s out=%dx%dx%d phases=%d", mode_name(plan->desc.mode), plan->m.nOut,
+                         g.OH, g.OW, plan->ps.n);
+        for (int i = 0; i < plan->ps.n && n > 0 && (size_t)n < len; i++)
+            n += snprintf(buf + n, len - n, " %dx%d:MB=%d,RC=%d", plan->ps.g[i].FH, plan->ps.g[i].FW,
+                          plan->ps.c[i].MB, plan->ps.c[i].RC);
+        break;
+    }
     case KK_FA_IGEMM_LP: {
         static const char* pn[] = {"fp32", "bf16", "fp16", "split_bf16"};
         snprintf(buf, len, "fa_igemm_%s %s out=%dx%dx%d MB=%d CBW=%d RR=%d CW=%d groups=%d splits=%d lds=%zu",
@@ -787,6 +866,10 @@ extern "C" int cogconv_launch(cogconv_plan plan, const float* in0, const float* 
         break;
     case KK_IGEMM_LP:
         e = launch_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->icfg, in0, in1, out, workspace, s);
+        break;
+    case KK_IGEMM_UP:
+        for (int i = 0; i < plan->ps.n && e == hipSuccess; i++)
+            e = launch_igemm_f32_phase(plan->ps.g[i], plan->m, plan->ps.c[i], plan->ps.p[i], in0, in1, out, s);
         break;
     case KK_FA_IGEMM_LP:
         e = launch_fa_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->fcfg, in0, in1, out,

@@ -305,6 +305,33 @@ def test_c3_filter_adjoint_fp32_full_size():
     assert_parity(got, ref, mag, what="c3 FA")
 
 
+UP_CASES = [(mode, border, F) for mode in ("pf_brs", "bpf_brs") for border in (O.ZERO, O.CYCLIC, O.VALID, O.FULL)
+            for F in (3, 5, 7, 9)]
+
+
+@pytest.mark.parametrize("mode,border,F", UP_CASES)
+def test_upsampled_implicit_gemm_gpu(mode, border, F):
+    """UpsampleInputConvolution(2) on the frame block-reduce modes through the
+    four-phase implicit GEMM (each phase a dense sub-filter convolution)."""
+    rng = np.random.default_rng(zlib.crc32(repr(("up", mode, border, F)).encode()))
+    B, N, K, H, W = 2, 5, 3, 13, 22
+    a = rng.uniform(-1, 1, (B * N, H, W)).astype(np.float32)
+    k = rng.uniform(-1, 1, (K * N, F, F)).astype(np.float32)
+    orient = O.CONV if mode == "bpf_brs" else O.XCORR
+    od, cd = desc_pair(MODES[mode], border, orient, O.UP, 2, B, a.shape, k.shape, algo=L.ALGO_IMPLICIT_GEMM)
+    try:
+        O.output_type(od)
+    except O.OracleError:
+        return
+    plan = L.Plan(cd)
+    announce("up2 igemm", mode, border, F, plan.description)
+    assert plan.description.startswith("igemm_f32_up2")
+    plan.close()
+    got = run_gpu(cd, a, k)
+    ref, mag = oracle_ref(od, a, k)
+    assert_parity(got, ref, mag, what="up2 %s b%d F%d" % (mode, border, F))
+
+
 def test_c4_plane_by_plane_full_size():
     """PlaneByPlane 1024x1024 scalar field, 11x11 Gaussian-plus-noise filter, BorderClamp."""
     rng = np.random.default_rng(44)
