@@ -279,8 +279,8 @@ def serving_workload(args):
     filters with 16-64 outputs, 70/20/10 ProjectFrame / BackProjectFrame /
     FilterAdjoint (all with block reduction), 64 filter banks resident in HBM,
     an 8 GiB admission budget.  Run once with FP32 plans and once with BF16
-    plans for the frame modes (FilterAdjoint stays FP32), open loop at the
-    Poisson rate and closed loop (all requests queued at once) for capacity."""
+    plans, open loop at the Poisson rate and closed loop (all requests queued
+    at once) for capacity."""
     import cogconv._lib as L
     L.init(0)
     rng = np.random.default_rng(2000)
@@ -316,7 +316,7 @@ def serving_workload(args):
                                1, (2, s, s, K), (2, F, F, K * N), precision), True, K * s * s, N * s * s
         d = s - F + 1
         return L.make_desc(L.FILTER_ADJOINT_BLOCK_REDUCE_SUM, L.BORDER_VALID, L.CROSS_CORRELATION, L.NO_SAMPLING, 1,
-                           1, (2, s, s, N), (2, d, d, K), L.FP32), False, N * s * s, K * N * F * F
+                           1, (2, s, s, N), (2, d, d, K), precision), False, N * s * s, K * N * F * F
 
     # Resident slots: field state already in HBM (graph state), so requests move no data over PCIe.
     dpool = [(L.DeviceBuffer(4 * maxe), L.DeviceBuffer(4 * maxe), L.DeviceBuffer(4 * maxe)) for _ in range(SLOTS)]

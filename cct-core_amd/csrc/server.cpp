@@ -347,8 +347,11 @@ struct cogconv_server_s {
             if (inflight.empty()) {
                 cv_work.wait(lk, [&] { return stop || !pending.empty(); });
             } else {
+                // Launches in flight: poll again at once (a timed sleep rounds up to
+                // the kernel's timer slack, tens of microseconds, which would cap
+                // the request rate); yield so submitters can take the lock.
                 lk.unlock();
-                std::this_thread::sleep_for(std::chrono::microseconds(10));
+                std::this_thread::yield();
                 lk.lock();
             }
         }
