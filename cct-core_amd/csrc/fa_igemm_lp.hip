@@ -20,10 +20,15 @@
 // waves' sums meet in LDS and one partial dW per block is written, added
 // afterwards in a fixed order (launch_reduce_partials).
 #include "lp.h"
+#include "stage.h"
 
 namespace cogconv {
 
 hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long total, hipStream_t s);
+
+// Lane byte offset with bit 31 set: past every staged buffer (each is under
+// 2^31 bytes), so the buffer unit's range check answers the load with 0.
+constexpr int FA_OOB = (int)0x80000000u;
 
 template <int MB, int CBG, int PREC>
 __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
@@ -79,14 +84,23 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
 #pragma unroll
         for (int q = 0; q < CBG; q++) acc[i][q] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-    // Staging lanes: LPR lanes per row, each holding 8 consecutive columns.
+    // Staging lanes: LPR lanes per staged row, each holding 8 consecutive
+    // columns.  dy: RPP consecutive planes per wave pass, lane row myRow.  x:
+    // one row per pass, lane (copy C = q*RPP + lane/LPR, column group) for
+    // pass q of QP per row.  Loads go through buffer resources with a
+    // wave-uniform row offset and lane offsets fixed per tap-column chunk;
+    // masked samples use an out-of-range lane offset and read 0.
     const int LPR = c.DWp >> 3;
-    const int RPP = 64 / LPR;  // rows per wave pass
+    const int RPP = 64 / LPR;  // dy rows per wave pass; also copies per x pass
     const int myRow = lane / LPR, cgrp = lane - myRow * LPR;
-    const float invOW = 1.0f / OW, invXR = 1.0f / c.xRows;
-    int dColE[8];
-    unsigned dInMask = 0;
+    const int QP = (OW + RPP - 1) / RPP;  // x passes per staged row
+    int dv[8];   // dy lane byte offsets (plane myRow of the pass, tap column), or FA_OOB
+    int xv[8];   // x lane byte offsets (copy C of pass 0, column), or FA_OOB
     int jChunk = -1, j0 = 0;
+    // Rows need no per-sample mask when every staged dy plane is real (K fills
+    // the 16-plane blocks) and no tap row is off the up-sampling lattice; x
+    // rows past the field's last row are never read by a valid output.
+    const bool dMask = K != MT || g.upFr > 1;
 
     const long items = (long)m.B * c.colChunks * c.rowChunks;
     const long it0 = items * s / c.splits, it1 = items * (s + 1) / c.splits;
@@ -100,7 +114,6 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
             jChunk = cc;
             j0 = cc * c.CW;
             const int jEnd = min(j0 + c.CW, g.FWe);
-            dInMask = 0;
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const int jj = j0 + 8 * cgrp + e;
@@ -110,8 +123,9 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
                     on = on && (tj % g.upFc) == 0;
                     tj /= g.upFc;
                 }
-                dColE[e] = min(max(tj, 0), g.FW - 1);
-                dInMask |= on ? 1u << e : 0u;
+                dv[e] = on ? 4 * (myRow * FHW + min(max(tj, 0), g.FW - 1)) : FA_OOB;
+                const int xc = j0 + myRow * g.dnC + 8 * cgrp + e;  // copy C = myRow in pass q = 0
+                xv[e] = (myRow < OW && xc < g.W) ? 4 * xc : FA_OOB;
             }
         }
         // Per-lane table (lane = staged tap row il < RR <= 8) of the dy row
@@ -131,25 +145,28 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
         }
         __syncthreads();
         constexpr int DU = 4;  // wave passes per batch: 32 loads in flight per lane
-        // dy rows (il, k), k fastest; a pass's rows share il (RPP divides MT).
+        // dy rows (il, k), k fastest; a pass's RPP rows share il (RPP divides MT).
         const int dRows = MT * rr;
-        const float* DYb = DY + (size_t)b * K * FHW;
+        const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * FHW, (uint32_t)K * FHW * 4u);
         for (int pb = wave * RPP; pb < dRows; pb += 4 * RPP * DU) {
             float v[DU][8];
-            bool onD[DU];
             int dstD[DU];
 #pragma unroll
             for (int d = 0; d < DU; d++) {
                 const int p0 = pb + d * 4 * RPP;            // wave-uniform
                 const int il = min(p0 / MT, 7);
-                const int row = p0 + myRow;
-                const int k = row - (p0 / MT) * MT;
-                const bool live = p0 < dRows;
-                onD[d] = live && k < K && ((tiMask >> il) & 1);
-                dstD[d] = live ? (il * MT + k) * DP + 8 * cgrp : -1;
-                const float* src = DYb + min(k, K - 1) * FHW + __builtin_amdgcn_readlane(tiOffLane, il);
+                const int k0 = p0 - (p0 / MT) * MT;
+                dstD[d] = p0 < dRows ? (il * MT + k0 + myRow) * DP + 8 * cgrp : -1;
+                // Without the mask K == MT, so k0 + myRow < K for every lane.
+                const int soff = 4 * (min(k0, K - 1) * FHW + __builtin_amdgcn_readlane(tiOffLane, il));
+                if (dMask) {  // wave-uniform, decided per block
+                    const bool rowOn = ((tiMask >> il) & 1) && k0 + myRow < K;
 #pragma unroll
-                for (int e = 0; e < 8; e++) v[d][e] = src[dColE[e]];
+                    for (int e = 0; e < 8; e++) v[d][e] = buffer_load_f32(dyr, rowOn ? dv[e] : FA_OOB, soff);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; e++) v[d][e] = buffer_load_f32(dyr, dv[e], soff);
+                }
             }
             __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll
@@ -157,45 +174,41 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
                 V hv, lv;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
-                    const float a = (onD[d] && ((dInMask >> e) & 1)) ? v[d][e] : 0.f;
-                    const T h = (T)a;
+                    const T h = (T)v[d][e];
                     hv[e] = h;
-                    if (SPLIT) lv[e] = (T)(a - (float)h);
+                    if (SPLIT) lv[e] = (T)(v[d][e] - (float)h);
                 }
                 const bool live = dstD[d] >= 0;
                 *reinterpret_cast<V*>(live ? sD + dstD[d] : sJunk + 8 * lane) = hv;
                 if (SPLIT) *reinterpret_cast<V*>(live ? sD + dTerm + dstD[d] : sJunk + 512 + 8 * lane) = lv;
             }
         }
-        // x rows (nl, r, C), C fastest: copy C holds x row i0 + r of plane
-        // n_lo + nl from column j0 + C*dn on.
-        const int xRowsAll = nCnt * c.xRows * OW;
-        const float* Xb = X + ((size_t)b * N + n_lo) * HWx;
-        for (int pb = wave * RPP; pb < xRowsAll; pb += 4 * RPP * DU) {
+        // x rows (nl, r): one staged row per pass q of QP; lane (copy C, column
+        // group) reads x columns j0 + C*dn + 8*cgrp .. +7 of row i0 + r of
+        // plane n_lo + nl, copy C holding the row shifted left by C*dn.
+        const int xPasses = nCnt * c.xRows * QP;
+        const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + ((size_t)b * N + n_lo) * HWx, (uint32_t)nCnt * HWx * 4u);
+        for (int tb = wave; tb < xPasses; tb += 4 * DU) {
             float v[DU][8];
-            bool onX[DU];
             int dstX[DU];
-            unsigned colMask[DU];
 #pragma unroll
             for (int d = 0; d < DU; d++) {
-                const int row = pb + d * 4 * RPP + myRow;
-                const bool live = row < xRowsAll;
-                const int t = fast_div(row, OW, invOW);
-                const int C = row - t * OW;
-                const int nl = fast_div(t, c.xRows, invXR);
-                const int r = t - nl * c.xRows;
-                const int gy = i0 + r;
-                onX[d] = live && gy < g.H;
-                dstX[d] = live ? row * DP + 8 * cgrp : -1;
-                const int col0 = j0 + C * g.dnC + 8 * cgrp;
-                const float* src = Xb + min(nl, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W;
-                unsigned cm = 0;
+                const int t = tb + 4 * d;                    // wave-uniform
+                const int u = QP == 1 ? t : t / QP, q = t - u * QP;
+                const int nl = u / c.xRows, r = u - nl * c.xRows;
+                const int C = q * RPP + myRow;
+                dstX[d] = (t < xPasses && C < OW) ? ((nl * c.xRows + r) * OW + C) * DP + 8 * cgrp : -1;
+                const int soff = 4 * (min(nl, nCnt - 1) * HWx + min(i0 + r, g.H - 1) * g.W);
+                if (QP == 1) {  // wave-uniform: the lane offsets of the chunk apply as they are
 #pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    v[d][e] = src[min(col0 + e, g.W - 1)];
-                    cm |= (col0 + e < g.W) ? 1u << e : 0u;
+                    for (int e = 0; e < 8; e++) v[d][e] = buffer_load_f32(xr, xv[e], soff);
+                } else {
+                    const int colStart = j0 + C * g.dnC + 8 * cgrp;
+#pragma unroll
+                    for (int e = 0; e < 8; e++)
+                        v[d][e] = buffer_load_f32(xr, (C < OW && colStart + e < g.W) ? 4 * (colStart + e) : FA_OOB,
+                                                  soff);
                 }
-                colMask[d] = cm;
             }
             __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll
@@ -203,10 +216,9 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
                 V hv, lv;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
-                    const float a = (onX[d] && ((colMask[d] >> e) & 1)) ? v[d][e] : 0.f;
-                    const T h = (T)a;
+                    const T h = (T)v[d][e];
                     hv[e] = h;
-                    if (SPLIT) lv[e] = (T)(a - (float)h);
+                    if (SPLIT) lv[e] = (T)(v[d][e] - (float)h);
                 }
                 const bool live = dstX[d] >= 0;
                 *reinterpret_cast<V*>(live ? sX + dstX[d] : sJunk + 8 * lane) = hv;

@@ -340,7 +340,7 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
 // BF16 / FP16 / split-BF16 MFMA split-reduction GEMM for
 // FilterAdjointBlockReduceSum, BorderValid (k_fa_igemm_lp).
 bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out, int cbw = 0,
-                        long partials = 0) {
+                        long partials = 0, int cw = 0) {
     if (d->mode != COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM || d->border != COGCONV_BORDER_VALID || d->in0.dims != 2)
         return false;
     FaCfg c;
@@ -357,6 +357,10 @@ bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m,
     // Tap columns per chunk: 32, 64 or 128 (whole 32-deep MFMA slices; 8
     // columns per staging lane).
     c.CW = g.FWe <= 32 ? 32 : (g.FWe <= 64 ? 64 : 128);
+    if (cw) {
+        if (cw > c.CW) return false;  // no wider than the heuristic chunk
+        c.CW = cw;
+    }
     c.DWp = c.CW;
     c.colChunks = (g.FWe + c.CW - 1) / c.CW;
     c.MTp = 16 * c.MB;
@@ -552,12 +556,13 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         FaCfg flp;
         if (choose_fa_igemm_lp(desc, g, m, &flp)) {
             add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &flp);
-            for (int cbw : {3, 6, 9})
-                for (long parts : {128L, 256L, 512L}) {
-                    FaCfg c2;
-                    if (choose_fa_igemm_lp(desc, g, m, &c2, cbw, parts))
-                        add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &c2);
-                }
+            for (int cw : {0, 64, 32})
+                for (int cbw : {3, 6, 9})
+                    for (long parts : {256L, 512L}) {
+                        FaCfg c2;
+                        if (choose_fa_igemm_lp(desc, g, m, &c2, cbw, parts, cw))
+                            add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &c2);
+                    }
             return;
         }
         IgemmCfg lp;

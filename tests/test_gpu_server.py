@@ -42,15 +42,17 @@ def make_requests(rng, banks, n):
 
 
 def run(server, reqs, one_at_a_time):
-    outs = []
-    tickets = []
-    for cd, od, a, b, bid, _ in reqs:
-        out = np.zeros(O.output_type(od)[3:4] + O.output_type(od)[1:3], np.float32)
-        t = server.submit(cd, a, out, bank=bid, in1=b)
-        outs.append(out)
-        tickets.append(t)
-        if one_at_a_time:
-            server.wait(t)
+    """Alone: one request at a time.  In the mix: all requests queued natively at
+    once (cogconv_server_submit_timed, zero arrival times), so that admission
+    finds a backlog and stacks requests of one bucket into shared launches."""
+    outs = [np.zeros(O.output_type(od)[3:4] + O.output_type(od)[1:3], np.float32) for _, od, *_ in reqs]
+    if one_at_a_time:
+        for (cd, od, a, b, bid, _), out in zip(reqs, outs):
+            server.wait(server.submit(cd, a, out, bank=bid, in1=b))
+        return outs
+    rq = [L.Server.request(cd, a, out, bank=bid, in1=b) for (cd, od, a, b, bid, _), out in zip(reqs, outs)]
+    tickets = server.submit_timed(rq, np.zeros(len(rq)))
+    assert all(tickets)
     for t in tickets:
         server.wait(t)
     return outs
