@@ -13,7 +13,8 @@
 // tap columns of one tap row (the four k-lanes, lane/16).  A block (4 wave64)
 // owns a column group of CBG 16-column blocks and all MB row blocks.  It
 // stages RR tap rows of all K dy planes of one image (stored k-fastest with a
-// row pitch of 16 mod 32 words, so the A-operand reads are bank-conflict free)
+// row pitch of 17 mod 32 words, spreading both the staging stores and the
+// A-operand reads over the banks)
 // plus the x rows they touch, for the input planes of its column group; staging
 // runs one row per wave pass with lanes across columns.  Wave w reduces tap
 // rows w, w+4, .. for every tile of the group, and the four waves' sums meet in
@@ -29,13 +30,16 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 template <int MB, int CBG>
 __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
                                                       const float* __restrict__ DY, float* __restrict__ ws) {
-    // Staged dy rows are k-fastest with a pitch of 16*MB padded to 16 mod 32 words.
-    constexpr int MTP = MB == 1 ? 16 : (MB == 4 ? 80 : 48);
+    // Staged dy rows are k-fastest with a pitch of 16*MB padded to 17 mod 32
+    // words: the staging stores (lanes along tap columns, stride MTP) hit 32
+    // different banks, and the A-operand reads ((jb+g4)*MTP + l16) of a
+    // half-wave hit at most one bank twice.
+    constexpr int MTP = MB == 1 ? 17 : (MB == 4 ? 81 : 49);
     extern __shared__ float smem[];
     const int MTp = MTP;
     float* sD = smem;                                  // [RR][DWp][MTP]
-    float* sX = smem + (size_t)c.RR * c.DWp * MTP;     // [nCnt][xRows][xCols]
-    float* sJunk = sX + (size_t)c.nCntMax * c.xRows * c.xCols;  // 128 scratch slots for masked-off stores
+    float* sX = smem + (size_t)c.RR * c.DWp * MTP;     // [nCnt][xRows][xColsP], plane pitch xPlaneP
+    float* sJunk = sX + (size_t)c.nCntMax * c.xPlaneP;  // 128 scratch slots for masked-off stores
     const int FHW = g.FH * g.FW;                       // 32-bit offsets: the plan checks the sizes
     const int HWx = g.H * g.W;
 
@@ -60,7 +64,7 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
     const int n_lo = colBase / P;
     const int n_hi = min(N - 1, (min(colBase + CBG * 16, Ncols) - 1) / P);
     const int nCnt = n_hi - n_lo + 1;
-    const int xPlane = c.xRows * c.xCols;
+    const int xPlane = c.xPlaneP;
 
     int xoff[CBG];
 #pragma unroll
@@ -68,9 +72,9 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         const int j = colBase + q * 16 + l16;
         int off = 0;
         if (j < Ncols) {
-            const int n = j / P, rem = j - n * P;
-            const int R = rem / g.OW, C = rem - R * g.OW;
-            off = (n - n_lo) * xPlane + R * g.dnR * c.xCols + C * g.dnC;
+            const int n = fast_div(j, P, 1.0f / P), rem = j - n * P;
+            const int R = fast_div(rem, g.OW, 1.0f / g.OW), C = rem - R * g.OW;
+            off = (n - n_lo) * xPlane + R * g.dnR * c.xColsP + C * g.dnC;
         }
         xoff[q] = off + g4;
     }
@@ -117,7 +121,7 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         // clamped in-bounds offsets and with no branch between them, then the
         // masked LDS stores; rows past the end and lanes past the row width
         // store into a scratch slot instead of branching.
-        constexpr int U = 16;
+        constexpr int U = 12;
         constexpr int MT = 16 * MB;    // dy planes the MFMAs read (K padded with zeros)
         const int dRows = MT * rr;     // (il, k); the pitch padding beyond MT is never read
         const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * FHW, (uint32_t)K * FHW * 4u);
@@ -136,53 +140,61 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
             tiOffLane = min(max(ti, 0), g.FH - 1) * g.FW;
             tiMask = __ballot(on);
         }
+        const int vd0 = dIn0 ? 4 * dCol0 : STAGE_OOB, vd1 = dIn1 ? 4 * dCol1 : STAGE_OOB;
         for (int base = wave; base < dRows; base += 4 * U) {
-            int offU[U], dstU[U];
-            bool onU[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
+            auto drow = [&](int u, int& soff, bool& on) {
                 const int row = base + 4 * u;
                 const int il = min(row / MT, 63);
                 const int k = row - (row / MT) * MT;
-                offU[u] = min(k, K - 1) * FHW + __builtin_amdgcn_readlane(tiOffLane, il);
-                onU[u] = row < dRows && k < K && ((tiMask >> il) & 1);
-                dstU[u] = row < dRows ? il * c.DWp * MTP + k : -1;
-            }
+                soff = 4 * (min(k, K - 1) * FHW + __builtin_amdgcn_readlane(tiOffLane, il));
+                on = row < dRows && k < K && ((tiMask >> il) & 1);
+            };
+            auto ddst = [&](int u) {
+                const int row = base + 4 * u;
+                const int il = row / MT;
+                return row < dRows ? il * c.DWp * MTP + (row - il * MT) : -1;
+            };
             if (dTwo)
-                stage_rows<true, U>(dyr, offU, onU, dstU, sD, sJunk, lane, c.DWp, MTP, dCol0, dCol1, dIn0, dIn1);
+                stage_rows_fn<true, U>(dyr, drow, ddst, sD, sJunk, lane, c.DWp, MTP, vd0, vd1);
             else
-                stage_rows<false, U>(dyr, offU, onU, dstU, sD, sJunk, lane, c.DWp, MTP, dCol0, dCol1, dIn0, dIn1);
+                stage_rows_fn<false, U>(dyr, drow, ddst, sD, sJunk, lane, c.DWp, MTP, vd0, vd1);
         }
         const int xRowsAll = nCnt * c.xRows;  // (plane, row)
         const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + ((size_t)b * N + n_lo) * HWx, (uint32_t)nCnt * HWx * 4u);
+        const int vx0 = xIn0 ? 4 * xCol0 : STAGE_OOB, vx1 = xIn1 ? 4 * xCol1 : STAGE_OOB;
         for (int base = wave; base < xRowsAll; base += 4 * U) {
-            int offU[U], dstU[U];
-            bool onU[U];
-            {
-                int nlq = base / c.xRows;
-                int lrq = base - nlq * c.xRows;
-#pragma unroll
-                for (int u = 0; u < U; u++) {
-                    const int gy = i0 + lrq;
-                    const bool live = base + 4 * u < xRowsAll;
-                    onU[u] = live && gy < g.H;
-                    offU[u] = min(nlq, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W;
-                    dstU[u] = live ? nlq * xPlane + lrq * c.xCols : -1;
+            // (plane, row) of the batch's rows, advanced incrementally in the
+            // load pass and again in the store pass (scalar work).
+            const int nl0 = base / c.xRows, lr0 = base - nl0 * c.xRows;
+            int nlq = nl0, lrq = lr0;
+            auto xrow = [&](int u, int& soff, bool& on) {
+                if (u) {
                     lrq += 4;
                     while (lrq >= c.xRows) { lrq -= c.xRows; nlq++; }
                 }
-            }
+                const int gy = i0 + lrq;
+                on = base + 4 * u < xRowsAll && gy < g.H;
+                soff = 4 * (min(nlq, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W);
+            };
+            int nls = nl0, lrs = lr0;
+            auto xdst = [&](int u) {
+                if (u) {
+                    lrs += 4;
+                    while (lrs >= c.xRows) { lrs -= c.xRows; nls++; }
+                }
+                return base + 4 * u < xRowsAll ? nls * xPlane + lrs * c.xColsP : -1;
+            };
             if (xTwo)
-                stage_rows<true, U>(xr, offU, onU, dstU, sX, sJunk, lane, c.xCols, 1, xCol0, xCol1, xIn0, xIn1);
+                stage_rows_fn<true, U>(xr, xrow, xdst, sX, sJunk, lane, c.xCols, 1, vx0, vx1);
             else
-                stage_rows<false, U>(xr, offU, onU, dstU, sX, sJunk, lane, c.xCols, 1, xCol0, xCol1, xIn0, xIn1);
+                stage_rows_fn<false, U>(xr, xrow, xdst, sX, sJunk, lane, c.xCols, 1, vx0, vx1);
         }
         __syncthreads();
         // Wave w reduces tap rows w, w+4, ... of the staged chunk for every
         // output tile of the block's column group.
         for (int il = wave; il < rr; il += 4) {
             const float* dRow = sD + ((size_t)il * c.DWp + g4) * MTp + l16;
-            const float* xRow = sX + il * c.xCols;
+            const float* xRow = sX + il * c.xColsP;
             for (int jb = 0; jb < c.DWp; jb += 4) {
                 float av[MB];
 #pragma unroll

@@ -71,8 +71,8 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
         const int j = colBase + q * 16 + l16;
         int off = 0;
         if (j < Ncols) {
-            const int n = j / P, rem = j - n * P;
-            const int R = rem / OW, C = rem - R * OW;
+            const int n = fast_div(j, P, 1.0f / P), rem = j - n * P;
+            const int R = fast_div(rem, OW, 1.0f / OW), C = rem - R * OW;
             off = (((n - n_lo) * c.xRows + R * g.dnR) * OW + C) * DP;
         }
         xoff[q] = off + 8 * g4;

@@ -132,29 +132,35 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
         // stores; items past the end and lanes past the tile width store into
         // a scratch slot instead of branching.
         const int nrows = c.RC * TRp;
-        constexpr int IU = 24;
+        constexpr int IU = 8;
+        const int vc0 = colIn0 ? 4 * colOff0 : STAGE_OOB, vc1 = colIn1 ? 4 * colOff1 : STAGE_OOB;
         for (int base = wave; base < nrows; base += 8 * IU) {
-            int offU[IU], dstU[IU];
-            bool onU[IU];
-            {
-                int rcq = base / TRp;
-                int tyq = base - rcq * TRp;
-#pragma unroll
-                for (int u = 0; u < IU; u++) {
-                    const int r = r0 + rcq;
-                    const bool live = base + 8 * u < nrows;
-                    const int ty = min(tyq, 63);  // tyq < TRp <= 64 for live items
-                    onU[u] = ((yMask >> ty) & 1) && r < m.R && live;
-                    offU[u] = min(r, m.R - 1) * HW + __builtin_amdgcn_readlane(yOffLane, ty);
-                    dstU[u] = live ? rcq * c.planeStride + tyq * TCp : -1;
+            // (plane, tile row) of the batch's rows, advanced incrementally in
+            // the load pass and again in the store pass (scalar work).
+            const int rc0 = base / TRp, ty0 = base - rc0 * TRp;
+            int rcq = rc0, tyq = ty0;
+            auto xrow = [&](int u, int& soff, bool& on) {
+                if (u) {
                     tyq += 8;
                     if (tyq >= TRp) { tyq -= TRp; rcq++; }
                 }
-            }
+                const int r = r0 + rcq;
+                const int ty = min(tyq, 63);  // tyq < TRp <= 64 for live rows
+                on = ((yMask >> ty) & 1) && r < m.R && base + 8 * u < nrows;
+                soff = 4 * (min(r, m.R - 1) * HW + __builtin_amdgcn_readlane(yOffLane, ty));
+            };
+            int rcs = rc0, tys = ty0;
+            auto xdst = [&](int u) {
+                if (u) {
+                    tys += 8;
+                    if (tys >= TRp) { tys -= TRp; rcs++; }
+                }
+                return base + 8 * u < nrows ? rcs * c.planeStride + tys * TCp : -1;
+            };
             if (twoPass)
-                stage_rows<true, IU>(xr, offU, onU, dstU, sB, sJunk, lane, TCp, 1, colOff0, colOff1, colIn0, colIn1);
+                stage_rows_fn<true, IU>(xr, xrow, xdst, sB, sJunk, lane, TCp, 1, vc0, vc1);
             else
-                stage_rows<false, IU>(xr, offU, onU, dstU, sB, sJunk, lane, TCp, 1, colOff0, colOff1, colIn0, colIn1);
+                stage_rows_fn<false, IU>(xr, xrow, xdst, sB, sJunk, lane, TCp, 1, vc0, vc1);
         }
         // Filter slice: thread -> (plane rc, filter mm); the F*F taps are
         // contiguous in W.  Loads are unconditional from a clamped pair.

@@ -166,6 +166,7 @@ struct FaCfg {
     int DWp;        // CW rounded up to 4
     int MTp;        // staged dy row pitch: 16*MB padded to 16 mod 32
     int xRows, xCols;
+    int xColsP, xPlaneP;  // LDS pitches of staged x rows and planes (FP32 kernel; bank-spread padding)
     int nCntMax;    // input planes staged per column group
     int colGroups;  // column groups of 16*CBW columns
     int rowChunks;  // ceil(FHe / RR)

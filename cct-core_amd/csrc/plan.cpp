@@ -307,15 +307,24 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     }
     c.CW = c.DWp;  // chunk starts on multiples of 4; the last chunk is masked at FWe
     c.colChunks = (g.FWe + c.CW - 1) / c.CW;
-    c.MTp = c.MB == 1 ? 16 : (c.MB == 4 ? 80 : 48);  // 16*MB padded to 16 mod 32 (k_fa_igemm_f32's MTP)
+    c.MTp = c.MB == 1 ? 17 : (c.MB == 4 ? 81 : 49);  // 16*MB padded to 17 mod 32 (k_fa_igemm_f32's MTP)
     if ((long)m.B * m.N * g.H * g.W >= (1L << 31) || (long)m.B * c.K * g.FH * g.FW >= (1L << 31)) return false;
     if ((long)m.N * g.H * g.W * 4 >= (1L << 31) || (long)c.K * g.FH * g.FW * 4 >= (1L << 31)) return false;
     const size_t red = (size_t)2 * c.MB * c.CBW * 4 * 64 * sizeof(float);  // cross-wave reduction buffer
     // Tap rows per item: as many as fit 80 KB (two blocks per CU); but at
     // least 4, one per wave, when the tap rows allow it and 160 KB holds them.
+    // x rows pitched at 4 mod 32 words and planes at 16 mod 32: the B-operand
+    // reads of a half-wave (output columns (n, R, C) x two tap-column lanes)
+    // then spread over the banks instead of piling onto a few.
+    c.xColsP = c.xCols + ((4 - c.xCols % 32) + 32) % 32;
+    auto plane_pitch = [&](int xRows) {
+        const int pp = xRows * c.xColsP;
+        return pp + ((16 - pp % 32) + 32) % 32;
+    };
     auto lds_for = [&](int rr) {
         const int xRows = rr + (g.OH - 1) * g.dnR;
-        const size_t bytes = ((size_t)c.MTp * rr * c.DWp + (size_t)c.nCntMax * xRows * c.xCols + 128) * sizeof(float);
+        const size_t bytes =
+            ((size_t)c.MTp * rr * c.DWp + (size_t)c.nCntMax * plane_pitch(xRows) + 128) * sizeof(float);
         return std::max(bytes, red);
     };
     c.RR = 0;
@@ -325,8 +334,9 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     if (c.RR < 4 && g.FHe >= 4 && lds_for(4) <= 160 * 1024) c.RR = 4;
     if (c.RR == 0) return false;
     c.xRows = c.RR + (g.OH - 1) * g.dnR;
+    c.xPlaneP = plane_pitch(c.xRows);
     // dy rows, x rows and 128 scratch slots for masked-off stores
-    c.lds_bytes = ((size_t)c.MTp * c.RR * c.DWp + (size_t)c.nCntMax * c.xRows * c.xCols + 128) * sizeof(float);
+    c.lds_bytes = ((size_t)c.MTp * c.RR * c.DWp + (size_t)c.nCntMax * c.xPlaneP + 128) * sizeof(float);
     c.lds_bytes = std::max(c.lds_bytes, red);
     c.rowChunks = (g.FHe + c.RR - 1) / c.RR;
     const long items = (long)m.B * c.colChunks * c.rowChunks;

@@ -47,4 +47,41 @@ __device__ __forceinline__ void stage_rows(__amdgpu_buffer_rsrc_t src, const int
     }
 }
 
+// Out-of-range lane byte offset: bit 31 set puts it past every staged buffer
+// (each under 2^31 bytes), so the buffer unit's range check returns 0.
+constexpr int STAGE_OOB = (int)0x80000000u;
+
+// Row staging with the row data computed on the fly: row(u) gives the row's
+// wave-uniform byte offset and whether it is in the field; dst(u) its LDS
+// element offset (< 0: not live, store to `junk`).  vcol0 / vcol1 are the
+// lanes' byte offsets of their columns (STAGE_OOB where the lane has no
+// column).  Rows off the field load through the out-of-range offset, so the
+// stores need no mask, and no per-row arrays are held across the loads (they
+// would spill scalar registers).
+template <bool TWO, int IU, class RowFn, class DstFn>
+__device__ __forceinline__ void stage_rows_fn(__amdgpu_buffer_rsrc_t src, RowFn row, DstFn dst, float* lds, float* junk,
+                                              int lane, int width, int pitch, int vcol0, int vcol1) {
+    float v0[IU], v1[IU];
+#pragma unroll
+    for (int u = 0; u < IU; u++) {
+        int soff;
+        bool on;
+        row(u, soff, on);
+        v0[u] = buffer_load_f32(src, on ? vcol0 : STAGE_OOB, soff);
+        if (TWO) v1[u] = buffer_load_f32(src, on ? vcol1 : STAGE_OOB, soff);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
+#pragma unroll
+    for (int u = 0; u < IU; u++) {
+        const int d = dst(u);
+        const bool live = d >= 0;  // wave-uniform
+        float* d0 = (live && lane < width) ? lds + d + lane * pitch : junk + lane;
+        *d0 = v0[u];
+        if (TWO) {
+            float* d1 = (live && lane + 64 < width) ? lds + d + (lane + 64) * pitch : junk + 64 + lane;
+            *d1 = v1[u];
+        }
+    }
+}
+
 }  // namespace cogconv
