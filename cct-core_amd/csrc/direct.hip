@@ -9,6 +9,7 @@
 // gfx950: 64-wide wavefronts, one wavefront per 64 output columns, the input
 // tile and filter chunk staged in LDS, and wavefront shuffles for reductions.
 #include "internal.h"
+#include "stage.h"
 
 namespace cogconv {
 
@@ -140,6 +141,240 @@ hipError_t launch_direct(const Geom& g, const PlaneMap& m, const DirectCfg& c, c
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_direct, dim3((unsigned)blocks), dim3(64 * c.RG), c.lds_bytes, s, g, m, c, a, f, out);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Register-tiled direct kernel for ops without sampling and filters of at most
+// 16 x 15 (FW a template constant): a block of 256 threads owns an (8 RPT) x 128
+// output tile of one output plane; thread (rg, cg) computes rows RPT rg ..
+// RPT rg + RPT - 1 and columns 4cg .. 4cg+3.  For each plane pair the input
+// tile (border applied) and the taps (flip applied, rows padded to a multiple
+// of 4) are staged in LDS.  Each staged input row is read into registers once
+// per thread (16-byte reads of 4 + FW - 1 values) and feeds the RPT output rows
+// it touches, so a tap row costs 4 RPT FW FMAs for about FW/4 + RPT FW/4 LDS
+// reads.  RPT = 2 or 1 shrinks the tile for planes that give too few blocks.
+// ---------------------------------------------------------------------------
+constexpr int RT_COLS = 128;
+
+// Index v of a plane dimension of n samples after the border policy: clamped,
+// wrapped, or (Zero, Valid, Full) false where the sample is outside.
+__device__ __forceinline__ bool border_index(int v, int n, int border, int& out) {
+    if (border == COGCONV_BORDER_CLAMP) {
+        out = min(max(v, 0), n - 1);
+        return true;
+    }
+    if (border == COGCONV_BORDER_CYCLIC) {
+        v %= n;
+        out = v < 0 ? v + n : v;
+        return true;
+    }
+    out = v;
+    return (unsigned)v < (unsigned)n;
+}
+
+__host__ __device__ inline int rt_seg_vecs(int fw) { return (fw + 6) / 4; }
+__host__ __device__ inline int rt_tap_pitch(int fw) { return (fw + 3) & ~3; }
+
+template <int FW, int RPT>
+__global__ __launch_bounds__(256) void k_direct_rt(Geom g, PlaneMap m, DirectCfg c, const float* __restrict__ A,
+                                                   const float* __restrict__ F, float* __restrict__ O) {
+    extern __shared__ float4 smem4[];
+    float* smem = reinterpret_cast<float*>(smem4);
+    constexpr int ROWS = 8 * RPT;         // output rows per tile
+    constexpr int NV = (FW + 6) / 4;      // float4 reads per staged row segment
+    constexpr int TP = (FW + 3) & ~3;     // tap row pitch
+    constexpr int NT = TP / 4;
+    const int TR = ROWS + g.FH - 1;       // staged rows
+    const int TCP = c.ldsCols;            // staged row pitch, a multiple of 4, >= 124 + 4 NV
+    float* st = smem;
+    float* sf = smem + (size_t)TR * TCP;  // FH x TP taps, cross-correlation order
+    int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int tc = bid % c.tilesC;
+    bid /= c.tilesC;
+    const int tr = bid % c.tilesR;
+    const int o = bid / c.tilesR;
+    const int tid = threadIdx.x;
+    const int cg = tid & 31, rg = tid >> 5;
+    const int orow0 = tr * ROWS, ocol0 = tc * RT_COLS;
+    const int y0 = orow0 - g.hR, x0 = ocol0 - g.hC;
+    const int TCc = RT_COLS + FW - 1;     // staged columns
+    // Staging: half-block h (wave-uniform) stages the first 128 columns of rows
+    // h, h+2, ... with lane column xs; the FW - 1 extra columns of every row are
+    // spread over the block (up to KE elements a thread).  Every load of a plane
+    // pair is issued before the first store.  Row indices after the border
+    // policy are wave-uniform; a sample outside the plane reads through the
+    // out-of-range offset (0).
+    constexpr int SR = 4 * RPT + 8;                                   // rows per half-block, FH <= 16
+    constexpr int KE = FW > 1 ? ((8 * RPT + 15) * (FW - 1) + 255) / 256 : 0;
+    const int h = __builtin_amdgcn_readfirstlane(tid >> 7), xs = tid & 127;  // 128 | wave: h is uniform
+    int xm;
+    const bool xmv = border_index(x0 + xs, g.W, g.border, xm);
+    const int vcol = xmv ? 4 * xm : STAGE_OOB;
+    const int E = TR * (FW - 1);
+    // Row table of the half-block: lane l holds the byte offset of row h + 2l
+    // after the border policy; bit l of rmask says whether it is in the plane.
+    // The load loop reads both with constant lane indices (scalar registers).
+    int rowoff;
+    unsigned long long rmask;
+    {
+        const int l = tid & 63, ty = h + 2 * l;
+        int yi;
+        const bool rv = l < SR && ty < TR && border_index(y0 + ty, g.H, g.border, yi);
+        rowoff = rv ? 4 * yi * g.W : 0;
+        rmask = __ballot(rv);
+    }
+    float acc[RPT][4];
+#pragma unroll
+    for (int a = 0; a < RPT; a++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[a][q] = 0.f;
+    for (int r = 0; r < m.R; r++) {
+        int pa, pf;
+        plane_pair(m, o, r, pa, pf);
+        const __amdgpu_buffer_rsrc_t ar = buffer_rsrc(A + (size_t)pa * g.H * g.W, 4u * (uint32_t)g.H * g.W);
+        const float* f = F + (size_t)pf * g.FH * g.FW;
+        __syncthreads();
+        for (int q = tid; q < g.FH * TP; q += 256) {
+            const int i = q / TP, j = q - i * TP;
+            sf[q] = j < FW ? read_tap(f, g, i, j) : 0.f;
+        }
+        float vm[SR], ve[KE > 0 ? KE : 1];
+        // Opaque copies: the per-row offsets, selects and store conditions are
+        // formed inside the loop instead of being hoisted out of it into
+        // registers held (and spilled) across the whole loop.
+        int ro = rowoff, vc = vcol, trr = TR;
+        unsigned long long rm = rmask;
+        asm volatile("" : "+v"(ro), "+v"(vc), "+s"(trr), "+s"(rm));
+#pragma unroll
+        for (int u = 0; u < SR; u++) {
+            const bool yv = (rm >> u) & 1ull;
+            vm[u] = buffer_load_f32(ar, yv ? vc : STAGE_OOB, __builtin_amdgcn_readlane(ro, u));
+        }
+#pragma unroll
+        for (int k = 0; k < KE; k++) {
+            const int e = tid + 256 * k;
+            const int ty = e / (FW > 1 ? FW - 1 : 1), tx = e - ty * (FW > 1 ? FW - 1 : 1);
+            int yi, xi;
+            const bool v = e < E && border_index(y0 + ty, g.H, g.border, yi) &&
+                           border_index(x0 + RT_COLS + tx, g.W, g.border, xi);
+            ve[k] = buffer_load_f32(ar, v ? 4 * (yi * g.W + xi) : STAGE_OOB, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);  // every load ahead of the first store
+#pragma unroll
+        for (int u = 0; u < SR; u++) {
+            const int ty = h + 2 * u;
+            if (ty < trr) st[ty * TCP + xs] = vm[u];
+        }
+#pragma unroll
+        for (int k = 0; k < KE; k++) {
+            const int e = tid + 256 * k;
+            const int ty = e / (FW > 1 ? FW - 1 : 1), tx = e - ty * (FW > 1 ? FW - 1 : 1);
+            if (e < E) st[ty * TCP + RT_COLS + tx] = ve[k];
+        }
+        __syncthreads();
+        // Staged row RPT rg + yy feeds output row RPT rg + a through tap row yy - a.
+        for (int yy = 0; yy < g.FH + RPT - 1; yy++) {
+            const float4* row = reinterpret_cast<const float4*>(st + (RPT * rg + yy) * TCP) + cg;
+            float seg[4 * NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                const float4 v = row[k];
+                seg[4 * k] = v.x;
+                seg[4 * k + 1] = v.y;
+                seg[4 * k + 2] = v.z;
+                seg[4 * k + 3] = v.w;
+            }
+#pragma unroll
+            for (int a = 0; a < RPT; a++) {
+                const int i = yy - a;
+                if (i < 0 || i >= g.FH) continue;
+                const float4* w4 = reinterpret_cast<const float4*>(sf + i * TP);
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    const float4 w = w4[t];
+                    const float wt[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++) {
+                        const int j = 4 * t + jj;
+                        if (j >= FW) break;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) acc[a][q] = fmaf(wt[jj], seg[q + j], acc[a][q]);
+                    }
+                }
+            }
+        }
+    }
+    float* out = O + (size_t)o * g.OH * g.OW;
+    const int col = ocol0 + 4 * cg;
+    // 16-byte stores where every row starts 16-byte aligned (then a column
+    // group is either wholly inside the row or wholly past it).
+    const bool vec = (g.OW & 3) == 0 && (reinterpret_cast<uintptr_t>(O) & 15) == 0;
+    if (vec) {
+#pragma unroll
+        for (int a = 0; a < RPT; a++) {
+            const int row = orow0 + RPT * rg + a;
+            if (row < g.OH && col < g.OW)
+                *reinterpret_cast<float4*>(out + (size_t)row * g.OW + col) =
+                    make_float4(acc[a][0], acc[a][1], acc[a][2], acc[a][3]);
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < RPT; a++) {
+            const int row = orow0 + RPT * rg + a;
+            if (row >= g.OH) continue;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (col + q < g.OW) out[(size_t)row * g.OW + col + q] = acc[a][q];
+        }
+    }
+}
+
+bool direct_rt_supported(const Geom& g) {
+    return g.upAr == 1 && g.upAc == 1 && g.dnR == 1 && g.dnC == 1 && g.upFr == 1 && g.upFc == 1 && g.FH >= 1 &&
+           g.FH <= 16 && g.FW >= 1 && g.FW <= 15 && (size_t)g.H * g.W * 4 < 0x80000000ull;
+}
+
+// Tile geometry and LDS of the register-tiled kernel with `rpt` (1, 2 or 4)
+// output rows per thread.  The staged row pitch is a multiple of 4 (16-byte
+// row reads) covering the 124 + 4 NV columns the last column group reads.
+void direct_rt_config(const Geom& g, int rpt, DirectCfg& c) {
+    c.RG = rpt;
+    c.FCR = g.FH;
+    c.FCC = g.FW;
+    c.ldsRows = 8 * rpt + g.FH - 1;
+    int p = RT_COLS + g.FW - 1;
+    if (p < 124 + 4 * rt_seg_vecs(g.FW)) p = 124 + 4 * rt_seg_vecs(g.FW);
+    c.ldsCols = (p + 3) & ~3;
+    c.tilesR = (g.OH + 8 * rpt - 1) / (8 * rpt);
+    c.tilesC = (g.OW + RT_COLS - 1) / RT_COLS;
+    c.lds_bytes = ((size_t)c.ldsRows * c.ldsCols + (size_t)g.FH * rt_tap_pitch(g.FW)) * sizeof(float);
+}
+
+template <int RPT>
+static hipError_t launch_rt(const Geom& g, const PlaneMap& m, const DirectCfg& c, const float* a, const float* f,
+                            float* out, hipStream_t s, dim3 grid) {
+    const dim3 block(256);
+    switch (g.FW) {
+#define RT_CASE(W) \
+    case W: hipLaunchKernelGGL((k_direct_rt<W, RPT>), grid, block, c.lds_bytes, s, g, m, c, a, f, out); break;
+        RT_CASE(1) RT_CASE(2) RT_CASE(3) RT_CASE(4) RT_CASE(5) RT_CASE(6) RT_CASE(7) RT_CASE(8) RT_CASE(9)
+        RT_CASE(10) RT_CASE(11) RT_CASE(12) RT_CASE(13) RT_CASE(14) RT_CASE(15)
+#undef RT_CASE
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_direct_rt(const Geom& g, const PlaneMap& m, const DirectCfg& c, const float* a, const float* f,
+                            float* out, hipStream_t s) {
+    const long blocks = (long)c.tilesC * c.tilesR * m.nOut;
+    if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    if (!direct_rt_supported(g) || c.ldsRows != 8 * c.RG + g.FH - 1) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)blocks);
+    if (c.RG == 4) return launch_rt<4>(g, m, c, a, f, out, s, grid);
+    if (c.RG == 2) return launch_rt<2>(g, m, c, a, f, out, s, grid);
+    if (c.RG == 1) return launch_rt<1>(g, m, c, a, f, out, s, grid);
+    return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------

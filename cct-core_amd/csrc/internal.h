@@ -122,6 +122,7 @@ enum KernelKind {
     KK_IGEMM_LP = 5,
     KK_FA_IGEMM_LP = 6,
     KK_IGEMM_UP = 7,
+    KK_DIRECT_RT = 8,
 };
 
 // LDS-tiled direct kernel configuration.
@@ -200,6 +201,11 @@ enum IgemmVariant {
 };
 
 // Kernel launchers (device code in *.hip).
+// Register-tiled direct kernel: no sampling, FH <= 16, FW <= 15.
+bool direct_rt_supported(const Geom& g);
+void direct_rt_config(const Geom& g, int rpt, DirectCfg& c);
+hipError_t launch_direct_rt(const Geom& g, const PlaneMap& m, const DirectCfg& c, const float* a, const float* f,
+                            float* out, hipStream_t s);
 hipError_t launch_direct(const Geom& g, const PlaneMap& m, const DirectCfg& c, const float* a, const float* f,
                          float* out, hipStream_t s);
 hipError_t launch_small_output(const Geom& g, const PlaneMap& m, const SmallCfg& c, const float* a, const float* f,

@@ -592,6 +592,14 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
     const bool fa_ok = choose_fa_igemm(desc, g, m, &fa);
     const SmallCfg sm = choose_small(g, m);
     const DirectCfg dr = choose_direct(g, 4);
+    DirectCfg drt[3];
+    const bool rt_ok = direct_rt_supported(g);
+    int rt_first = 0;
+    if (rt_ok) {
+        for (int i = 0; i < 3; i++) direct_rt_config(g, 4 >> i, drt[i]);
+        // The largest tile that still gives two blocks per CU.
+        while (rt_first < 2 && (long)drt[rt_first].tilesR * drt[rt_first].tilesC * m.nOut < 512) rt_first++;
+    }
     PhaseSet up;
     const bool up_ok = choose_igemm_up(desc, g, m, &up);
     // Heuristic default.
@@ -599,6 +607,7 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
     else if (igemm_ok) add(KK_IGEMM, nullptr, nullptr, &ig, nullptr);
     else if (fa_ok) add(KK_FA_IGEMM, nullptr, nullptr, nullptr, &fa);
     else if (small_out) add(KK_SMALL_OUTPUT, nullptr, &sm, nullptr, nullptr);
+    else if (rt_ok) add(KK_DIRECT_RT, &drt[rt_first], nullptr, nullptr, nullptr);
     else add(KK_DIRECT, &dr, nullptr, nullptr, nullptr);
     // Alternatives.
     if (igemm_ok) {
@@ -621,6 +630,8 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
             add(KK_SMALL_OUTPUT, nullptr, &c, nullptr, nullptr);
         }
     }
+    if (rt_ok)
+        for (const DirectCfg& c : drt) add(KK_DIRECT_RT, &c, nullptr, nullptr, nullptr);
     for (int rg : {4, 2, 1}) {
         const DirectCfg c = choose_direct(g, rg);
         add(KK_DIRECT, &c, nullptr, nullptr, nullptr);
@@ -629,7 +640,7 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
 
 bool kind_matches_algo(int kind, int algo) {
     switch (algo) {
-    case COGCONV_ALGO_DIRECT: return kind == KK_DIRECT;
+    case COGCONV_ALGO_DIRECT: return kind == KK_DIRECT || kind == KK_DIRECT_RT;
     case COGCONV_ALGO_SMALL_OUTPUT: return kind == KK_SMALL_OUTPUT;
     case COGCONV_ALGO_IMPLICIT_GEMM:
         return kind == KK_IGEMM || kind == KK_FA_IGEMM || kind == KK_IGEMM_LP || kind == KK_FA_IGEMM_LP ||
@@ -812,6 +823,11 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
         snprintf(buf, len, "direct %s out=%dx%dx%d tile=%dx64 fchunk=%dx%d lds=%zu", mode_name(plan->desc.mode),
                  plan->m.nOut, g.OH, g.OW, 4 * plan->dcfg.RG, plan->dcfg.FCR, plan->dcfg.FCC, plan->dcfg.lds_bytes);
         break;
+    case KK_DIRECT_RT:
+        snprintf(buf, len, "direct_rt %s out=%dx%dx%d tile=%dx128 thread=%dx4 filter=%dx%d lds=%zu",
+                 mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, 8 * plan->dcfg.RG, plan->dcfg.RG, g.FH, g.FW,
+                 plan->dcfg.lds_bytes);
+        break;
     case KK_SMALL_OUTPUT:
         snprintf(buf, len, "small_output %s out=%dx%dx%d splits=%d", mode_name(plan->desc.mode), plan->m.nOut, g.OH,
                  g.OW, plan->scfg.splits);
@@ -869,6 +885,9 @@ extern "C" int cogconv_launch(cogconv_plan plan, const float* in0, const float* 
     switch (plan->kind) {
     case KK_DIRECT:
         e = launch_direct(plan->g, plan->m, plan->dcfg, in0, in1, out, s);
+        break;
+    case KK_DIRECT_RT:
+        e = launch_direct_rt(plan->g, plan->m, plan->dcfg, in0, in1, out, s);
         break;
     case KK_SMALL_OUTPUT:
         e = launch_small_output(plan->g, plan->m, plan->scfg, in0, in1, out, static_cast<float*>(workspace), s);

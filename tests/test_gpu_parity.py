@@ -201,6 +201,39 @@ def test_every_kernel_family_gpu(mode, border, sampling, step, algo):
     assert_parity(got, ref, mag, what="%s algo %d" % (mode, algo))
 
 
+RT_CASES = []
+for _fh, _fw in ((1, 1), (2, 4), (5, 5), (3, 8), (11, 11), (16, 15), (7, 13), (15, 15)):
+    for _border in (O.ZERO, O.CLAMP, O.CYCLIC, O.VALID, O.FULL):
+        if (_fh % 2 and _fw % 2) or _border in (O.VALID, O.FULL):  # even filters need Valid or Full
+            RT_CASES.append((_fh, _fw, _border))
+
+
+@pytest.mark.parametrize("fh,fw,border", RT_CASES)
+@pytest.mark.parametrize("mode", ("pbp", "pf"))
+def test_register_tiled_direct_gpu(fh, fw, border, mode):
+    """The register-tiled direct kernel (no sampling, filters up to 16x15) on planes that leave ragged
+    32x128 output tiles, for every border, both orientations."""
+    rng = np.random.default_rng(zlib.crc32(repr((fh, fw, border, mode)).encode()))
+    H, W = 45 + fh, 150 + 2 * fw
+    if mode == "pbp":
+        B, a = 1, rng.uniform(-1, 1, (3, H, W)).astype(np.float32)
+        k = rng.uniform(-1, 1, (3, fh, fw)).astype(np.float32)
+    else:
+        B, N, K = 2, 2, 3
+        a = rng.uniform(-1, 1, (B * N, H, W)).astype(np.float32)
+        k = rng.uniform(-1, 1, (K * N, fh, fw)).astype(np.float32)
+    for orient in (O.XCORR, O.CONV):
+        od, cd = desc_pair(MODES[mode], border, orient, O.NONE, 1, B, a.shape, k.shape, algo=L.ALGO_DIRECT)
+        plan = L.Plan(cd)
+        desc = plan.description
+        plan.close()
+        assert desc.startswith("direct_rt"), desc
+        announce("direct_rt", mode, border, orient, a.shape, k.shape)
+        got = run_gpu(cd, a, k)
+        ref, mag = oracle_ref(od, a, k)
+        assert_parity(got, ref, mag, what="direct_rt %s b%d o%d %dx%d" % (mode, border, orient, fh, fw))
+
+
 def test_c1_project_frame_block_reduce_sum():
     """x 32x32x3 uniform[-1,1), 8 filters 5x5x3, BorderZero: projectFrame + blockReduceSum(3) -> 32x32x8."""
     announce("c1")
@@ -424,8 +457,9 @@ def test_cog_merging_flags_gpu():
     finally:
         cogconv.Cog.projectFrameMerging, cogconv.Cog.fastProjectFrame = saved
     assert results[0][1] == ["igemm_f32"]
-    assert results[1][1] == ["direct"]          # unfused: ProjectFrame then blockReduceSum
-    assert results[2][1] == ["direct"]          # fused, one output plane per block
+    # The direct family (LDS-tiled or register-tiled) for the other two.
+    assert [k.startswith("direct") for k in results[1][1]] == [True]   # unfused: ProjectFrame then blockReduceSum
+    assert [k.startswith("direct") for k in results[2][1]] == [True]   # fused, one output plane per block
     for r, _ in results[1:]:
         assert np.abs(r - results[0][0]).max() <= 1e-5 * np.abs(results[0][0]).max()
 
