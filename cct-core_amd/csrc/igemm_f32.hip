@@ -246,14 +246,30 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
                 }
     }
     __syncthreads();
-    const int col = tid & 31;
-    const int ocol = tc * 32 + col;
-    for (int item = tid >> 5; item < MT * 8; item += 16) {
-        const int mm = item >> 3, row = item & 7;
-        const int orow = tr * 8 + row;
-        if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
-            O[((size_t)(b * Mout + m0 + mm) * ph.OHf + ph.oRow0 + ph.oStride * orow) * ph.OWf + ph.oCol0 +
-              ph.oStride * ocol] = sO[mm * OP + row * 32 + col];
+    // 16-byte stores (8 lanes per 32-column row) for dense outputs whose rows
+    // start 16-byte aligned; then a 4-column group is wholly inside or past
+    // the row.
+    if (ph.oStride == 1 && ((ph.OWf | ph.oCol0) & 3) == 0 && (reinterpret_cast<uintptr_t>(O) & 15) == 0) {
+        const int c4 = 4 * (tid & 7);
+        const int ocol = tc * 32 + c4;
+#pragma unroll 4
+        for (int item = tid >> 3; item < MT * 8; item += 64) {
+            const int mm = item >> 3, row = item & 7;
+            const int orow = tr * 8 + row;
+            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
+                *reinterpret_cast<float4*>(O + ((size_t)(b * Mout + m0 + mm) * ph.OHf + ph.oRow0 + orow) * ph.OWf +
+                                           ph.oCol0 + ocol) = *reinterpret_cast<const float4*>(sO + mm * OP + row * 32 + c4);
+        }
+    } else {
+        const int col = tid & 31;
+        const int ocol = tc * 32 + col;
+        for (int item = tid >> 5; item < MT * 8; item += 16) {
+            const int mm = item >> 3, row = item & 7;
+            const int orow = tr * 8 + row;
+            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
+                O[((size_t)(b * Mout + m0 + mm) * ph.OHf + ph.oRow0 + ph.oStride * orow) * ph.OWf + ph.oCol0 +
+                  ph.oStride * ocol] = sO[mm * OP + row * 32 + col];
+        }
     }
 }
 

@@ -179,6 +179,62 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
         boff[pb] = ((row * g.dnR) * TCp + col * g.dnC) * LP_RCP + 8 * g4;
     }
 
+    // Input tile: items (8-plane group, row); each lane loads the eight planes
+    // of its column, rounds them and writes one 16-byte LDS vector.  The loads
+    // of IU items are unconditional and all in flight before the first is
+    // used; lanes beyond the tile width store into a scratch slot, so no
+    // lane-dependent branch separates the loads.  Masked samples (rows or
+    // columns outside the field for Zero/Full/Valid borders) read through an
+    // out-of-range lane offset, which the buffer unit answers with 0: one
+    // select per item instead of one per sample.  The plane goes into the lane
+    // offset and the row into the uniform offset, so planes at or past R (=
+    // the image's planes) fall outside the image's buffer and read 0 too; the
+    // plan keeps every such offset below 2^32.  A wave owns TRp items of a
+    // chunk (7 dn + F); IU covers all of them when dn = 1 and F <= 3, so the
+    // chunk's staging costs one memory round trip.
+    const int nitems = 4 * TRp;
+    constexpr int IU = F + 7 < 10 ? F + 7 : 10;
+    const int planeBytes = 4 * HW;
+    const bool inTile = lane < TCp;
+    auto issue = [&](int r0, int base, float (&v)[IU][8]) {
+        int grp = base / TRp;  // wave-uniform; TRp >= 8, so one wrap per step of 4
+        int ty = base - grp * TRp;
+#pragma unroll
+        for (int u = 0; u < IU; u++) {
+            const int tyc = min(ty, 63);
+            const bool rowOn = base + 4 * u < nitems && ((yMask >> tyc) & 1);
+            const int rowByte = 4 * __builtin_amdgcn_readlane(yOffLane, tyc);
+            int vo = rowOn ? colByte + planeBytes * (r0 + 8 * grp) : LP_OOB;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                v[u][e] = buffer_load_f32(xr, vo, rowByte);
+                vo += planeBytes;
+            }
+            ty += 4;
+            if (ty >= TRp) { ty -= TRp; grp++; }
+        }
+    };
+    auto commit = [&](int base, const float (&v)[IU][8]) {
+        int grp = base / TRp;
+        int ty = base - grp * TRp;
+#pragma unroll
+        for (int u = 0; u < IU; u++) {
+            if (base + 4 * u < nitems) {  // wave-uniform
+                V hv, lv;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const T h = (T)v[u][e];
+                    hv[e] = h;
+                    if (SPLIT) lv[e] = (T)(v[u][e] - (float)h);
+                }
+                const size_t o = (size_t)(ty * TCp + lane) * LP_RCP + 8 * grp;
+                *reinterpret_cast<V*>(sB + (inTile ? o : sJunk + 8 * lane)) = hv;
+                if (SPLIT) *reinterpret_cast<V*>(sB + (inTile ? (size_t)npix * LP_RCP + o : sJunk + 512 + 8 * lane)) = lv;
+            }
+            ty += 4;
+            if (ty >= TRp) { ty -= TRp; grp++; }
+        }
+    };
     for (int r0 = 0; r0 < m.R; r0 += LP_RC) {
         __syncthreads();
         V wv[WPRE ? WPT : 1];
@@ -187,59 +243,11 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
             for (int i = 0; i < (WPRE ? WPT : 1); i++)
                 wv[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, wOff[i], 2 * r0, 0));
         }
-        // Input tile: items (8-plane group, row); each lane loads the eight
-        // planes of its column, rounds them and writes one 16-byte LDS vector.
-        // The loads of IU items are unconditional from clamped, in-bounds
-        // addresses and all in flight before the first is used; masks are
-        // applied afterwards, and lanes beyond the tile width store into a
-        // scratch slot, so no lane-dependent branch separates the loads.
-        // Masked samples (rows or columns outside the field for Zero/Full/Valid
-        // borders) read through an out-of-range lane offset, which the buffer
-        // unit answers with 0: one select per item instead of one per sample.
-        const int nitems = 4 * TRp;
-        constexpr int IU = 4;
         for (int base = wave; base < nitems; base += 4 * IU) {
             float v[IU][8];
-            int grpU[IU], tyU[IU];
-            int grp = base / TRp;  // wave-uniform; TRp >= 8, so one wrap per step of 4
-            int ty = base - grp * TRp;
-#pragma unroll
-            for (int u = 0; u < IU; u++) {
-                const int item = base + 4 * u;
-                const int tyc = min(ty, 63);
-                const bool rowOn = item < nitems && ((yMask >> tyc) & 1);
-                grpU[u] = grp;
-                tyU[u] = ty;
-                const int rowOff = __builtin_amdgcn_readlane(yOffLane, tyc);
-                const int vo = rowOn ? colByte : LP_OOB;
-#pragma unroll
-                for (int e = 0; e < 8; e++)
-                    v[u][e] = buffer_load_f32(xr, vo, 4 * (min(r0 + 8 * grp + e, m.R - 1) * HW + rowOff));
-                ty += 4;
-                if (ty >= TRp) { ty -= TRp; grp++; }
-            }
+            issue(r0, base, v);
             __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
-#pragma unroll
-            for (int u = 0; u < IU; u++) {
-                if (base + 4 * u < nitems) {  // wave-uniform
-                    if (r0 + 8 * grpU[u] + 8 > m.R) {  // wave-uniform: planes past R in this group
-#pragma unroll
-                        for (int e = 0; e < 8; e++)
-                            if (r0 + 8 * grpU[u] + e >= m.R) v[u][e] = 0.f;
-                    }
-                    V hv, lv;
-#pragma unroll
-                    for (int e = 0; e < 8; e++) {
-                        const T h = (T)v[u][e];
-                        hv[e] = h;
-                        if (SPLIT) lv[e] = (T)(v[u][e] - (float)h);
-                    }
-                    const bool inTile = lane < TCp;
-                    const size_t o = (size_t)(tyU[u] * TCp + lane) * LP_RCP + 8 * grpU[u];
-                    *reinterpret_cast<V*>(sB + (inTile ? o : sJunk + 8 * lane)) = hv;
-                    if (SPLIT) *reinterpret_cast<V*>(sB + (inTile ? (size_t)npix * LP_RCP + o : sJunk + 512 + 8 * lane)) = lv;
-                }
-            }
+            commit(base, v);
         }
         // Filter slice from the pre-packed bank: 16-byte units (term, t, m, q).
         if (WPRE) {
@@ -326,13 +334,28 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
             for (int v = 0; v < 4; v++)
                 sO[(mb * 16 + 4 * g4 + v) * OP + (2 * wave + (pb >> 1)) * 32 + (pb & 1) * 16 + l16] = acc[mb][pb][v];
     __syncthreads();
-    const int col = tid & 31;
-    const int ocol = tc * 32 + col;
-    for (int item = tid >> 5; item < MT * 8; item += 8) {
-        const int mm = item >> 3, row = item & 7;
-        const int orow = tr * 8 + row;
-        if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
-            O[((size_t)(b * Mout + m0 + mm) * g.OH + orow) * g.OW + ocol] = sO[mm * OP + row * 32 + col];
+    // 16-byte stores (8 lanes per 32-column row) where every output row starts
+    // 16-byte aligned; then a 4-column group is wholly inside or past the row.
+    if ((g.OW & 3) == 0 && (reinterpret_cast<uintptr_t>(O) & 15) == 0) {
+        const int c4 = 4 * (tid & 7);
+        const int ocol = tc * 32 + c4;
+#pragma unroll 4
+        for (int item = tid >> 3; item < MT * 8; item += 32) {
+            const int mm = item >> 3, row = item & 7;
+            const int orow = tr * 8 + row;
+            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
+                *reinterpret_cast<float4*>(O + ((size_t)(b * Mout + m0 + mm) * g.OH + orow) * g.OW + ocol) =
+                    *reinterpret_cast<const float4*>(sO + mm * OP + row * 32 + c4);
+        }
+    } else {
+        const int col = tid & 31;
+        const int ocol = tc * 32 + col;
+        for (int item = tid >> 5; item < MT * 8; item += 8) {
+            const int mm = item >> 3, row = item & 7;
+            const int orow = tr * 8 + row;
+            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
+                O[((size_t)(b * Mout + m0 + mm) * g.OH + orow) * g.OW + ocol] = sO[mm * OP + row * 32 + col];
+        }
     }
 }
 

@@ -412,6 +412,9 @@ bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ig
     if (7 * g.dnR + g.FH > 64) return false;   // one lane per staged tile row in the row table
     if ((long)d->in0.planes * g.H * g.W >= (1L << 31)) return false;
     if ((long)(d->in0.planes / d->batch) * g.H * g.W * 4 >= (1L << 31)) return false;  // buffer byte offsets
+    // Staging offsets reach 8 planes past the image's last plane on top of the
+    // 2^31 out-of-range marker.
+    if ((long)(m.N + 8) * g.H * g.W * 4 >= (1L << 31)) return false;
     IgemmCfg c;
     std::memset(&c, 0, sizeof c);
     c.Mout = m.T / m.N;
