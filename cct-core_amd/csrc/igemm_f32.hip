@@ -260,6 +260,17 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
                 *reinterpret_cast<float4*>(O + ((size_t)(b * Mout + m0 + mm) * ph.OHf + ph.oRow0 + orow) * ph.OWf +
                                            ph.oCol0 + ocol) = *reinterpret_cast<const float4*>(sO + mm * OP + row * 32 + c4);
         }
+    } else if (ph.oStride == 1 && ((ph.OWf | ph.oCol0) & 1) == 0 && (reinterpret_cast<uintptr_t>(O) & 7) == 0) {
+        const int c2 = 2 * (tid & 15);  // 8-byte stores, 16 lanes per row
+        const int ocol = tc * 32 + c2;
+#pragma unroll 4
+        for (int item = tid >> 4; item < MT * 8; item += 32) {
+            const int mm = item >> 3, row = item & 7;
+            const int orow = tr * 8 + row;
+            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
+                *reinterpret_cast<float2*>(O + ((size_t)(b * Mout + m0 + mm) * ph.OHf + ph.oRow0 + orow) * ph.OWf +
+                                           ph.oCol0 + ocol) = *reinterpret_cast<const float2*>(sO + mm * OP + row * 32 + c2);
+        }
     } else {
         const int col = tid & 31;
         const int ocol = tc * 32 + col;
