@@ -86,11 +86,13 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         for (int q = 0; q < CBG; q++) acc[i][q] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     // Items: (image b, tap-column chunk, tap-row chunk), row chunks fastest.
-    const long items = (long)m.B * c.colChunks * c.rowChunks;
-    const long it0 = items * s / c.splits, it1 = items * (s + 1) / c.splits;
-    for (long it = it0; it < it1; it++) {
-        const int b = (int)(it / ((long)c.colChunks * c.rowChunks));
-        const int rest = (int)(it - (long)b * c.colChunks * c.rowChunks);
+    // 32-bit item arithmetic: the plan keeps items * (splits + 1) below 2^31.
+    const int perImage = c.colChunks * c.rowChunks;
+    const int items = m.B * perImage;
+    const int it0 = items * s / c.splits, it1 = items * (s + 1) / c.splits;
+    for (int it = it0; it < it1; it++) {
+        const int b = it / perImage;
+        const int rest = it - b * perImage;
         const int cc = rest / c.rowChunks;
         const int i0 = (rest - cc * c.rowChunks) * c.RR;
         const int rr = min(c.RR, g.FHe - i0);
@@ -116,12 +118,15 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         }
         __syncthreads();
         // Staging of the dy tap rows and x rows: one row per wave pass, lanes
-        // across columns (at most 128).  Per batch of U rows: the rows' offsets
-        // and masks first (wave-uniform scalar work), then all loads, from
-        // clamped in-bounds offsets and with no branch between them, then the
-        // masked LDS stores; rows past the end and lanes past the row width
-        // store into a scratch slot instead of branching.
-        constexpr int U = 12;
+        // across columns (at most 128).  Per batch of UD dy rows and UX x rows
+        // (rows w, w+4, ... of each set): the rows' offsets and masks
+        // (wave-uniform scalar work), then all loads of both sets, from
+        // in-bounds offsets or the out-of-range offset, with no branch between
+        // them, then the LDS stores; rows past the end and lanes past the row
+        // width store into a scratch slot instead of branching.  One batch
+        // covers a block's rows for the common shapes, so staging costs one
+        // memory round trip.
+        constexpr int UD = 16 * MB, UX = 24;  // UD: a wave's dy rows when RR = 4
         constexpr int MT = 16 * MB;    // dy planes the MFMAs read (K padded with zeros)
         const int dRows = MT * rr;     // (il, k); the pitch padding beyond MT is never read
         const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * FHW, (uint32_t)K * FHW * 4u);
@@ -141,31 +146,25 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
             tiMask = __ballot(on);
         }
         const int vd0 = dIn0 ? 4 * dCol0 : STAGE_OOB, vd1 = dIn1 ? 4 * dCol1 : STAGE_OOB;
-        for (int base = wave; base < dRows; base += 4 * U) {
+        const int xRowsAll = nCnt * c.xRows;  // (plane, row)
+        const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + ((size_t)b * N + n_lo) * HWx, (uint32_t)nCnt * HWx * 4u);
+        const int vx0 = xIn0 ? 4 * xCol0 : STAGE_OOB, vx1 = xIn1 ? 4 * xCol1 : STAGE_OOB;
+        for (int bd = wave, bx = wave; bd < dRows || bx < xRowsAll; bd += 4 * UD, bx += 4 * UX) {
             auto drow = [&](int u, int& soff, bool& on) {
-                const int row = base + 4 * u;
+                const int row = bd + 4 * u;
                 const int il = min(row / MT, 63);
                 const int k = row - (row / MT) * MT;
                 soff = 4 * (min(k, K - 1) * FHW + __builtin_amdgcn_readlane(tiOffLane, il));
                 on = row < dRows && k < K && ((tiMask >> il) & 1);
             };
             auto ddst = [&](int u) {
-                const int row = base + 4 * u;
+                const int row = bd + 4 * u;
                 const int il = row / MT;
                 return row < dRows ? il * c.DWp * MTP + (row - il * MT) : -1;
             };
-            if (dTwo)
-                stage_rows_fn<true, U>(dyr, drow, ddst, sD, sJunk, lane, c.DWp, MTP, vd0, vd1);
-            else
-                stage_rows_fn<false, U>(dyr, drow, ddst, sD, sJunk, lane, c.DWp, MTP, vd0, vd1);
-        }
-        const int xRowsAll = nCnt * c.xRows;  // (plane, row)
-        const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + ((size_t)b * N + n_lo) * HWx, (uint32_t)nCnt * HWx * 4u);
-        const int vx0 = xIn0 ? 4 * xCol0 : STAGE_OOB, vx1 = xIn1 ? 4 * xCol1 : STAGE_OOB;
-        for (int base = wave; base < xRowsAll; base += 4 * U) {
-            // (plane, row) of the batch's rows, advanced incrementally in the
+            // (plane, row) of the batch's x rows, advanced incrementally in the
             // load pass and again in the store pass (scalar work).
-            const int nl0 = base / c.xRows, lr0 = base - nl0 * c.xRows;
+            const int nl0 = bx / c.xRows, lr0 = bx - nl0 * c.xRows;
             int nlq = nl0, lrq = lr0;
             auto xrow = [&](int u, int& soff, bool& on) {
                 if (u) {
@@ -173,7 +172,7 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
                     while (lrq >= c.xRows) { lrq -= c.xRows; nlq++; }
                 }
                 const int gy = i0 + lrq;
-                on = base + 4 * u < xRowsAll && gy < g.H;
+                on = bx + 4 * u < xRowsAll && gy < g.H;
                 soff = 4 * (min(nlq, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W);
             };
             int nls = nl0, lrs = lr0;
@@ -182,12 +181,18 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
                     lrs += 4;
                     while (lrs >= c.xRows) { lrs -= c.xRows; nls++; }
                 }
-                return base + 4 * u < xRowsAll ? nls * xPlane + lrs * c.xColsP : -1;
+                return bx + 4 * u < xRowsAll ? nls * xPlane + lrs * c.xColsP : -1;
             };
-            if (xTwo)
-                stage_rows_fn<true, U>(xr, xrow, xdst, sX, sJunk, lane, c.xCols, 1, vx0, vx1);
-            else
-                stage_rows_fn<false, U>(xr, xrow, xdst, sX, sJunk, lane, c.xCols, 1, vx0, vx1);
+            float dv0[UD], dv1[UD], xv0[UX], xv1[UX];
+            if (dTwo) load_rows_fn<true, UD>(dyr, drow, vd0, vd1, dv0, dv1);
+            else load_rows_fn<false, UD>(dyr, drow, vd0, vd1, dv0, dv1);
+            if (xTwo) load_rows_fn<true, UX>(xr, xrow, vx0, vx1, xv0, xv1);
+            else load_rows_fn<false, UX>(xr, xrow, vx0, vx1, xv0, xv1);
+            __builtin_amdgcn_sched_barrier(0);  // every load of the batch ahead of the first store
+            if (dTwo) store_rows_fn<true, UD>(ddst, sD, sJunk, lane, c.DWp, MTP, dv0, dv1);
+            else store_rows_fn<false, UD>(ddst, sD, sJunk, lane, c.DWp, MTP, dv0, dv1);
+            if (xTwo) store_rows_fn<true, UX>(xdst, sX, sJunk, lane, c.xCols, 1, xv0, xv1);
+            else store_rows_fn<false, UX>(xdst, sX, sJunk, lane, c.xCols, 1, xv0, xv1);
         }
         __syncthreads();
         // Wave w reduces tap rows w, w+4, ... of the staged chunk for every
@@ -210,8 +215,10 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
             }
         }
     }
-    // The four waves' sums meet in LDS in a fixed order, (w0 + w2) + (w1 + w3),
-    // then wave 0 writes the block's partial dW, ws[s][k][j].
+    // The four waves' sums meet in LDS in a fixed order, (w0 + w2) + (w1 + w3):
+    // waves 2 and 3 park their sums, waves 0 and 1 add them and park the
+    // results, then every wave writes a quarter of the block's partial dW,
+    // ws[s][k][j] (wave w the accumulator element v = w).
     // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
     constexpr int E = MB * CBG * 4;  // accumulator floats per lane
     __syncthreads();
@@ -232,35 +239,21 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
 #pragma unroll
             for (int q = 0; q < CBG; q++)
 #pragma unroll
-                for (int v = 0; v < 4; v++)
-                    acc[mb][q][v] += red[(size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane];
-    }
-    __syncthreads();
-    if (wave == 1) {
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-            for (int q = 0; q < CBG; q++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) red[(size_t)((mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
-    }
-    __syncthreads();
-    if (wave == 0) {
-        float* out = ws + (size_t)s * K * Ncols;
-#pragma unroll
-        for (int q = 0; q < CBG; q++) {
-            const int j = colBase + q * 16 + l16;
-            if (j >= Ncols) continue;
-#pragma unroll
-            for (int mb = 0; mb < MB; mb++)
-#pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int k = mb * 16 + 4 * g4 + v;
-                    if (k < K)
-                        out[(size_t)k * Ncols + j] =
-                            acc[mb][q][v] + red[(size_t)((mb * CBG + q) * 4 + v) * 64 + lane];
+                    float* r = red + (size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane;
+                    *r = acc[mb][q][v] + *r;
                 }
-        }
+    }
+    __syncthreads();
+    float* out = ws + (size_t)s * K * Ncols;
+#pragma unroll
+    for (int t = 0; t < MB * CBG; t++) {
+        const int mb = t / CBG, q = t - mb * CBG;
+        const int e = 4 * t + wave;
+        const float val = red[(size_t)e * 64 + lane] + red[(size_t)(E + e) * 64 + lane];
+        const int j = colBase + q * 16 + l16;
+        const int k = mb * 16 + 4 * g4 + wave;
+        if (j < Ncols && k < K) out[(size_t)k * Ncols + j] = val;
     }
 }
 

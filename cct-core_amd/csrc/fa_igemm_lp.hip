@@ -102,11 +102,13 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
     // rows past the field's last row are never read by a valid output.
     const bool dMask = K != MT || g.upFr > 1;
 
-    const long items = (long)m.B * c.colChunks * c.rowChunks;
-    const long it0 = items * s / c.splits, it1 = items * (s + 1) / c.splits;
-    for (long it = it0; it < it1; it++) {
-        const int b = (int)(it / ((long)c.colChunks * c.rowChunks));
-        const int rest = (int)(it - (long)b * c.colChunks * c.rowChunks);
+    // 32-bit item arithmetic: the plan keeps items * (splits + 1) below 2^31.
+    const int perImage = c.colChunks * c.rowChunks;
+    const int items = m.B * perImage;
+    const int it0 = items * s / c.splits, it1 = items * (s + 1) / c.splits;
+    for (int it = it0; it < it1; it++) {
+        const int b = it / perImage;
+        const int rest = it - b * perImage;
         const int cc = rest / c.rowChunks;
         const int i0 = (rest - cc * c.rowChunks) * c.RR;
         const int rr = min(c.RR, g.FHe - i0);
@@ -257,8 +259,10 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
             }
         }
     }
-    // The four waves' sums meet in LDS in a fixed order, (w0 + w2) + (w1 + w3),
-    // then wave 0 writes the block's partial dW, ws[s][k][j].
+    // The four waves' sums meet in LDS in a fixed order, (w0 + w2) + (w1 + w3):
+    // waves 2 and 3 park their sums, waves 0 and 1 add them and park the
+    // results, then every wave writes a quarter of the block's partial dW,
+    // ws[s][k][j] (wave w the accumulator element v = w).
     // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
     constexpr int E = MB * CBG * 4;
     __syncthreads();
@@ -279,35 +283,21 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
 #pragma unroll
             for (int q = 0; q < CBG; q++)
 #pragma unroll
-                for (int v = 0; v < 4; v++)
-                    acc[mb][q][v] += red[(size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane];
-    }
-    __syncthreads();
-    if (wave == 1) {
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-            for (int q = 0; q < CBG; q++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) red[(size_t)((mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
-    }
-    __syncthreads();
-    if (wave == 0) {
-        float* out = ws + (size_t)s * K * Ncols;
-#pragma unroll
-        for (int q = 0; q < CBG; q++) {
-            const int j = colBase + q * 16 + l16;
-            if (j >= Ncols) continue;
-#pragma unroll
-            for (int mb = 0; mb < MB; mb++)
-#pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    const int k = mb * 16 + 4 * g4 + v;
-                    if (k < K)
-                        out[(size_t)k * Ncols + j] =
-                            acc[mb][q][v] + red[(size_t)((mb * CBG + q) * 4 + v) * 64 + lane];
+                    float* r = red + (size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane;
+                    *r = acc[mb][q][v] + *r;
                 }
-        }
+    }
+    __syncthreads();
+    float* out = ws + (size_t)s * K * Ncols;
+#pragma unroll
+    for (int t = 0; t < MB * CBG; t++) {
+        const int mb = t / CBG, q = t - mb * CBG;
+        const int e = 4 * t + wave;
+        const float val = red[(size_t)e * 64 + lane] + red[(size_t)(E + e) * 64 + lane];
+        const int j = colBase + q * 16 + l16;
+        const int k = mb * 16 + 4 * g4 + wave;
+        if (j < Ncols && k < K) out[(size_t)k * Ncols + j] = val;
     }
 }
 

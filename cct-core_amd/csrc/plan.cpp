@@ -281,7 +281,7 @@ IgemmCfg choose_igemm_frame(const Geom& g, const PlaneMap& m, int mbOnly = 0, in
 // cbw (0 = by size) fixes the column-group width; partials (0 = by size) is
 // the target number of blocks, each of which writes one partial result.
 bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out, int cbw = 0,
-                     long partials = 0) {
+                     long partials = 0, int cw = 0) {
     if (d->mode != COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM || d->border != COGCONV_BORDER_VALID || d->in0.dims != 2)
         return false;
     FaCfg c;
@@ -297,7 +297,8 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     c.nCntMax = std::min(m.N, (16 * c.CBW + P - 1) / P + 1);
     // Tap columns in chunks whose staged x rows fit two 64-lane passes.
     if ((g.OW - 1) * g.dnC > 64) return false;
-    c.colChunks = 1;
+    // cw > 0 asks for narrower chunks (more items, so more blocks).
+    c.colChunks = cw > 0 ? (g.FWe + cw - 1) / cw : 1;
     for (;;) {
         c.CW = (g.FWe + c.colChunks - 1) / c.colChunks;
         c.DWp = (c.CW + 3) / 4 * 4;
@@ -343,6 +344,7 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     if (partials <= 0) partials = items * c.colGroups >= 8192 ? 512 : 256;  // blocks: 1-2 per CU
     long want = (partials + c.colGroups - 1) / c.colGroups;
     c.splits = (int)std::max(1L, std::min(items, want));
+    if (items * (c.splits + 1) >= (1L << 31)) return false;  // the kernel's 32-bit item arithmetic
     *out = c;
     return true;
 }
@@ -397,6 +399,7 @@ bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m,
     if (partials <= 0) partials = items * c.colGroups >= 8192 ? 512 : 256;
     long want = (partials + c.colGroups - 1) / c.colGroups;
     c.splits = (int)std::max(1L, std::min(items, want));
+    if (items * (c.splits + 1) >= (1L << 31)) return false;  // the kernel's 32-bit item arithmetic
     *out = c;
     return true;
 }
@@ -621,11 +624,13 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
             }
     }
     if (fa_ok) {
-        for (int cbw : {3, 6, 9})
-            for (int parts : {128, 256, 512}) {
-                FaCfg c;
-                if (choose_fa_igemm(desc, g, m, &c, cbw, parts)) add(KK_FA_IGEMM, nullptr, nullptr, nullptr, &c);
-            }
+        for (int cw : {0, 32, 16})
+            for (int cbw : {3, 6, 9})
+                for (int parts : {128, 256, 512, 1024}) {
+                    FaCfg c;
+                    if (choose_fa_igemm(desc, g, m, &c, cbw, parts, cw))
+                        add(KK_FA_IGEMM, nullptr, nullptr, nullptr, &c);
+                }
     }
     if (small_legal && (long)m.R * g.FHe * g.FWe >= 64) {
         for (long blocks : {2048L, 4096L, 8192L}) {

@@ -84,4 +84,38 @@ __device__ __forceinline__ void stage_rows_fn(__amdgpu_buffer_rsrc_t src, RowFn 
     }
 }
 
+// The two halves of stage_rows_fn, for callers that put the loads of several
+// row sets in flight before the first store: load_rows_fn issues the loads of
+// IU rows (row(u) as above) into v0 / v1, store_rows_fn stores them (dst(u)
+// as above).  A __builtin_amdgcn_sched_barrier(0) between the last load and
+// the first store keeps the loads together.
+template <bool TWO, int IU, class RowFn>
+__device__ __forceinline__ void load_rows_fn(__amdgpu_buffer_rsrc_t src, RowFn row, int vcol0, int vcol1,
+                                             float (&v0)[IU], float (&v1)[IU]) {
+#pragma unroll
+    for (int u = 0; u < IU; u++) {
+        int soff;
+        bool on;
+        row(u, soff, on);
+        v0[u] = buffer_load_f32(src, on ? vcol0 : STAGE_OOB, soff);
+        if (TWO) v1[u] = buffer_load_f32(src, on ? vcol1 : STAGE_OOB, soff);
+    }
+}
+
+template <bool TWO, int IU, class DstFn>
+__device__ __forceinline__ void store_rows_fn(DstFn dst, float* lds, float* junk, int lane, int width, int pitch,
+                                              const float (&v0)[IU], const float (&v1)[IU]) {
+#pragma unroll
+    for (int u = 0; u < IU; u++) {
+        const int d = dst(u);
+        const bool live = d >= 0;  // wave-uniform
+        float* d0 = (live && lane < width) ? lds + d + lane * pitch : junk + lane;
+        *d0 = v0[u];
+        if (TWO) {
+            float* d1 = (live && lane + 64 < width) ? lds + d + (lane + 64) * pitch : junk + 64 + lane;
+            *d1 = v1[u];
+        }
+    }
+}
+
 }  // namespace cogconv
