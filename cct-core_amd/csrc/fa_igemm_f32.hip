@@ -66,18 +66,26 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
     const int nCnt = n_hi - n_lo + 1;
     const int xPlane = c.xPlaneP;
 
+    // Staged-x offset of each of the group's CBG*16 output columns (n, R, C):
+    // one column per thread into LDS (the staging area is free until the first
+    // item's barrier), then each lane reads the CBG columns it feeds.
+    {
+        int* colTab = reinterpret_cast<int*>(smem);
+        if (tid < CBG * 16) {
+            const int j = colBase + tid;
+            int off = 0;
+            if (j < Ncols) {
+                const int n = fast_div(j, P, 1.0f / P), rem = j - n * P;
+                const int R = fast_div(rem, g.OW, 1.0f / g.OW), C = rem - R * g.OW;
+                off = (n - n_lo) * xPlane + R * g.dnR * c.xColsP + C * g.dnC;
+            }
+            colTab[tid] = off;
+        }
+        __syncthreads();
+    }
     int xoff[CBG];
 #pragma unroll
-    for (int q = 0; q < CBG; q++) {
-        const int j = colBase + q * 16 + l16;
-        int off = 0;
-        if (j < Ncols) {
-            const int n = fast_div(j, P, 1.0f / P), rem = j - n * P;
-            const int R = fast_div(rem, g.OW, 1.0f / g.OW), C = rem - R * g.OW;
-            off = (n - n_lo) * xPlane + R * g.dnR * c.xColsP + C * g.dnC;
-        }
-        xoff[q] = off + g4;
-    }
+    for (int q = 0; q < CBG; q++) xoff[q] = reinterpret_cast<const int*>(smem)[q * 16 + l16] + g4;
 
     floatx4 acc[MB][CBG];
 #pragma unroll
@@ -119,8 +127,8 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         __syncthreads();
         // Staging of the dy tap rows and x rows: one row per wave pass, lanes
         // across columns (at most 128).  Per batch of UD dy rows and UX x rows
-        // (rows w, w+4, ... of each set): the rows' offsets and masks
-        // (wave-uniform scalar work), then all loads of both sets, from
+        // (rows w, w+4, ... of each set): the rows' offsets and masks (per-lane
+        // tables), then all loads of both sets, from
         // in-bounds offsets or the out-of-range offset, with no branch between
         // them, then the LDS stores; rows past the end and lanes past the row
         // width store into a scratch slot instead of branching.  One batch
@@ -130,59 +138,51 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         constexpr int MT = 16 * MB;    // dy planes the MFMAs read (K padded with zeros)
         const int dRows = MT * rr;     // (il, k); the pitch padding beyond MT is never read
         const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * FHW, (uint32_t)K * FHW * 4u);
-        // Per-lane table (lane = staged tap row il < RR <= 8) of the dy row
-        // offsets, flip and up-sampling lattice applied, and the rows' mask.
-        int tiOffLane;
-        uint64_t tiMask;
-        {
-            int ti = i0 + lane;
-            if (g.flip) ti = g.FHe - 1 - ti;
-            bool on = lane < rr;
-            if (g.upFr > 1) {
-                on = on && (ti % g.upFr) == 0;
-                ti /= g.upFr;
-            }
-            tiOffLane = min(max(ti, 0), g.FH - 1) * g.FW;
-            tiMask = __ballot(on);
-        }
         const int vd0 = dIn0 ? 4 * dCol0 : STAGE_OOB, vd1 = dIn1 ? 4 * dCol1 : STAGE_OOB;
         const int xRowsAll = nCnt * c.xRows;  // (plane, row)
         const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + ((size_t)b * N + n_lo) * HWx, (uint32_t)nCnt * HWx * 4u);
         const int vx0 = xIn0 ? 4 * xCol0 : STAGE_OOB, vx1 = xIn1 ? 4 * xCol1 : STAGE_OOB;
         for (int bd = wave, bx = wave; bd < dRows || bx < xRowsAll; bd += 4 * UD, bx += 4 * UX) {
+            // Per-lane row tables of the batch (lane u = the wave's u-th dy row
+            // and u-th x row): byte offsets (flip, up-sampling lattice and
+            // clamps applied), LDS destinations (-1: not live) and in-field
+            // masks, computed in parallel across lanes and read back per row
+            // with readlane / a ballot bit.
+            int dSoff, dDst, xSoff, xDst;
+            uint64_t dMask, xMask;
+            {
+                const int row = bd + 4 * lane;
+                const int il = row / MT, k = row - il * MT;
+                int ti = i0 + il;
+                if (g.flip) ti = g.FHe - 1 - ti;
+                bool on = lane < UD && row < dRows && k < K && il < rr;
+                if (g.upFr > 1) {
+                    on = on && (ti % g.upFr) == 0;
+                    ti /= g.upFr;
+                }
+                dSoff = 4 * (min(k, K - 1) * FHW + min(max(ti, 0), g.FH - 1) * g.FW);
+                dDst = (lane < UD && row < dRows) ? il * c.DWp * MTP + k : -1;
+                dMask = __ballot(on);
+            }
+            {
+                const int row = bx + 4 * lane;
+                const int nl = row / c.xRows, lr = row - nl * c.xRows;
+                const int gy = i0 + lr;
+                const bool live = lane < UX && row < xRowsAll;
+                xSoff = 4 * (min(nl, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W);
+                xDst = live ? nl * xPlane + lr * c.xColsP : -1;
+                xMask = __ballot(live && gy < g.H);
+            }
             auto drow = [&](int u, int& soff, bool& on) {
-                const int row = bd + 4 * u;
-                const int il = min(row / MT, 63);
-                const int k = row - (row / MT) * MT;
-                soff = 4 * (min(k, K - 1) * FHW + __builtin_amdgcn_readlane(tiOffLane, il));
-                on = row < dRows && k < K && ((tiMask >> il) & 1);
+                soff = __builtin_amdgcn_readlane(dSoff, u);
+                on = (dMask >> u) & 1;
             };
-            auto ddst = [&](int u) {
-                const int row = bd + 4 * u;
-                const int il = row / MT;
-                return row < dRows ? il * c.DWp * MTP + (row - il * MT) : -1;
-            };
-            // (plane, row) of the batch's x rows, advanced incrementally in the
-            // load pass and again in the store pass (scalar work).
-            const int nl0 = bx / c.xRows, lr0 = bx - nl0 * c.xRows;
-            int nlq = nl0, lrq = lr0;
+            auto ddst = [&](int u) { return __builtin_amdgcn_readlane(dDst, u); };
             auto xrow = [&](int u, int& soff, bool& on) {
-                if (u) {
-                    lrq += 4;
-                    while (lrq >= c.xRows) { lrq -= c.xRows; nlq++; }
-                }
-                const int gy = i0 + lrq;
-                on = bx + 4 * u < xRowsAll && gy < g.H;
-                soff = 4 * (min(nlq, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W);
+                soff = __builtin_amdgcn_readlane(xSoff, u);
+                on = (xMask >> u) & 1;
             };
-            int nls = nl0, lrs = lr0;
-            auto xdst = [&](int u) {
-                if (u) {
-                    lrs += 4;
-                    while (lrs >= c.xRows) { lrs -= c.xRows; nls++; }
-                }
-                return bx + 4 * u < xRowsAll ? nls * xPlane + lrs * c.xColsP : -1;
-            };
+            auto xdst = [&](int u) { return __builtin_amdgcn_readlane(xDst, u); };
             float dv0[UD], dv1[UD], xv0[UX], xv1[UX];
             if (dTwo) load_rows_fn<true, UD>(dyr, drow, vd0, vd1, dv0, dv1);
             else load_rows_fn<false, UD>(dyr, drow, vd0, vd1, dv0, dv1);
