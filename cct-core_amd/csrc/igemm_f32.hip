@@ -87,25 +87,6 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
         colOff0 = xs[0]; colOff1 = xs[1];
         colIn0 = ins[0]; colIn1 = ins[1];
     }
-    // Per-lane table of the staged tile's row offsets (lane = tile row, at
-    // most 64 rows) with the border applied, and the rows' in-field mask: the
-    // staging loop reads them with one readlane / bit test per row.
-    int yOffLane;
-    uint64_t yMask;
-    {
-        int y = y0 + lane;
-        bool in = (unsigned)y < (unsigned)g.H;
-        if (g.border == COGCONV_BORDER_CYCLIC) {
-            y %= g.H;
-            y += y < 0 ? g.H : 0;
-            in = true;
-        } else {
-            in = in || g.border == COGCONV_BORDER_CLAMP;
-            y = min(max(y, 0), g.H - 1);
-        }
-        yOffLane = y * g.W;
-        yMask = __ballot(in);
-    }
     const int half = c.RC >> 1;
 
     floatx4 acc[MB][4];
@@ -132,31 +113,38 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
         // stores; items past the end and lanes past the tile width store into
         // a scratch slot instead of branching.
         const int nrows = c.RC * TRp;
-        constexpr int IU = 8;
+        constexpr int IU = 12;
         const int vc0 = colIn0 ? 4 * colOff0 : STAGE_OOB, vc1 = colIn1 ? 4 * colOff1 : STAGE_OOB;
         for (int base = wave; base < nrows; base += 8 * IU) {
-            // (plane, tile row) of the batch's rows, advanced incrementally in
-            // the load pass and again in the store pass (scalar work).
-            const int rc0 = base / TRp, ty0 = base - rc0 * TRp;
-            int rcq = rc0, tyq = ty0;
+            // Per-lane row table of the batch (lane u = the wave's u-th item):
+            // byte offset (border applied), LDS destination (-1: not live) and
+            // in-field mask, read back per row with readlane / a ballot bit.
+            int rSoff, rDst;
+            uint64_t rMask;
+            {
+                const int item = base + 8 * lane;
+                const int rc = item / TRp, ty = item - rc * TRp;
+                const int r = r0 + rc;
+                int y = y0 + ty;
+                bool in = (unsigned)y < (unsigned)g.H;
+                if (g.border == COGCONV_BORDER_CYCLIC) {
+                    y %= g.H;
+                    y += y < 0 ? g.H : 0;
+                    in = true;
+                } else {
+                    in = in || g.border == COGCONV_BORDER_CLAMP;
+                    y = min(max(y, 0), g.H - 1);
+                }
+                const bool live = lane < IU && item < nrows;
+                rSoff = 4 * (min(r, m.R - 1) * HW + y * g.W);
+                rDst = live ? rc * c.planeStride + ty * TCp : -1;
+                rMask = __ballot(live && in && r < m.R);
+            }
             auto xrow = [&](int u, int& soff, bool& on) {
-                if (u) {
-                    tyq += 8;
-                    if (tyq >= TRp) { tyq -= TRp; rcq++; }
-                }
-                const int r = r0 + rcq;
-                const int ty = min(tyq, 63);  // tyq < TRp <= 64 for live rows
-                on = ((yMask >> ty) & 1) && r < m.R && base + 8 * u < nrows;
-                soff = 4 * (min(r, m.R - 1) * HW + __builtin_amdgcn_readlane(yOffLane, ty));
+                soff = __builtin_amdgcn_readlane(rSoff, u);
+                on = (rMask >> u) & 1;
             };
-            int rcs = rc0, tys = ty0;
-            auto xdst = [&](int u) {
-                if (u) {
-                    tys += 8;
-                    if (tys >= TRp) { tys -= TRp; rcs++; }
-                }
-                return base + 8 * u < nrows ? rcs * c.planeStride + tys * TCp : -1;
-            };
+            auto xdst = [&](int u) { return __builtin_amdgcn_readlane(rDst, u); };
             if (twoPass)
                 stage_rows_fn<true, IU>(xr, xrow, xdst, sB, sJunk, lane, TCp, 1, vc0, vc1);
             else
