@@ -65,18 +65,25 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
     const int nCnt = n_hi - n_lo + 1;
 
     // B-operand offset of each column block: copy C of x row R*dn (+ tap row).
+    // One column per thread into LDS (free until the first item's barrier),
+    // then each lane reads the CBG columns it feeds.
+    {
+        int* colTab = reinterpret_cast<int*>(fa_lp_smem);
+        if (tid < CBG * 16) {
+            const int j = colBase + tid;
+            int off = 0;
+            if (j < Ncols) {
+                const int n = fast_div(j, P, 1.0f / P), rem = j - n * P;
+                const int R = fast_div(rem, OW, 1.0f / OW), C = rem - R * OW;
+                off = (((n - n_lo) * c.xRows + R * g.dnR) * OW + C) * DP;
+            }
+            colTab[tid] = off;
+        }
+        __syncthreads();
+    }
     int xoff[CBG];
 #pragma unroll
-    for (int q = 0; q < CBG; q++) {
-        const int j = colBase + q * 16 + l16;
-        int off = 0;
-        if (j < Ncols) {
-            const int n = fast_div(j, P, 1.0f / P), rem = j - n * P;
-            const int R = fast_div(rem, OW, 1.0f / OW), C = rem - R * OW;
-            off = (((n - n_lo) * c.xRows + R * g.dnR) * OW + C) * DP;
-        }
-        xoff[q] = off + 8 * g4;
-    }
+    for (int q = 0; q < CBG; q++) xoff[q] = reinterpret_cast<const int*>(fa_lp_smem)[q * 16 + l16] + 8 * g4;
 
     floatx4 acc[MB][CBG];
 #pragma unroll
@@ -130,39 +137,41 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
                 xv[e] = (myRow < OW && xc < g.W) ? 4 * xc : FA_OOB;
             }
         }
-        // Per-lane table (lane = staged tap row il < RR <= 8) of the dy row
-        // offsets, flip and up-sampling lattice applied, and the rows' mask.
-        int tiOffLane;
-        uint64_t tiMask;
-        {
-            int ti = i0 + lane;
-            if (g.flip) ti = g.FHe - 1 - ti;
-            bool on = lane < rr;
-            if (g.upFr > 1) {
-                on = on && (ti % g.upFr) == 0;
-                ti /= g.upFr;
-            }
-            tiOffLane = min(max(ti, 0), g.FH - 1) * g.FW;
-            tiMask = __ballot(on);
-        }
         __syncthreads();
-        constexpr int DU = 4;  // wave passes per batch: 32 loads in flight per lane
-        // dy rows (il, k), k fastest; a pass's RPP rows share il (RPP divides MT).
+        // dy rows (il, k), k fastest; a pass's RPP rows share il (RPP divides
+        // MT).  Per batch of DUD passes: a per-lane table (lane d = the wave's
+        // d-th pass) of the pass's byte offset (flip and up-sampling lattice
+        // applied), LDS row base (-1: not live), planes left (K - k0) and
+        // tap-row mask, read back per pass with readlane / a ballot bit; then
+        // every load, then the rounding stores.
+        constexpr int DUD = 8, DUX = 12;  // passes per batch: 64 / 96 samples in flight per lane
         const int dRows = MT * rr;
         const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * FHW, (uint32_t)K * FHW * 4u);
-        for (int pb = wave * RPP; pb < dRows; pb += 4 * RPP * DU) {
-            float v[DU][8];
-            int dstD[DU];
+        for (int pb = wave * RPP; pb < dRows; pb += 4 * RPP * DUD) {
+            int tSoff, tDst, tKl;
+            uint64_t tOk;
+            {
+                const int p0 = pb + 4 * RPP * lane;
+                const int il = p0 / MT, k0 = p0 - il * MT;
+                int ti = i0 + il;
+                if (g.flip) ti = g.FHe - 1 - ti;
+                bool on = lane < DUD && p0 < dRows && il < rr;
+                if (g.upFr > 1) {
+                    on = on && (ti % g.upFr) == 0;
+                    ti /= g.upFr;
+                }
+                tSoff = 4 * (min(k0, K - 1) * FHW + min(max(ti, 0), g.FH - 1) * g.FW);
+                tDst = (lane < DUD && p0 < dRows) ? (il * MT + k0) * DP : -1;
+                tKl = K - k0;
+                tOk = __ballot(on);
+            }
+            float v[DUD][8];
 #pragma unroll
-            for (int d = 0; d < DU; d++) {
-                const int p0 = pb + d * 4 * RPP;            // wave-uniform
-                const int il = min(p0 / MT, 7);
-                const int k0 = p0 - (p0 / MT) * MT;
-                dstD[d] = p0 < dRows ? (il * MT + k0 + myRow) * DP + 8 * cgrp : -1;
-                // Without the mask K == MT, so k0 + myRow < K for every lane.
-                const int soff = 4 * (min(k0, K - 1) * FHW + __builtin_amdgcn_readlane(tiOffLane, il));
+            for (int d = 0; d < DUD; d++) {
+                const int soff = __builtin_amdgcn_readlane(tSoff, d);
                 if (dMask) {  // wave-uniform, decided per block
-                    const bool rowOn = ((tiMask >> il) & 1) && k0 + myRow < K;
+                    // Without the mask K == MT, so k0 + myRow < K for every lane.
+                    const bool rowOn = ((tOk >> d) & 1) && myRow < __builtin_amdgcn_readlane(tKl, d);
 #pragma unroll
                     for (int e = 0; e < 8; e++) v[d][e] = buffer_load_f32(dyr, rowOn ? dv[e] : FA_OOB, soff);
                 } else {
@@ -172,7 +181,7 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
             }
             __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll
-            for (int d = 0; d < DU; d++) {
+            for (int d = 0; d < DUD; d++) {
                 V hv, lv;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
@@ -180,31 +189,41 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
                     hv[e] = h;
                     if (SPLIT) lv[e] = (T)(v[d][e] - (float)h);
                 }
-                const bool live = dstD[d] >= 0;
-                *reinterpret_cast<V*>(live ? sD + dstD[d] : sJunk + 8 * lane) = hv;
-                if (SPLIT) *reinterpret_cast<V*>(live ? sD + dTerm + dstD[d] : sJunk + 512 + 8 * lane) = lv;
+                const int uni = __builtin_amdgcn_readlane(tDst, d);
+                const bool live = uni >= 0;
+                const int dst = uni + myRow * DP + 8 * cgrp;
+                *reinterpret_cast<V*>(live ? sD + dst : sJunk + 8 * lane) = hv;
+                if (SPLIT) *reinterpret_cast<V*>(live ? sD + dTerm + dst : sJunk + 512 + 8 * lane) = lv;
             }
         }
         // x rows (nl, r): one staged row per pass q of QP; lane (copy C, column
         // group) reads x columns j0 + C*dn + 8*cgrp .. +7 of row i0 + r of
-        // plane n_lo + nl, copy C holding the row shifted left by C*dn.
+        // plane n_lo + nl, copy C holding the row shifted left by C*dn.  The
+        // passes of a batch come from a per-lane table as above.
         const int xPasses = nCnt * c.xRows * QP;
         const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + ((size_t)b * N + n_lo) * HWx, (uint32_t)nCnt * HWx * 4u);
-        for (int tb = wave; tb < xPasses; tb += 4 * DU) {
-            float v[DU][8];
-            int dstX[DU];
-#pragma unroll
-            for (int d = 0; d < DU; d++) {
-                const int t = tb + 4 * d;                    // wave-uniform
-                const int u = QP == 1 ? t : t / QP, q = t - u * QP;
+        for (int tb = wave; tb < xPasses; tb += 4 * DUX) {
+            int tSoff, tDst, tQ;
+            {
+                const int t = tb + 4 * lane;
+                const int u = t / QP, q = t - u * QP;
                 const int nl = u / c.xRows, r = u - nl * c.xRows;
-                const int C = q * RPP + myRow;
-                dstX[d] = (t < xPasses && C < OW) ? ((nl * c.xRows + r) * OW + C) * DP + 8 * cgrp : -1;
-                const int soff = 4 * (min(nl, nCnt - 1) * HWx + min(i0 + r, g.H - 1) * g.W);
+                tSoff = 4 * (min(nl, nCnt - 1) * HWx + min(i0 + r, g.H - 1) * g.W);
+                tDst = (lane < DUX && t < xPasses) ? (nl * c.xRows + r) * OW * DP : -1;
+                tQ = q;
+            }
+            float v[DUX][8];
+            int cU[DUX];
+#pragma unroll
+            for (int d = 0; d < DUX; d++) {
+                const int soff = __builtin_amdgcn_readlane(tSoff, d);
                 if (QP == 1) {  // wave-uniform: the lane offsets of the chunk apply as they are
+                    cU[d] = myRow;
 #pragma unroll
                     for (int e = 0; e < 8; e++) v[d][e] = buffer_load_f32(xr, xv[e], soff);
                 } else {
+                    const int C = __builtin_amdgcn_readlane(tQ, d) * RPP + myRow;
+                    cU[d] = C;
                     const int colStart = j0 + C * g.dnC + 8 * cgrp;
 #pragma unroll
                     for (int e = 0; e < 8; e++)
@@ -214,7 +233,7 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
             }
             __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll
-            for (int d = 0; d < DU; d++) {
+            for (int d = 0; d < DUX; d++) {
                 V hv, lv;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
@@ -222,9 +241,11 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
                     hv[e] = h;
                     if (SPLIT) lv[e] = (T)(v[d][e] - (float)h);
                 }
-                const bool live = dstX[d] >= 0;
-                *reinterpret_cast<V*>(live ? sX + dstX[d] : sJunk + 8 * lane) = hv;
-                if (SPLIT) *reinterpret_cast<V*>(live ? sX + xTerm + dstX[d] : sJunk + 512 + 8 * lane) = lv;
+                const int uni = __builtin_amdgcn_readlane(tDst, d);
+                const bool live = uni >= 0 && cU[d] < OW;
+                const int dst = uni + cU[d] * DP + 8 * cgrp;
+                *reinterpret_cast<V*>(live ? sX + dst : sJunk + 8 * lane) = hv;
+                if (SPLIT) *reinterpret_cast<V*>(live ? sX + xTerm + dst : sJunk + 512 + 8 * lane) = lv;
             }
         }
         __syncthreads();
