@@ -16,7 +16,9 @@
 // 32 consecutive input planes at one tap, so LDS holds both operands
 // planes-fastest ("channels last"): a lane's eight k values are one 16-byte
 // read.  A block (4 wave64) owns an 8 x 32 pixel tile and 16*MB output planes;
-// wave w owns pixel rows 2w, 2w+1.  Input planes are staged 32 at a time.
+// wave w owns pixel rows 2w, 2w+1.  Input planes are staged 32 at a time.  The
+// MFMAs take the pixels as A and the filters as B, so a lane's accumulator
+// holds four adjacent pixels of one output plane (direct 16-byte stores).
 //
 // Split-BF16: x = xh + xl with xh = bf16(x), xl = bf16(x - xh) (same for the
 // filter); x*w ~ xh*wh + xh*wl + xl*wh, each term on the MFMA, all into the
@@ -310,72 +312,48 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
                     for (int mb = 0; mb < MB; mb++)
 #pragma unroll
                         for (int pb = 0; pb < 4; pb++) {
-                            acc[mb][pb] = lp_mfma<PREC>(al[mb], bv[pb], acc[mb][pb]);
-                            acc[mb][pb] = lp_mfma<PREC>(av[mb], bl[pb], acc[mb][pb]);
+                            acc[mb][pb] = lp_mfma<PREC>(bv[pb], al[mb], acc[mb][pb]);
+                            acc[mb][pb] = lp_mfma<PREC>(bl[pb], av[mb], acc[mb][pb]);
                         }
                 }
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++)
 #pragma unroll
-                    for (int pb = 0; pb < 4; pb++) acc[mb][pb] = lp_mfma<PREC>(av[mb], bv[pb], acc[mb][pb]);
+                    for (int pb = 0; pb < 4; pb++) acc[mb][pb] = lp_mfma<PREC>(bv[pb], av[mb], acc[mb][pb]);
             }
         }
     }
-    // Epilogue: LDS output tile [MT][8][32] (plane pitch 260 floats), then
-    // coalesced row stores.  D[i][j] of a 16x16 block: lane j + 16*(i/4), element i%4.
-    __syncthreads();
-    constexpr int OP = 260;
-    float* sO = reinterpret_cast<float*>(lp_smem);
+    // Epilogue: the MFMAs run pixels x planes (A = the staged pixels, B = the
+    // filter slice), so lane (g4, l16) of block (mb, pb) holds output plane
+    // m0 + 16 mb + l16 at the four adjacent pixels 4 g4 .. 4 g4 + 3 of its
+    // 16-pixel run: one 16-byte store each where output rows are 16-byte
+    // aligned (a 4-column group is then wholly inside or past the row).
+    const bool vec = (g.OW & 3) == 0 && (reinterpret_cast<uintptr_t>(O) & 15) == 0;
 #pragma unroll
-    for (int mb = 0; mb < MB; mb++)
+    for (int mb = 0; mb < MB; mb++) {
+        const int mm = m0 + mb * 16 + l16;
 #pragma unroll
-        for (int pb = 0; pb < 4; pb++)
+        for (int pb = 0; pb < 4; pb++) {
+            const int orow = tr * 8 + 2 * wave + (pb >> 1);
+            const int ocol = tc * 32 + (pb & 1) * 16 + 4 * g4;
+            if (mm >= Mout || orow >= g.OH || ocol >= g.OW) continue;
+            float* dst = O + ((size_t)(b * Mout + mm) * g.OH + orow) * g.OW + ocol;
+            if (vec) {
+                *reinterpret_cast<float4*>(dst) =
+                    make_float4(acc[mb][pb][0], acc[mb][pb][1], acc[mb][pb][2], acc[mb][pb][3]);
+            } else {
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-                sO[(mb * 16 + 4 * g4 + v) * OP + (2 * wave + (pb >> 1)) * 32 + (pb & 1) * 16 + l16] = acc[mb][pb][v];
-    __syncthreads();
-    // 16-byte stores (8 lanes per 32-column row) where every output row starts
-    // 16-byte aligned; then a 4-column group is wholly inside or past the row.
-    if ((g.OW & 3) == 0 && (reinterpret_cast<uintptr_t>(O) & 15) == 0) {
-        const int c4 = 4 * (tid & 7);
-        const int ocol = tc * 32 + c4;
-#pragma unroll 4
-        for (int item = tid >> 3; item < MT * 8; item += 32) {
-            const int mm = item >> 3, row = item & 7;
-            const int orow = tr * 8 + row;
-            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
-                *reinterpret_cast<float4*>(O + ((size_t)(b * Mout + m0 + mm) * g.OH + orow) * g.OW + ocol) =
-                    *reinterpret_cast<const float4*>(sO + mm * OP + row * 32 + c4);
-        }
-    } else if ((g.OW & 1) == 0 && (reinterpret_cast<uintptr_t>(O) & 7) == 0) {
-        const int c2 = 2 * (tid & 15);  // 8-byte stores, 16 lanes per row
-        const int ocol = tc * 32 + c2;
-#pragma unroll 4
-        for (int item = tid >> 4; item < MT * 8; item += 16) {
-            const int mm = item >> 3, row = item & 7;
-            const int orow = tr * 8 + row;
-            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
-                *reinterpret_cast<float2*>(O + ((size_t)(b * Mout + m0 + mm) * g.OH + orow) * g.OW + ocol) =
-                    *reinterpret_cast<const float2*>(sO + mm * OP + row * 32 + c2);
-        }
-    } else {
-        const int col = tid & 31;
-        const int ocol = tc * 32 + col;
-        for (int item = tid >> 5; item < MT * 8; item += 8) {
-            const int mm = item >> 3, row = item & 7;
-            const int orow = tr * 8 + row;
-            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
-                O[((size_t)(b * Mout + m0 + mm) * g.OH + orow) * g.OW + ocol] = sO[mm * OP + row * 32 + col];
+                for (int v = 0; v < 4; v++)
+                    if (ocol + v < g.OW) dst[v] = acc[mb][pb][v];
+            }
         }
     }
 }
 
 size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision) {
     const int terms = precision == COGCONV_SPLIT_BF16 ? 2 : 1;
-    const size_t stage = (size_t)terms * ((size_t)c.tileRows * c.tileCols + (size_t)F * F * 16 * c.MB) * LP_RCP * 2 +
-                         (size_t)terms * 64 * 8 * 2;  // scratch slots of the lanes beyond the tile width
-    const size_t epi = (size_t)16 * c.MB * 260 * sizeof(float);
-    return stage > epi ? stage : epi;
+    return (size_t)terms * ((size_t)c.tileRows * c.tileCols + (size_t)F * F * 16 * c.MB) * LP_RCP * 2 +
+           (size_t)terms * 64 * 8 * 2;  // scratch slots of the lanes beyond the tile width
 }
 
 size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision) {
