@@ -15,7 +15,8 @@
 // wave w < 4 reduces over the first half of each staged plane chunk, wave
 // w >= 4 over the second half, and the two partial sums are added through LDS
 // at the end.  The four k-lanes of one MFMA (lane/16) carry four consecutive
-// input planes at the same tap.  Input planes are staged RC at a time: the
+// input planes at the same tap; the MFMA takes the pixels as A and the filters
+// as B, so a lane's accumulator holds four adjacent pixels of one output plane.  Input planes are staged RC at a time: the
 // tile [RC][rows][cols] (one row per wave pass, lanes across columns) and the
 // filter slice [RC][F*F][16*MB] (each thread copies the F*F contiguous taps of
 // one (plane, filter) pair).  Both staged plane strides are 16 mod 32 words so
@@ -198,57 +199,69 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
                     for (int mb = 0; mb < MB; mb++)
 #pragma unroll
                         for (int pb = 0; pb < 4; pb++)
-                            acc[mb][pb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mb], bv[pb], acc[mb][pb], 0, 0, 0);
+                            acc[mb][pb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[pb], av[mb], acc[mb][pb], 0, 0, 0);
                 }
             }
         }
     }
-    // Epilogue: the two plane groups' sums meet in an LDS output tile
-    // [16*MB planes][8 rows][32 cols] (plane pitch 260 words), which is then
-    // written row by row with coalesced stores.
-    // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
+    // Epilogue.  The MFMAs run pixels x planes (A = the staged pixels, B = the
+    // filter slice), so lane (g4, l16) of block (mb, pb) holds output plane
+    // m0 + 16 mb + l16 at the four adjacent pixels 4 g4 .. 4 g4 + 3 of its
+    // 16-pixel run.  Dense outputs whose rows start 16-byte aligned: plane
+    // group 1 parks its sums in LDS (one 16-byte vector per block and lane),
+    // plane group 0 adds them and stores each lane's four pixels as one
+    // 16-byte store (a 4-column group is then wholly inside or past the row).
+    // Other outputs: the sums meet in an LDS tile [16*MB planes][8 rows][32
+    // cols] (plane pitch 260 words, 16-byte writes), written row by row with
+    // 8-byte or single-float stores.
     __syncthreads();
+    const bool vec = ph.oStride == 1 && ((ph.OWf | ph.oCol0) & 3) == 0 && (reinterpret_cast<uintptr_t>(O) & 15) == 0;
+    if (vec) {  // block-uniform
+        floatx4* red4 = reinterpret_cast<floatx4*>(smem);  // [rw][MB][4][64]
+        if (pg == 1) {
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int pb = 0; pb < 4; pb++) red4[((rw * MB + mb) * 4 + pb) * 64 + lane] = acc[mb][pb];
+        }
+        __syncthreads();
+        if (pg == 0) {
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) {
+                const int mm = m0 + mb * 16 + l16;
+#pragma unroll
+                for (int pb = 0; pb < 4; pb++) {
+                    const floatx4 sum = acc[mb][pb] + red4[((rw * MB + mb) * 4 + pb) * 64 + lane];
+                    const int orow = tr * 8 + 2 * rw + (pb >> 1);
+                    const int ocol = tc * 32 + (pb & 1) * 16 + 4 * g4;
+                    if (mm < Mout && orow < g.OH && ocol < g.OW)
+                        *reinterpret_cast<float4*>(O + ((size_t)(b * Mout + mm) * ph.OHf + ph.oRow0 + orow) * ph.OWf +
+                                                   ph.oCol0 + ocol) = make_float4(sum[0], sum[1], sum[2], sum[3]);
+                }
+            }
+        }
+        return;
+    }
     constexpr int OP = 260;
     float* sO = smem;
+    auto tile = [&](int mb, int pb) {
+        return reinterpret_cast<floatx4*>(sO + (mb * 16 + l16) * OP + (2 * rw + (pb >> 1)) * 32 + (pb & 1) * 16 + 4 * g4);
+    };
     if (pg == 1) {
 #pragma unroll
         for (int mb = 0; mb < MB; mb++)
 #pragma unroll
-            for (int pb = 0; pb < 4; pb++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int mm = mb * 16 + 4 * g4 + v;
-                    sO[mm * OP + (2 * rw + (pb >> 1)) * 32 + (pb & 1) * 16 + l16] = acc[mb][pb][v];
-                }
+            for (int pb = 0; pb < 4; pb++) *tile(mb, pb) = acc[mb][pb];
     }
     __syncthreads();
     if (pg == 0) {
 #pragma unroll
         for (int mb = 0; mb < MB; mb++)
 #pragma unroll
-            for (int pb = 0; pb < 4; pb++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int mm = mb * 16 + 4 * g4 + v;
-                    sO[mm * OP + (2 * rw + (pb >> 1)) * 32 + (pb & 1) * 16 + l16] += acc[mb][pb][v];
-                }
+            for (int pb = 0; pb < 4; pb++) *tile(mb, pb) = acc[mb][pb] + *tile(mb, pb);
     }
     __syncthreads();
-    // 16-byte stores (8 lanes per 32-column row) for dense outputs whose rows
-    // start 16-byte aligned; then a 4-column group is wholly inside or past
-    // the row.
-    if (ph.oStride == 1 && ((ph.OWf | ph.oCol0) & 3) == 0 && (reinterpret_cast<uintptr_t>(O) & 15) == 0) {
-        const int c4 = 4 * (tid & 7);
-        const int ocol = tc * 32 + c4;
-#pragma unroll 4
-        for (int item = tid >> 3; item < MT * 8; item += 64) {
-            const int mm = item >> 3, row = item & 7;
-            const int orow = tr * 8 + row;
-            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
-                *reinterpret_cast<float4*>(O + ((size_t)(b * Mout + m0 + mm) * ph.OHf + ph.oRow0 + orow) * ph.OWf +
-                                           ph.oCol0 + ocol) = *reinterpret_cast<const float4*>(sO + mm * OP + row * 32 + c4);
-        }
-    } else if (ph.oStride == 1 && ((ph.OWf | ph.oCol0) & 1) == 0 && (reinterpret_cast<uintptr_t>(O) & 7) == 0) {
+    if (ph.oStride == 1 && ((ph.OWf | ph.oCol0) & 1) == 0 && (reinterpret_cast<uintptr_t>(O) & 7) == 0) {
         const int c2 = 2 * (tid & 15);  // 8-byte stores, 16 lanes per row
         const int ocol = tc * 32 + c2;
 #pragma unroll 4
@@ -294,7 +307,7 @@ bool igemm_f32_supported_rc(int FR, int FC) {
 bool igemm_f32_supported(int F) { return igemm_f32_supported_rc(F, F); }
 
 // LDS floats the output tile of k_igemm_f32's epilogue needs.
-size_t igemm_f32_exchange_floats(int MB) { return (size_t)16 * MB * 260; }
+size_t igemm_f32_exchange_floats(int MB) { return (size_t)16 * MB * 260; }  // >= the [4 waves][MB][4][64] float4 exchange
 
 hipError_t launch_igemm_f32_phase(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const Phase& ph, const float* x,
                                   const float* w, float* out, hipStream_t s) {
