@@ -27,7 +27,7 @@ namespace cogconv {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-template <int FR, int FC, int MB>
+template <int FR, int FC, int MB, bool PH>
 __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg c, Phase ph, const float* __restrict__ X,
                                                    const float* __restrict__ Wt, float* __restrict__ O) {
     constexpr int TAPS = FR * FC;
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
             // orientation; contiguous when there is one phase.
             float t[TAPS];
             float* dst = sA + (size_t)rc * aStride + mm;
-            if (ph.step == 1) {  // wave-uniform: one phase, the taps are contiguous
+            if (!PH || ph.step == 1) {  // one phase (PH = false: always), the taps are contiguous
 #pragma unroll
                 for (int i = 0; i < TAPS; i++) t[i] = src[i];
 #pragma unroll
@@ -285,13 +285,17 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
     }
 }
 
-template <int FR, int FC>
+// PH = true instantiations gather the taps of a phase sub-filter; the others
+// (one phase) copy contiguous taps and carry no gather code, whose per-tap
+// index arithmetic would otherwise be hoisted into (and spill) scalar
+// registers in every launch.
+template <int FR, int FC, bool PH>
 static hipError_t launch_rc(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const Phase& ph, const float* x,
                             const float* w, float* out, hipStream_t s, dim3 grid) {
     switch (c.MB) {
-    case 1: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 1>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
-    case 2: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 2>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
-    default: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 4>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
+    case 1: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 1, PH>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
+    case 2: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 2, PH>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
+    default: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 4, PH>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
     }
     return hipGetLastError();
 }
@@ -315,22 +319,32 @@ hipError_t launch_igemm_f32_phase(const Geom& g, const PlaneMap& m, const IgemmC
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)blocks);
     const int key = g.FH * 10 + g.FW;
+    if (ph.step == 1) {
+        switch (key) {
+        case 11: return launch_rc<1, 1, false>(g, m, c, ph, x, w, out, s, grid);
+        case 33: return launch_rc<3, 3, false>(g, m, c, ph, x, w, out, s, grid);
+        case 55: return launch_rc<5, 5, false>(g, m, c, ph, x, w, out, s, grid);
+        case 77: return launch_rc<7, 7, false>(g, m, c, ph, x, w, out, s, grid);
+        case 99: return launch_rc<9, 9, false>(g, m, c, ph, x, w, out, s, grid);
+        case 22: return launch_rc<2, 2, false>(g, m, c, ph, x, w, out, s, grid);
+        case 44: return launch_rc<4, 4, false>(g, m, c, ph, x, w, out, s, grid);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (key) {
-    case 11: return launch_rc<1, 1>(g, m, c, ph, x, w, out, s, grid);
-    case 33: return launch_rc<3, 3>(g, m, c, ph, x, w, out, s, grid);
-    case 55: return launch_rc<5, 5>(g, m, c, ph, x, w, out, s, grid);
-    case 77: return launch_rc<7, 7>(g, m, c, ph, x, w, out, s, grid);
-    case 99: return launch_rc<9, 9>(g, m, c, ph, x, w, out, s, grid);
-    case 22: return launch_rc<2, 2>(g, m, c, ph, x, w, out, s, grid);
-    case 44: return launch_rc<4, 4>(g, m, c, ph, x, w, out, s, grid);
-    case 12: return launch_rc<1, 2>(g, m, c, ph, x, w, out, s, grid);
-    case 21: return launch_rc<2, 1>(g, m, c, ph, x, w, out, s, grid);
-    case 23: return launch_rc<2, 3>(g, m, c, ph, x, w, out, s, grid);
-    case 32: return launch_rc<3, 2>(g, m, c, ph, x, w, out, s, grid);
-    case 34: return launch_rc<3, 4>(g, m, c, ph, x, w, out, s, grid);
-    case 43: return launch_rc<4, 3>(g, m, c, ph, x, w, out, s, grid);
-    case 45: return launch_rc<4, 5>(g, m, c, ph, x, w, out, s, grid);
-    case 54: return launch_rc<5, 4>(g, m, c, ph, x, w, out, s, grid);
+    case 11: return launch_rc<1, 1, true>(g, m, c, ph, x, w, out, s, grid);
+    case 22: return launch_rc<2, 2, true>(g, m, c, ph, x, w, out, s, grid);
+    case 33: return launch_rc<3, 3, true>(g, m, c, ph, x, w, out, s, grid);
+    case 44: return launch_rc<4, 4, true>(g, m, c, ph, x, w, out, s, grid);
+    case 55: return launch_rc<5, 5, true>(g, m, c, ph, x, w, out, s, grid);
+    case 12: return launch_rc<1, 2, true>(g, m, c, ph, x, w, out, s, grid);
+    case 21: return launch_rc<2, 1, true>(g, m, c, ph, x, w, out, s, grid);
+    case 23: return launch_rc<2, 3, true>(g, m, c, ph, x, w, out, s, grid);
+    case 32: return launch_rc<3, 2, true>(g, m, c, ph, x, w, out, s, grid);
+    case 34: return launch_rc<3, 4, true>(g, m, c, ph, x, w, out, s, grid);
+    case 43: return launch_rc<4, 3, true>(g, m, c, ph, x, w, out, s, grid);
+    case 45: return launch_rc<4, 5, true>(g, m, c, ph, x, w, out, s, grid);
+    case 54: return launch_rc<5, 4, true>(g, m, c, ph, x, w, out, s, grid);
     default: return hipErrorInvalidValue;
     }
 }
