@@ -123,6 +123,18 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     }
     const int colByte = colIn ? 4 * colOff : LP_OOB;
 
+    // 16-byte staging (c.vec4): the staged columns re-based to xa = x0 rounded
+    // down to a multiple of 4; lane (si, gc) loads the 4-column group gc of
+    // item slot si (IP items of NG groups per wave pass), eight planes as
+    // eight 16-byte loads, and writes the group's (up to) four pixels.
+    const int s4 = ((x0 % 4) + 4) % 4;
+    const int NG = (s4 + TCp + 3) >> 2;
+    const int IP = 64 / NG;
+    const int si = lane / NG, gc = lane - si * NG;
+    const int colA = x0 - s4 + 4 * gc;
+    const bool gIn = si < IP && (unsigned)colA < (unsigned)g.W;
+    const float invTRp = 1.0f / TRp;
+
     // Filter-slice units (term, tap, output plane, 8-plane quarter) of this
     // thread: byte offsets in the packed bank (the chunk adds 2*r0 as the
     // uniform offset) and LDS destinations (elements from sB; units past the
@@ -245,11 +257,53 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
             for (int i = 0; i < (WPRE ? WPT : 1); i++)
                 wv[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, wOff[i], 2 * r0, 0));
         }
-        for (int base = wave; base < nitems; base += 4 * IU) {
-            float v[IU][8];
-            issue(r0, base, v);
+        if (c.vec4) {  // block-uniform
+            constexpr int VP = (F + 12) / 6;  // passes: a wave owns TRp = F + 7 items, IP >= 6 a pass
+            float4 q[VP][8];
+            int dBase[VP];
+#pragma unroll
+            for (int pp = 0; pp < VP; pp++) {
+                const int j = pp * IP + si;  // the wave's item slot
+                const int it = wave + 4 * j;
+                const int grp = fast_div(it, TRp, invTRp), ty = it - grp * TRp;
+                const bool itemOk = si < IP && j < TRp;
+                const int y = y0 + ty;
+                const bool on = itemOk && gIn && (unsigned)y < (unsigned)g.H;
+                int vo = on ? 4 * ((r0 + 8 * grp) * HW + y * g.W + colA) : LP_OOB;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    q[pp][e] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, vo, 0, 0));
+                    vo += planeBytes;
+                }
+                dBase[pp] = itemOk ? ty * TCp * LP_RCP + 8 * grp : -1;
+            }
             __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
-            commit(base, v);
+#pragma unroll
+            for (int pp = 0; pp < VP; pp++) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int col = 4 * gc + k - s4;
+                    V hv, lv;
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const float x = k == 0 ? q[pp][e].x : (k == 1 ? q[pp][e].y : (k == 2 ? q[pp][e].z : q[pp][e].w));
+                        const T h = (T)x;
+                        hv[e] = h;
+                        if (SPLIT) lv[e] = (T)(x - (float)h);
+                    }
+                    const bool ok = dBase[pp] >= 0 && (unsigned)col < (unsigned)TCp;
+                    const size_t o = (size_t)dBase[pp] + (size_t)col * LP_RCP;
+                    *reinterpret_cast<V*>(sB + (ok ? o : sJunk + 8 * lane)) = hv;
+                    if (SPLIT) *reinterpret_cast<V*>(sB + (ok ? (size_t)npix * LP_RCP + o : sJunk + 512 + 8 * lane)) = lv;
+                }
+            }
+        } else {
+            for (int base = wave; base < nitems; base += 4 * IU) {
+                float v[IU][8];
+                issue(r0, base, v);
+                __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
+                commit(base, v);
+            }
         }
         // Filter slice from the pre-packed bank: 16-byte units (term, t, m, q).
         if (WPRE) {
@@ -403,10 +457,12 @@ static hipError_t launch_lp1(const Geom& g, const PlaneMap& m, const IgemmCfg& c
 
 bool igemm_lp_supported(int F) { return F == 1 || F == 3 || F == 5 || F == 7 || F == 9; }
 
-hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x,
+hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& cfg, const float* x,
                            const float* w, float* out, void* ws, hipStream_t s) {
-    const long blocks = (long)c.tilesC * c.tilesR * c.mTiles * m.B;
+    const long blocks = (long)cfg.tilesC * cfg.tilesR * cfg.mTiles * m.B;
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    IgemmCfg c = cfg;
+    if (reinterpret_cast<uintptr_t>(x) & 15) c.vec4 = 0;  // 16-byte loads need 16-byte rows
     switch (precision) {
     case COGCONV_BF16: return launch_lp1<COGCONV_BF16>(g, m, c, x, w, out, ws, s);
     case COGCONV_FP16: return launch_lp1<COGCONV_FP16>(g, m, c, x, w, out, ws, s);

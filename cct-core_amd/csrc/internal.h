@@ -153,6 +153,7 @@ struct IgemmCfg {
     int planeStride;  // LDS floats per staged input plane (padded)
     int aStride;      // LDS floats per staged filter-slice plane (padded)
     int splits;       // split-K count (FilterAdjoint)
+    int vec4;         // k_igemm_lp: stage 4-column groups with 16-byte loads (see choose_igemm_lp)
     size_t lds_bytes;
 };
 

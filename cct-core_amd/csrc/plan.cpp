@@ -426,6 +426,12 @@ bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ig
     c.tileRows = 7 * g.dnR + g.FH;
     c.tileCols = 31 * g.dnC + g.FW;
     c.RC = (m.R + 31) / 32 * 32;  // planes padded to whole 32-deep MFMA slices
+    // 16-byte staging loads of 4-column groups: groups aligned to 4 columns are
+    // wholly inside or outside a row when W % 4 == 0, so borders that read 0
+    // outside the field need no per-column remap; unit steps keep the staged
+    // columns contiguous.  (The launch also checks the input's alignment.)
+    c.vec4 = (g.border == COGCONV_BORDER_ZERO || g.border == COGCONV_BORDER_VALID ||
+              g.border == COGCONV_BORDER_FULL) && g.dnR == 1 && g.dnC == 1 && g.W % 4 == 0 && g.FH <= 9;
     for (int MB = c.Mout <= 16 ? 1 : (c.Mout <= 32 ? 2 : 4); MB >= 1; MB = MB == 4 ? 2 : MB - 1) {
         c.MB = MB;
         c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
@@ -842,8 +848,9 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
         break;
     case KK_IGEMM_LP: {
         static const char* pn[] = {"fp32", "bf16", "fp16", "split_bf16"};
-        snprintf(buf, len, "igemm_%s %s out=%dx%dx%d MB=%d tile=8x32 threads=256 lds=%zu", pn[plan->desc.precision],
-                 mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->icfg.MB, plan->icfg.lds_bytes);
+        snprintf(buf, len, "igemm_%s %s out=%dx%dx%d MB=%d tile=8x32 threads=256%s lds=%zu", pn[plan->desc.precision],
+                 mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->icfg.MB, plan->icfg.vec4 ? " ld16" : "",
+                 plan->icfg.lds_bytes);
         break;
     }
     case KK_IGEMM_UP: {

@@ -63,6 +63,42 @@ def test_precision_variant_accuracy(precision, mode, border, F):
         assert err <= TOL[precision], (NAMES[precision], wide, err)
 
 
+VEC4_CASES = []
+for _mode in (O.PF_BRS, O.BPF_BRS):
+    for _border in (O.ZERO, O.VALID, O.FULL):
+        for _F in (1, 3, 5, 7, 9):
+            VEC4_CASES.append((_mode, _border, _F))
+
+
+@pytest.mark.parametrize("precision", (L.BF16, L.FP16, L.SPLIT_BF16))
+@pytest.mark.parametrize("mode,border,F", VEC4_CASES)
+def test_precision_vec4_staging(precision, mode, border, F):
+    """Field widths that are multiples of 4 with the borders that read 0 outside
+    the field: the kernel stages 4-column groups with 16-byte loads.  Ragged
+    column and row tiles; the plane count leaves a partial 8-plane group."""
+    rng = np.random.default_rng(700 + 100 * precision + 10 * border + F + mode)
+    B, N, K, H, W = 2, 36, 20, 23, 68
+    a = inputs(rng, (B * N, H, W), False)
+    k = inputs(rng, (K * N, F, F), False)
+    orient = O.CONV if mode == O.BPF_BRS else O.XCORR
+    od, cd = desc_pair(mode, border, orient, O.NONE, 1, B, a.shape, k.shape, precision=precision)
+    try:
+        plan = L.Plan(cd)
+    except L.CogconvError as e:
+        assert precision == L.SPLIT_BF16 and e.status == L.ERR_UNSUPPORTED
+        return
+    desc = plan.description
+    plan.close()
+    assert "ld16" in desc, desc
+    print("CASE vec4", NAMES[precision], mode, border, F, desc, flush=True)
+    got = run_gpu(cd, a, k)
+    ref = O.conv(od, a, k)
+    err = normwise(got, ref)
+    print("  normwise error %.3g" % err)
+    assert got.shape == ref.shape
+    assert err <= TOL[precision]
+
+
 def test_precision_c3_shape_uniform():
     """B32 128x128x64 planes, 64 filters 3x3x64, BorderZero on a 4-image slice."""
     rng = np.random.default_rng(33)
