@@ -273,7 +273,8 @@ def large_workload(args):
     print(json.dumps(line), flush=True)
 
 
-RATE_SWEEP = (4000, 8000, 12000, 16000, 24000, 32000, 48000, 64000)  # offered req/s, resident transport
+RATE_SWEEP = (4000, 8000, 12000, 16000, 20000, 24000, 28000, 32000, 40000, 48000, 56000, 64000, 72000, 80000,
+              96000)  # offered req/s, resident transport
 P99_TARGETS_MS = (0.5, 1.0, 2.0)
 
 

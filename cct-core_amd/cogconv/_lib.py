@@ -133,6 +133,8 @@ SIGNATURES = {
     "cogconv_plan_bytes": [_P, ctypes.POINTER(ctypes.c_double)],
     "cogconv_plan_describe": [_P, ctypes.c_char_p, _SZ],
     "cogconv_launch": [_P, _P, _P, _P, _P, _P],
+    "cogconv_plan_pack_operand": [_P, _P, _P, _P, _P],
+    "cogconv_launch_packed": [_P, _P, _P, _P, _P],
     "cogconv_block_reduce_sum": [_P, _P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _P],
     "cogconv_plan_variant_count": [ctypes.POINTER(DescC), ctypes.POINTER(ctypes.c_int32)],
     "cogconv_plan_create_variant": [ctypes.POINTER(DescC), ctypes.c_int32, ctypes.POINTER(_P)],
@@ -366,6 +368,22 @@ class Plan:
             return x.ptr if isinstance(x, DeviceBuffer) else x
         s = stream.handle if isinstance(stream, Stream) else stream
         check(lib().cogconv_launch(self.handle, ptr(in0), ptr(in1), ptr(out), ptr(workspace), s))
+
+    def pack_operand(self, in1, workspace, stream=None):
+        """cogconv_plan_pack_operand: True when the plan packed operand 2 into workspace."""
+        def ptr(x):
+            return x.ptr if isinstance(x, DeviceBuffer) else x
+        s = stream.handle if isinstance(stream, Stream) else stream
+        packed = ctypes.c_int32()
+        check(lib().cogconv_plan_pack_operand(self.handle, ptr(in1), ptr(workspace), s, ctypes.byref(packed)))
+        return bool(packed.value)
+
+    def launch_packed(self, in0, out, workspace, stream=None):
+        """cogconv_launch_packed: run on the operand 2 packed earlier into workspace."""
+        def ptr(x):
+            return x.ptr if isinstance(x, DeviceBuffer) else x
+        s = stream.handle if isinstance(stream, Stream) else stream
+        check(lib().cogconv_launch_packed(self.handle, ptr(in0), ptr(out), ptr(workspace), s))
 
     def close(self):
         if getattr(self, "handle", None):

@@ -197,7 +197,6 @@ __global__ __launch_bounds__(256) void k_direct_rt(Geom g, PlaneMap m, DirectCfg
     const int cg = tid & 31, rg = tid >> 5;
     const int orow0 = tr * ROWS, ocol0 = tc * RT_COLS;
     const int y0 = orow0 - g.hR, x0 = ocol0 - g.hC;
-    const int TCc = RT_COLS + FW - 1;     // staged columns
     // Staging: half-block h (wave-uniform) stages the first 128 columns of rows
     // h, h+2, ... with lane column xs; the FW - 1 extra columns of every row are
     // spread over the block (up to KE elements a thread).  Every load of a plane

@@ -415,17 +415,21 @@ size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision) {
     return (size_t)terms * F * F * (size_t)c.mTiles * 16 * c.MB * c.RC * 2;
 }
 
+// stage: LP_PACK_AND_RUN (both kernels), LP_PACK_ONLY (the filter pre-pack
+// into ws), LP_RUN_ONLY (the convolution on a bank packed earlier into ws).
 template <int F, int MB, int PREC>
 static hipError_t launch_lp3(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
-                             float* out, void* ws, hipStream_t s) {
+                             float* out, void* ws, hipStream_t s, int stage) {
     typedef typename LpType<PREC>::T T;
     T* wp = static_cast<T*>(ws);
-    const long packed = (long)F * F * c.mTiles * 16 * MB * c.RC;
-    const long pblocks = std::min((packed + 255) / 256, 4096L);
-    hipLaunchKernelGGL((k_pack_filters_lp<PREC>), dim3((unsigned)pblocks), dim3(256), 0, s, g, m, c.Mout,
-                       c.mTiles * 16 * MB, c.RC, w, wp);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (stage != LP_RUN_ONLY) {
+        const long packed = (long)F * F * c.mTiles * 16 * MB * c.RC;
+        const long pblocks = std::min((packed + 255) / 256, 4096L);
+        hipLaunchKernelGGL((k_pack_filters_lp<PREC>), dim3((unsigned)pblocks), dim3(256), 0, s, g, m, c.Mout,
+                           c.mTiles * 16 * MB, c.RC, w, wp);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess || stage == LP_PACK_ONLY) return e;
+    }
     const long blocks = (long)c.tilesC * c.tilesR * c.mTiles * m.B;
     hipLaunchKernelGGL((k_igemm_lp<F, MB, PREC>), dim3((unsigned)blocks), dim3(256), c.lds_bytes, s, g, m, c, x, wp,
                        out);
@@ -434,23 +438,23 @@ static hipError_t launch_lp3(const Geom& g, const PlaneMap& m, const IgemmCfg& c
 
 template <int F, int PREC>
 static hipError_t launch_lp2(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
-                             float* out, void* ws, hipStream_t s) {
+                             float* out, void* ws, hipStream_t s, int stage) {
     switch (c.MB) {
-    case 1: return launch_lp3<F, 1, PREC>(g, m, c, x, w, out, ws, s);
-    case 2: return launch_lp3<F, 2, PREC>(g, m, c, x, w, out, ws, s);
-    default: return launch_lp3<F, 4, PREC>(g, m, c, x, w, out, ws, s);
+    case 1: return launch_lp3<F, 1, PREC>(g, m, c, x, w, out, ws, s, stage);
+    case 2: return launch_lp3<F, 2, PREC>(g, m, c, x, w, out, ws, s, stage);
+    default: return launch_lp3<F, 4, PREC>(g, m, c, x, w, out, ws, s, stage);
     }
 }
 
 template <int PREC>
 static hipError_t launch_lp1(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
-                             float* out, void* ws, hipStream_t s) {
+                             float* out, void* ws, hipStream_t s, int stage) {
     switch (g.FH) {
-    case 1: return launch_lp2<1, PREC>(g, m, c, x, w, out, ws, s);
-    case 3: return launch_lp2<3, PREC>(g, m, c, x, w, out, ws, s);
-    case 5: return launch_lp2<5, PREC>(g, m, c, x, w, out, ws, s);
-    case 7: return launch_lp2<7, PREC>(g, m, c, x, w, out, ws, s);
-    case 9: return launch_lp2<9, PREC>(g, m, c, x, w, out, ws, s);
+    case 1: return launch_lp2<1, PREC>(g, m, c, x, w, out, ws, s, stage);
+    case 3: return launch_lp2<3, PREC>(g, m, c, x, w, out, ws, s, stage);
+    case 5: return launch_lp2<5, PREC>(g, m, c, x, w, out, ws, s, stage);
+    case 7: return launch_lp2<7, PREC>(g, m, c, x, w, out, ws, s, stage);
+    case 9: return launch_lp2<9, PREC>(g, m, c, x, w, out, ws, s, stage);
     default: return hipErrorInvalidValue;
     }
 }
@@ -458,15 +462,15 @@ static hipError_t launch_lp1(const Geom& g, const PlaneMap& m, const IgemmCfg& c
 bool igemm_lp_supported(int F) { return F == 1 || F == 3 || F == 5 || F == 7 || F == 9; }
 
 hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& cfg, const float* x,
-                           const float* w, float* out, void* ws, hipStream_t s) {
+                           const float* w, float* out, void* ws, hipStream_t s, int stage) {
     const long blocks = (long)cfg.tilesC * cfg.tilesR * cfg.mTiles * m.B;
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     IgemmCfg c = cfg;
     if (reinterpret_cast<uintptr_t>(x) & 15) c.vec4 = 0;  // 16-byte loads need 16-byte rows
     switch (precision) {
-    case COGCONV_BF16: return launch_lp1<COGCONV_BF16>(g, m, c, x, w, out, ws, s);
-    case COGCONV_FP16: return launch_lp1<COGCONV_FP16>(g, m, c, x, w, out, ws, s);
-    case COGCONV_SPLIT_BF16: return launch_lp1<COGCONV_SPLIT_BF16>(g, m, c, x, w, out, ws, s);
+    case COGCONV_BF16: return launch_lp1<COGCONV_BF16>(g, m, c, x, w, out, ws, s, stage);
+    case COGCONV_FP16: return launch_lp1<COGCONV_FP16>(g, m, c, x, w, out, ws, s, stage);
+    case COGCONV_SPLIT_BF16: return launch_lp1<COGCONV_SPLIT_BF16>(g, m, c, x, w, out, ws, s, stage);
     default: return hipErrorInvalidValue;
     }
 }

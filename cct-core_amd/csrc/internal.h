@@ -226,8 +226,9 @@ hipError_t launch_fa_igemm_lp(int precision, const Geom& g, const PlaneMap& m, c
 bool igemm_lp_supported(int F);
 size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision);
 size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision);
+enum LpStage { LP_PACK_AND_RUN = 0, LP_PACK_ONLY = 1, LP_RUN_ONLY = 2 };
 hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x,
-                           const float* w, float* out, void* ws, hipStream_t s);
+                           const float* w, float* out, void* ws, hipStream_t s, int stage = LP_PACK_AND_RUN);
 hipError_t launch_block_reduce_sum(const float* in, float* out, int64_t points, int out_planes, int factor,
                                    hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);

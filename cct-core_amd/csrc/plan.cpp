@@ -889,6 +889,30 @@ extern "C" int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len) {
     return COGCONV_OK;
 }
 
+extern "C" int cogconv_plan_pack_operand(cogconv_plan plan, const float* in1, void* workspace, cogconv_stream stream,
+                                         int32_t* packed) {
+    clear_error();
+    if (!plan || !packed) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    *packed = 0;
+    if (plan->kind != KK_IGEMM_LP) return COGCONV_OK;
+    if (!in1 || !workspace) return fail(COGCONV_ERR_INVALID_ARG, "null operand or workspace");
+    const hipError_t e = launch_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->icfg, nullptr, in1, nullptr,
+                                         workspace, reinterpret_cast<hipStream_t>(stream), LP_PACK_ONLY);
+    if (int st = check_hip(e, "operand pack")) return st;
+    *packed = 1;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_launch_packed(cogconv_plan plan, const float* in0, float* out, void* workspace,
+                                     cogconv_stream stream) {
+    clear_error();
+    if (!plan || !in0 || !out || !workspace) return fail(COGCONV_ERR_INVALID_ARG, "null plan or buffer");
+    if (plan->kind != KK_IGEMM_LP) return fail(COGCONV_ERR_INVALID_ARG, "plan does not pack operand 2");
+    const hipError_t e = launch_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->icfg, in0, nullptr, out,
+                                         workspace, reinterpret_cast<hipStream_t>(stream), LP_RUN_ONLY);
+    return check_hip(e, "convolution launch");
+}
+
 extern "C" int cogconv_launch(cogconv_plan plan, const float* in0, const float* in1, float* out, void* workspace,
                               cogconv_stream stream) {
     clear_error();

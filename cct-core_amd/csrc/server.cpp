@@ -142,7 +142,24 @@ struct cogconv_server_s {
     std::map<std::string, cogconv_plan> plans;
     std::map<std::string, std::pair<size_t, double>> plan_info;  // desc -> (workspace, flops), for submit
     std::vector<hipEvent_t> event_pool;
-    std::thread worker;
+    // Resident banks packed once per (plan, bank) for the plans that pack
+    // operand 2 (the low-precision frame kernels): arena offset, kept for the
+    // server's life; plans that do not pack map to SIZE_MAX.
+    std::map<std::pair<cogconv_plan, int>, size_t> packed;
+    // Worker threads (two when there are two streams or more): each admits
+    // requests, issues launches and retires completions.  All shared state is
+    // guarded by mu; issue() drops the lock only around runtime calls
+    // (launches, copies, event records), which touch no shared structure, so
+    // that the workers' runtime calls and the submitters proceed in parallel.
+    std::vector<std::thread> workers;
+    int issuing = 0;  // launches being issued with mu dropped (count against max_inflight)
+    template <class Fn>
+    static auto unlocked(std::unique_lock<std::mutex>& lk, Fn fn) {
+        lk.unlock();
+        auto r = fn();
+        lk.lock();
+        return r;
+    }
     bool stop = false;
     uint64_t next_ticket = 1;
     int rr = 0;
@@ -183,7 +200,7 @@ struct cogconv_server_s {
 
     // Issue one launch for the requests in g (one bucket).  Called with mu held.
     // Returns false when the arena cannot hold it now (nothing issued).
-    bool issue(const std::vector<Req*>& g) {
+    bool issue(const std::vector<Req*>& g, std::unique_lock<std::mutex>& lk) {
         const Req* h = g.front();
         cogconv_desc d = h->r.desc;
         int B = 0, P0 = 0;
@@ -207,19 +224,55 @@ struct cogconv_server_s {
         const bool resident = (h->r.flags & COGCONV_REQUEST_RESIDENT) != 0;
         if (resident) {
             // State is already in HBM: only the workspace comes from the budget.
-            size_t o_ws = 0;
-            if (ws && !arena.alloc(ws, &o_ws)) return false;
-            std::vector<std::pair<size_t, size_t>> blocks;
-            if (ws) blocks.push_back({o_ws, ws});
             const int si = rr++ % (int)streams.size();
             hipStream_t s = streams[si];
             const float* in1 = h->r.bank < 0 ? h->r.host_in1 : banks[h->r.bank].dptr;
-            int st = cogconv_launch(plan, h->r.host_in0, in1, h->r.host_out, ws ? arena.base + o_ws : nullptr,
-                                    reinterpret_cast<cogconv_stream>(s));
+            // A resident bank a plan packs is packed once (synchronously, the
+            // first time) and every later request runs on the packed form.
+            size_t pk = SIZE_MAX;
+            if (h->r.bank >= 0 && ws) {
+                const auto key = std::make_pair(plan, h->r.bank);
+                auto it = packed.find(key);
+                if (it == packed.end()) {
+                    size_t o = 0;
+                    if (!arena.alloc(ws, &o)) return false;
+                    int32_t did = 0;
+                    const int pst = unlocked(lk, [&] {
+                        int r = cogconv_plan_pack_operand(plan, in1, arena.base + o,
+                                                          reinterpret_cast<cogconv_stream>(s), &did);
+                        if (r == COGCONV_OK && did) r = check_hip(hipStreamSynchronize(s), "bank pack");
+                        return r;
+                    });
+                    if (pst != COGCONV_OK) {
+                        arena.release(o, ws);
+                        fail_reqs(g, pst, cogconv_last_error());
+                        return true;
+                    }
+                    it = packed.find(key);  // another worker may have packed it meanwhile
+                    if (it != packed.end() || !did) arena.release(o, ws);
+                    if (it == packed.end()) it = packed.emplace(key, did ? o : SIZE_MAX).first;
+                }
+                pk = it->second;
+            }
+            std::vector<std::pair<size_t, size_t>> blocks;
+            size_t o_ws = 0;
+            if (pk == SIZE_MAX && ws) {
+                if (!arena.alloc(ws, &o_ws)) return false;
+                blocks.push_back({o_ws, ws});
+            }
             hipEvent_t ev = nullptr;
-            hipError_t e = hipSuccess;
-            if (st == COGCONV_OK) e = take_event(&ev);
-            if (st == COGCONV_OK && e == hipSuccess) e = hipEventRecord(ev, s);
+            hipError_t e = take_event(&ev);
+            const int st = e != hipSuccess ? COGCONV_ERR_RUNTIME : unlocked(lk, [&] {
+                int r;
+                if (pk != SIZE_MAX)
+                    r = cogconv_launch_packed(plan, h->r.host_in0, h->r.host_out, arena.base + pk,
+                                              reinterpret_cast<cogconv_stream>(s));
+                else
+                    r = cogconv_launch(plan, h->r.host_in0, in1, h->r.host_out, ws ? arena.base + o_ws : nullptr,
+                                       reinterpret_cast<cogconv_stream>(s));
+                if (r == COGCONV_OK) e = hipEventRecord(ev, s);
+                return r;
+            });
             if (st != COGCONV_OK || e != hipSuccess) {
                 (void)hipStreamSynchronize(s);
                 for (auto& b : blocks) arena.release(b.first, b.second);
@@ -247,30 +300,32 @@ struct cogconv_server_s {
         const int si = rr++ % (int)streams.size();
         hipStream_t s = streams[si];
         char* base = arena.base;
-        size_t at = 0;
-        hipError_t e = hipSuccess;
-        for (Req* q : g) {
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(base + o_in0 + at, q->r.host_in0, q->in0_bytes, hipMemcpyHostToDevice, s);
-            at += q->in0_bytes;
-        }
-        const float* in1 = sendIn1 ? reinterpret_cast<const float*>(base + o_in1) : banks[h->r.bank].dptr;
-        if (e == hipSuccess && sendIn1)
-            e = hipMemcpyAsync(base + o_in1, h->r.host_in1, h->in1_bytes, hipMemcpyHostToDevice, s);
-        int st = COGCONV_OK;
-        if (e == hipSuccess)
-            st = cogconv_launch(plan, reinterpret_cast<const float*>(base + o_in0), in1,
-                                reinterpret_cast<float*>(base + o_out), ws ? base + o_ws : nullptr,
-                                reinterpret_cast<cogconv_stream>(s));
-        at = 0;
-        for (Req* q : g) {
-            if (e == hipSuccess && st == COGCONV_OK)
-                e = hipMemcpyAsync(q->r.host_out, base + o_out + at, q->out_bytes, hipMemcpyDeviceToHost, s);
-            at += q->out_bytes;
-        }
         hipEvent_t ev = nullptr;
-        if (e == hipSuccess && st == COGCONV_OK) e = take_event(&ev);
-        if (e == hipSuccess && st == COGCONV_OK) e = hipEventRecord(ev, s);
+        hipError_t e = take_event(&ev);
+        const float* in1 = sendIn1 ? reinterpret_cast<const float*>(base + o_in1) : banks[h->r.bank].dptr;
+        const int st = e != hipSuccess ? COGCONV_ERR_RUNTIME : unlocked(lk, [&] {
+            size_t at = 0;
+            for (Req* q : g) {
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(base + o_in0 + at, q->r.host_in0, q->in0_bytes, hipMemcpyHostToDevice, s);
+                at += q->in0_bytes;
+            }
+            if (e == hipSuccess && sendIn1)
+                e = hipMemcpyAsync(base + o_in1, h->r.host_in1, h->in1_bytes, hipMemcpyHostToDevice, s);
+            int r = COGCONV_OK;
+            if (e == hipSuccess)
+                r = cogconv_launch(plan, reinterpret_cast<const float*>(base + o_in0), in1,
+                                   reinterpret_cast<float*>(base + o_out), ws ? base + o_ws : nullptr,
+                                   reinterpret_cast<cogconv_stream>(s));
+            at = 0;
+            for (Req* q : g) {
+                if (e == hipSuccess && r == COGCONV_OK)
+                    e = hipMemcpyAsync(q->r.host_out, base + o_out + at, q->out_bytes, hipMemcpyDeviceToHost, s);
+                at += q->out_bytes;
+            }
+            if (e == hipSuccess && r == COGCONV_OK) e = hipEventRecord(ev, s);
+            return r;
+        });
         if (e != hipSuccess || st != COGCONV_OK) {
             (void)hipStreamSynchronize(s);
             for (auto& b : blocks) arena.release(b.first, b.second);
@@ -318,7 +373,10 @@ struct cogconv_server_s {
                 cv_done.notify_all();
             }
             // Admissions, in arrival order, while slots and budget allow.
-            while (!pending.empty() && (int)inflight.size() < cfg.max_inflight) {
+            // The requests of a launch leave `pending` before it is issued (the
+            // other worker must not take them while mu is dropped) and go back
+            // to its front if the budget cannot hold them yet.
+            while (!pending.empty() && (int)inflight.size() + issuing < cfg.max_inflight) {
                 Req* h = pending.front();
                 std::vector<Req*> g{h};
                 std::vector<size_t> idx{0};
@@ -332,20 +390,23 @@ struct cogconv_server_s {
                         }
                     }
                 }
-                if (!issue(g)) {
-                    if (g.size() > 1) {
-                        g.resize(1);
-                        idx.resize(1);
-                        if (!issue(g)) break;
-                    } else {
-                        break;  // wait for memory
-                    }
-                }
                 for (size_t k = idx.size(); k-- > 0;) pending.erase(pending.begin() + idx[k]);
+                issuing++;
+                bool ok = issue(g, lk);
+                if (!ok && g.size() > 1) {
+                    for (size_t k = g.size(); k-- > 1;) pending.push_front(g[k]);
+                    g.resize(1);
+                    ok = issue(g, lk);
+                }
+                issuing--;
+                if (!ok) {
+                    pending.push_front(g[0]);
+                    break;  // wait for memory
+                }
             }
-            if (stop && pending.empty() && inflight.empty()) break;
+            if (stop && pending.empty() && inflight.empty() && issuing == 0) break;
             if (inflight.empty()) {
-                cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+                cv_work.wait(lk, [&] { return stop || !pending.empty() || !inflight.empty(); });
             } else {
                 // Launches in flight: poll again at once (a timed sleep rounds up to
                 // the kernel's timer slack, tens of microseconds, which would cap
@@ -387,7 +448,8 @@ extern "C" int cogconv_server_create(const cogconv_server_config* config, cogcon
         }
         s->streams.push_back(st);
     }
-    s->worker = std::thread([s] { s->run(); });
+    const int nw = config->streams >= 2 ? 2 : 1;
+    for (int i = 0; i < nw; i++) s->workers.emplace_back([s] { s->run(); });
     *server = s;
     return COGCONV_OK;
 }
@@ -400,7 +462,8 @@ extern "C" int cogconv_server_destroy(cogconv_server s) {
         s->stop = true;
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    for (auto& w : s->workers)
+        if (w.joinable()) w.join();
     for (auto st : s->streams) (void)hipStreamDestroy(st);
     for (auto ev : s->event_pool) (void)hipEventDestroy(ev);
     for (auto& b : s->banks) (void)hipFree(b.dptr);
@@ -504,7 +567,7 @@ extern "C" int cogconv_server_submit(cogconv_server s, const cogconv_request* re
         s->pending.push_back(q);
         *ticket = q->ticket;
     }
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return COGCONV_OK;
 }
 
@@ -543,7 +606,7 @@ extern "C" int cogconv_server_drain(cogconv_server s) {
     clear_error();
     if (!s) return fail(COGCONV_ERR_INVALID_ARG, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
-    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->inflight.empty(); });
+    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->inflight.empty() && s->issuing == 0; });
     return COGCONV_OK;
 }
 

@@ -218,6 +218,18 @@ int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len);
 int cogconv_launch(cogconv_plan plan, const float* in0, const float* in1, float* out, void* workspace,
                    cogconv_stream stream);
 
+/* Resident operand 2 (no counterpart in the reference: for callers that apply one
+ * filter bank to many inputs, such as the serving loop).  The BF16 / FP16 /
+ * split-BF16 frame plans convert operand 2 into a packed low-precision form in
+ * the workspace on every cogconv_launch.  cogconv_plan_pack_operand does that
+ * conversion once (*packed = 1; 0 for plans that do not pack, which leave the
+ * workspace alone), and cogconv_launch_packed then runs the convolution on the
+ * packed form kept in `workspace` (COGCONV_ERR_INVALID_ARG for plans that do
+ * not pack).  The workspace must not be written between the two. */
+int cogconv_plan_pack_operand(cogconv_plan plan, const float* in1, void* workspace, cogconv_stream stream,
+                              int32_t* packed);
+int cogconv_launch_packed(cogconv_plan plan, const float* in0, float* out, void* workspace, cogconv_stream stream);
+
 /* ---- autotuning ----
  * Counterpart of Profiler.bestVariant ($M/runtime/execution/Profiler.scala:131-190) and
  * ProfilerCache ($M/runtime/execution/ProfilerCache.scala:39-122).  Every plan has a list of
