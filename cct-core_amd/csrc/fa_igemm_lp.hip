@@ -144,7 +144,9 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
         // applied), LDS row base (-1: not live), planes left (K - k0) and
         // tap-row mask, read back per pass with readlane / a ballot bit; then
         // every load, then the rounding stores.
-        constexpr int DUD = 8, DUX = 12;  // passes per batch: 64 / 96 samples in flight per lane
+        // Passes per batch: 64 / 96 samples in flight per lane (half that for the
+        // two-term split, whose rounding doubles the registers a sample holds).
+        constexpr int DUD = SPLIT ? 4 : 8, DUX = SPLIT ? 6 : 12;
         const int dRows = MT * rr;
         const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * FHW, (uint32_t)K * FHW * 4u);
         for (int pb = wave * RPP; pb < dRows; pb += 4 * RPP * DUD) {
