@@ -256,20 +256,32 @@ IgemmCfg choose_igemm_frame(const Geom& g, const PlaneMap& m, int mbOnly = 0, in
         }
         int as = taps * 16 * MB;
         as += ((16 - as % 32) + 32) % 32;
+        // Of the chunks that fit, the one staging the fewest plane slots in all
+        // (ceil(R / RC) * RC: 32 planes go as 16 + 16, not 24 + 8), then the
+        // larger chunk.
+        int best = 0;
+        size_t bestBytes = 0;
+        long bestSlots = 0;
         for (int rc = maxRC; rc >= 8; rc -= 8) {
-            const int RC = rc;
-            if (rcOnly && RC != rcOnly) continue;
-            const size_t stage = ((size_t)RC * (ps + as) + 128) * sizeof(float);  // + scratch slots
+            if (rcOnly && rc != rcOnly) continue;
+            const size_t stage = ((size_t)rc * (ps + as) + 128) * sizeof(float);  // + scratch slots
             const size_t xch = igemm_f32_exchange_floats(MB) * sizeof(float);
             const size_t bytes = std::max(stage, xch);
-            if (bytes <= limit) {
-                c.MB = MB;
-                c.RC = RC;
-                c.aStride = as;
-                c.lds_bytes = bytes;
-                c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
-                return c;
+            if (bytes > limit) continue;
+            const long slots = (long)((m.R + rc - 1) / rc) * rc;
+            if (!best || slots < bestSlots) {
+                best = rc;
+                bestBytes = bytes;
+                bestSlots = slots;
             }
+        }
+        if (best) {
+            c.MB = MB;
+            c.RC = best;
+            c.aStride = as;
+            c.lds_bytes = bestBytes;
+            c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
+            return c;
         }
         if (MB == 1) break;
     }
