@@ -10,15 +10,16 @@
 // (image b, tap row i, tap column j) -- tens of thousands of terms.
 //
 // gfx950 mapping: v_mfma_f32_16x16x4_f32; each MFMA consumes four consecutive
-// tap columns of one tap row (the four k-lanes, lane/16).  A block (4 wave64)
-// owns a column group of CBG 16-column blocks and all MB row blocks.  It
+// tap columns of one tap row (the four k-lanes, lane/16).  A block (NW = 4 or
+// 8 wave64) owns a column group of CBG 16-column blocks and all MB row blocks.  It
 // stages RR tap rows of all K dy planes of one image (stored k-fastest with a
 // row pitch of 17 mod 32 words, spreading both the staging stores and the
 // A-operand reads over the banks)
 // plus the x rows they touch, for the input planes of its column group; staging
 // runs one row per wave pass with lanes across columns.  Wave w reduces tap
-// rows w, w+4, .. for every tile of the group, and the four waves' sums meet in
-// LDS.  Blocks split the (image, tap-row chunk) space; each writes one partial
+// rows w%4, w%4+4, .. for every tile of the group; with 8 waves, waves w and
+// w+4 (one SIMD) take the two halves of the tap columns, so one wave's MFMAs
+// run while the other waits on LDS.  The waves' sums meet in LDS.  Blocks split the (image, tap-row chunk) space; each writes one partial
 // dW, and the partials are added in a fixed order by a two-level reduction, so
 // results do not depend on scheduling.
 #include "stage.h"
@@ -27,8 +28,8 @@ namespace cogconv {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-template <int MB, int CBG>
-__global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
+template <int MB, int CBG, int NW>
+__global__ __launch_bounds__(64 * NW) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
                                                       const float* __restrict__ DY, float* __restrict__ ws) {
     // Staged dy rows are k-fastest with a pitch of 16*MB padded to 17 mod 32
     // words: the staging stores (lanes along tap columns, stride MTP) hit 32
@@ -127,14 +128,14 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         __syncthreads();
         // Staging of the dy tap rows and x rows: one row per wave pass, lanes
         // across columns (at most 128).  Per batch of UD dy rows and UX x rows
-        // (rows w, w+4, ... of each set): the rows' offsets and masks (per-lane
+        // (rows w, w+NW, ... of each set): the rows' offsets and masks (per-lane
         // tables), then all loads of both sets, from
         // in-bounds offsets or the out-of-range offset, with no branch between
         // them, then the LDS stores; rows past the end and lanes past the row
         // width store into a scratch slot instead of branching.  One batch
         // covers a block's rows for the common shapes, so staging costs one
         // memory round trip.
-        constexpr int UD = 16 * MB, UX = 24;  // UD: a wave's dy rows when RR = 4
+        constexpr int UD = 64 * MB / NW, UX = 96 / NW;  // UD: a wave's dy rows when RR = 4
         constexpr int MT = 16 * MB;    // dy planes the MFMAs read (K padded with zeros)
         const int dRows = MT * rr;     // (il, k); the pitch padding beyond MT is never read
         const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * FHW, (uint32_t)K * FHW * 4u);
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
         const int xRowsAll = nCnt * c.xRows;  // (plane, row)
         const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + ((size_t)b * N + n_lo) * HWx, (uint32_t)nCnt * HWx * 4u);
         const int vx0 = xIn0 ? 4 * xCol0 : STAGE_OOB, vx1 = xIn1 ? 4 * xCol1 : STAGE_OOB;
-        for (int bd = wave, bx = wave; bd < dRows || bx < xRowsAll; bd += 4 * UD, bx += 4 * UX) {
+        for (int bd = wave, bx = wave; bd < dRows || bx < xRowsAll; bd += NW * UD, bx += NW * UX) {
             // Per-lane row tables of the batch (lane u = the wave's u-th dy row
             // and u-th x row): byte offsets (flip, up-sampling lattice and
             // clamps applied), LDS destinations (-1: not live) and in-field
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
             int dSoff, dDst, xSoff, xDst;
             uint64_t dMask, xMask;
             {
-                const int row = bd + 4 * lane;
+                const int row = bd + NW * lane;
                 const int il = row / MT, k = row - il * MT;
                 int ti = i0 + il;
                 if (g.flip) ti = g.FHe - 1 - ti;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
                 dMask = __ballot(on);
             }
             {
-                const int row = bx + 4 * lane;
+                const int row = bx + NW * lane;
                 const int nl = row / c.xRows, lr = row - nl * c.xRows;
                 const int gy = i0 + lr;
                 const bool live = lane < UX && row < xRowsAll;
@@ -195,12 +196,16 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
             else store_rows_fn<false, UX>(xdst, sX, sJunk, lane, c.xCols, 1, xv0, xv1);
         }
         __syncthreads();
-        // Wave w reduces tap rows w, w+4, ... of the staged chunk for every
+        // Wave w reduces tap rows w%4, w%4+4, ... of the staged chunk (tap
+        // columns [jb0, jb1): all of them, or half with 8 waves) for every
         // output tile of the block's column group.
-        for (int il = wave; il < rr; il += 4) {
+        constexpr int CP = NW / 4;
+        const int jq = (c.DWp / 4 + CP - 1) / CP * 4;
+        const int jb0 = (wave / 4) * jq, jb1 = min(c.DWp, jb0 + jq);
+        for (int il = wave & 3; il < rr; il += 4) {
             const float* dRow = sD + ((size_t)il * c.DWp + g4) * MTp + l16;
             const float* xRow = sX + il * c.xColsP;
-            for (int jb = 0; jb < c.DWp; jb += 4) {
+            for (int jb = jb0; jb < jb1; jb += 4) {
                 float av[MB];
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++) av[mb] = dRow[(size_t)jb * MTp + mb * 16];
@@ -215,45 +220,83 @@ __global__ __launch_bounds__(256) void k_fa_igemm_f32(Geom g, PlaneMap m, FaCfg 
             }
         }
     }
-    // The four waves' sums meet in LDS in a fixed order, (w0 + w2) + (w1 + w3):
-    // waves 2 and 3 park their sums, waves 0 and 1 add them and park the
-    // results, then every wave writes a quarter of the block's partial dW,
-    // ws[s][k][j] (wave w the accumulator element v = w).
+    // The waves' sums meet in LDS in a fixed order and every wave then writes
+    // a share of the block's partial dW, ws[s][k][j].
     // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
     constexpr int E = MB * CBG * 4;  // accumulator floats per lane
     __syncthreads();
     float* red = smem;
-    if (wave >= 2) {
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-            for (int q = 0; q < CBG; q++)
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    red[(size_t)((wave - 2) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
-    }
-    __syncthreads();
-    if (wave < 2) {
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-            for (int q = 0; q < CBG; q++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    float* r = red + (size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane;
-                    *r = acc[mb][q][v] + *r;
-                }
-    }
-    __syncthreads();
     float* out = ws + (size_t)s * K * Ncols;
+    if constexpr (NW == 4) {
+        // (w0 + w2) + (w1 + w3): waves 2 and 3 park their sums, waves 0 and 1
+        // add them and park the results, then wave w writes element v = w of
+        // every accumulator.
+        if (wave >= 2) {
 #pragma unroll
-    for (int t = 0; t < MB * CBG; t++) {
-        const int mb = t / CBG, q = t - mb * CBG;
-        const int e = 4 * t + wave;
-        const float val = red[(size_t)e * 64 + lane] + red[(size_t)(E + e) * 64 + lane];
-        const int j = colBase + q * 16 + l16;
-        const int k = mb * 16 + 4 * g4 + wave;
-        if (j < Ncols && k < K) out[(size_t)k * Ncols + j] = val;
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int q = 0; q < CBG; q++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        red[(size_t)((wave - 2) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
+        }
+        __syncthreads();
+        if (wave < 2) {
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int q = 0; q < CBG; q++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        float* r = red + (size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane;
+                        *r = acc[mb][q][v] + *r;
+                    }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < MB * CBG; t++) {
+            const int mb = t / CBG, q = t - mb * CBG;
+            const int e = 4 * t + wave;
+            const float val = red[(size_t)e * 64 + lane] + red[(size_t)(E + e) * 64 + lane];
+            const int j = colBase + q * 16 + l16;
+            const int k = mb * 16 + 4 * g4 + wave;
+            if (j < Ncols && k < K) out[(size_t)k * Ncols + j] = val;
+        }
+    } else {
+        // ((w0 + w4) + (w2 + w6)) + ((w1 + w5) + (w3 + w7)): waves 4..7 park
+        // their sums in slots 0..3, waves 0..3 add theirs into slot w, then
+        // the eight waves write accumulator elements e = w, w + 8, ...
+        if (wave >= 4) {
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int q = 0; q < CBG; q++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        red[(size_t)((wave - 4) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
+        }
+        __syncthreads();
+        if (wave < 4) {
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int q = 0; q < CBG; q++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        float* r = red + (size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane;
+                        *r = acc[mb][q][v] + *r;
+                    }
+        }
+        __syncthreads();
+        for (int e = wave; e < E; e += NW) {
+            const int t = e >> 2, v = e & 3;
+            const int mb = t / CBG, q = t - mb * CBG;
+            const float* r = red + (size_t)e * 64 + lane;
+            const float val = (r[0] + r[(size_t)2 * E * 64]) + (r[(size_t)E * 64] + r[(size_t)3 * E * 64]);
+            const int j = colBase + q * 16 + l16;
+            const int k = mb * 16 + 4 * g4 + v;
+            if (j < Ncols && k < K) out[(size_t)k * Ncols + j] = val;
+        }
     }
 }
 
@@ -300,14 +343,22 @@ hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long 
     return hipGetLastError();
 }
 
+template <int MB, int NW>
+static void launch_mbw(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy, float* ws,
+                       hipStream_t s, dim3 grid) {
+    const dim3 block(64 * NW);
+    switch (c.CBW) {
+    case 3: hipLaunchKernelGGL((k_fa_igemm_f32<MB, 3, NW>), grid, block, c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    case 6: hipLaunchKernelGGL((k_fa_igemm_f32<MB, 6, NW>), grid, block, c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    default: hipLaunchKernelGGL((k_fa_igemm_f32<MB, 9, NW>), grid, block, c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    }
+}
+
 template <int MB>
 static hipError_t launch_mb(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
                             float* ws, hipStream_t s, dim3 grid) {
-    switch (c.CBW) {
-    case 3: hipLaunchKernelGGL((k_fa_igemm_f32<MB, 3>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
-    case 6: hipLaunchKernelGGL((k_fa_igemm_f32<MB, 6>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
-    default: hipLaunchKernelGGL((k_fa_igemm_f32<MB, 9>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
-    }
+    if (c.NW == 8) launch_mbw<MB, 8>(g, m, c, x, dy, ws, s, grid);
+    else launch_mbw<MB, 4>(g, m, c, x, dy, ws, s, grid);
     return hipGetLastError();
 }
 

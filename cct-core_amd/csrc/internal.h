@@ -173,6 +173,7 @@ struct FaCfg {
     int colGroups;  // column groups of 16*CBW columns
     int rowChunks;  // ceil(FHe / RR)
     int splits;     // partial results
+    int NW;         // waves per block (FP32 kernel: 4, or 8 with the tap columns split between wave pairs)
     size_t lds_bytes;
 };
 

@@ -293,7 +293,7 @@ IgemmCfg choose_igemm_frame(const Geom& g, const PlaneMap& m, int mbOnly = 0, in
 // cbw (0 = by size) fixes the column-group width; partials (0 = by size) is
 // the target number of blocks, each of which writes one partial result.
 bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out, int cbw = 0,
-                     long partials = 0, int cw = 0) {
+                     long partials = 0, int cw = 0, int nw = 0) {
     if (d->mode != COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM || d->border != COGCONV_BORDER_VALID || d->in0.dims != 2)
         return false;
     FaCfg c;
@@ -323,7 +323,11 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     c.MTp = c.MB == 1 ? 17 : (c.MB == 4 ? 81 : 49);  // 16*MB padded to 17 mod 32 (k_fa_igemm_f32's MTP)
     if ((long)m.B * m.N * g.H * g.W >= (1L << 31) || (long)m.B * c.K * g.FH * g.FW >= (1L << 31)) return false;
     if ((long)m.N * g.H * g.W * 4 >= (1L << 31) || (long)c.K * g.FH * g.FW * 4 >= (1L << 31)) return false;
-    const size_t red = (size_t)2 * c.MB * c.CBW * 4 * 64 * sizeof(float);  // cross-wave reduction buffer
+    // nw = 0 (heuristic): 4 waves, or 8 when the grid is one block per CU
+    // (decided below, once the split count is known).
+    if (nw != 0 && nw != 4 && (nw != 8 || (c.MB == 4 && c.CBW == 9))) return false;  // <4, 9, 8> would spill
+    c.NW = nw ? nw : 4;
+    const size_t red = (size_t)(c.NW / 2) * c.MB * c.CBW * 4 * 64 * sizeof(float);  // cross-wave reduction buffer
     // Tap rows per item: as many as fit 80 KB (two blocks per CU); but at
     // least 4, one per wave, when the tap rows allow it and 160 KB holds them.
     // x rows pitched at 4 mod 32 words and planes at 16 mod 32: the B-operand
@@ -357,6 +361,14 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     long want = (partials + c.colGroups - 1) / c.colGroups;
     c.splits = (int)std::max(1L, std::min(items, want));
     if (items * (c.splits + 1) >= (1L << 31)) return false;  // the kernel's 32-bit item arithmetic
+    // One block per CU: 8 waves, two per SIMD, so one wave's MFMAs overlap
+    // the other's LDS reads (c2 FilterAdjoint 20.1 -> 18.4 us).
+    FaCfg c8;
+    if (nw == 0 && (long)c.colGroups * c.splits <= 256 && choose_fa_igemm(d, g, m, &c8, cbw, partials, cw, 8) &&
+        c8.splits == c.splits && c8.RR == c.RR && c8.CW == c.CW) {
+        *out = c8;
+        return true;
+    }
     *out = c;
     return true;
 }
@@ -644,11 +656,12 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
     if (fa_ok) {
         for (int cw : {0, 32, 16})
             for (int cbw : {3, 6, 9})
-                for (int parts : {128, 256, 512, 1024}) {
-                    FaCfg c;
-                    if (choose_fa_igemm(desc, g, m, &c, cbw, parts, cw))
-                        add(KK_FA_IGEMM, nullptr, nullptr, nullptr, &c);
-                }
+                for (int parts : {128, 256, 512, 1024})
+                    for (int nw : {4, 8}) {
+                        FaCfg c;
+                        if (choose_fa_igemm(desc, g, m, &c, cbw, parts, cw, nw))
+                            add(KK_FA_IGEMM, nullptr, nullptr, nullptr, &c);
+                    }
     }
     if (small_legal && (long)m.R * g.FHe * g.FWe >= 64) {
         for (long blocks : {2048L, 4096L, 8192L}) {
@@ -882,9 +895,9 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
         break;
     }
     case KK_FA_IGEMM:
-        snprintf(buf, len, "fa_igemm_f32 %s out=%dx%dx%d MB=%d CBW=%d RR=%d CW=%d groups=%d splits=%d lds=%zu",
+        snprintf(buf, len, "fa_igemm_f32 %s out=%dx%dx%d MB=%d CBW=%d RR=%d CW=%d groups=%d splits=%d waves=%d lds=%zu",
                  mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->fcfg.MB, plan->fcfg.CBW, plan->fcfg.RR,
-                 plan->fcfg.CW, plan->fcfg.colGroups, plan->fcfg.splits, plan->fcfg.lds_bytes);
+                 plan->fcfg.CW, plan->fcfg.colGroups, plan->fcfg.splits, plan->fcfg.NW, plan->fcfg.lds_bytes);
         break;
     default:
         snprintf(buf, len, "igemm_f32 %s out=%dx%dx%d MB=%d RC=%d tile=8x32 threads=512 lds=%zu", mode_name(plan->desc.mode),

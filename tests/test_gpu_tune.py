@@ -37,6 +37,11 @@ CASES = [
     (O.BPF_BRS, O.FULL, O.CONV, O.NONE, 1, 2, (2 * 3, 14, 18), (3 * 5, 3, 3)),
     (O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, 3, (3 * 4, 20, 22), (3 * 5, 18, 20)),
     (O.FA_BRS, O.VALID, O.CONV, O.UP, 2, 2, (2 * 3, 21, 23), (2 * 2, 9, 10)),
+    # FilterAdjoint with 2, 3 and 4 row blocks (K 32, 40, 64): the 4- and
+    # 8-wave kernels, one- and two-pass dy rows (68 tap columns), down-sampling.
+    (O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, 2, (2 * 16, 34, 36), (2 * 32, 32, 34)),
+    (O.FA_BRS, O.VALID, O.CONV, O.DOWN, 2, 2, (2 * 6, 34, 38), (2 * 40, 15, 17)),
+    (O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, 1, (1 * 3, 90, 70), (1 * 64, 85, 66)),
     (O.PBP, O.CLAMP, O.CONV, O.NONE, 1, 1, (2, 40, 44), (2, 7, 7)),
 ]
 
