@@ -18,8 +18,9 @@
 // plus the x rows they touch, for the input planes of its column group; staging
 // runs one row per wave pass with lanes across columns.  Wave w reduces tap
 // rows w%4, w%4+4, .. for every tile of the group; with 8 waves, waves w and
-// w+4 (one SIMD) take the two halves of the tap columns, so one wave's MFMAs
-// run while the other waits on LDS.  The waves' sums meet in LDS.  Blocks split the (image, tap-row chunk) space; each writes one partial
+// w+4 (one SIMD) take the two halves of the row blocks (even MB) or of the tap
+// columns (odd MB), so one wave's MFMAs run while the other waits on LDS.  The
+// waves' sums meet in LDS.  Blocks split the (image, tap-row chunk) space; each writes one partial
 // dW, and the partials are added in a fixed order by a two-level reduction, so
 // results do not depend on scheduling.
 #include "stage.h"
@@ -88,9 +89,16 @@ __global__ __launch_bounds__(64 * NW) void k_fa_igemm_f32(Geom g, PlaneMap m, Fa
 #pragma unroll
     for (int q = 0; q < CBG; q++) xoff[q] = reinterpret_cast<const int*>(smem)[q * 16 + l16] + g4;
 
-    floatx4 acc[MB][CBG];
+    // With 8 waves and MB = 4, the two wave quads (waves 0-3, 4-7; one wave
+    // of each per SIMD) own halves of the row blocks; otherwise they take the
+    // two halves of the tap columns (one row block per wave would double the
+    // B-operand LDS reads per MFMA).
+    constexpr bool MSPLIT = NW == 8 && MB == 4;
+    constexpr int MW = MSPLIT ? MB / 2 : MB;  // row blocks a wave accumulates
+    const int mb0 = MSPLIT ? (wave >> 2) * MW : 0;
+    floatx4 acc[MW][CBG];
 #pragma unroll
-    for (int i = 0; i < MB; i++)
+    for (int i = 0; i < MW; i++)
 #pragma unroll
         for (int q = 0; q < CBG; q++) acc[i][q] = floatx4{0.f, 0.f, 0.f, 0.f};
 
@@ -99,203 +107,228 @@ __global__ __launch_bounds__(64 * NW) void k_fa_igemm_f32(Geom g, PlaneMap m, Fa
     const int perImage = c.colChunks * c.rowChunks;
     const int items = m.B * perImage;
     const int it0 = items * s / c.splits, it1 = items * (s + 1) / c.splits;
-    for (int it = it0; it < it1; it++) {
-        const int b = it / perImage;
+    constexpr int UD = 64 * MB / NW, UX = 96 / NW;  // UD: a wave's dy rows when RR = 4
+    constexpr int MT = 16 * MB;                     // dy planes the MFMAs read (K padded with zeros)
+    const int xRowsAll = nCnt * c.xRows;            // staged x rows (plane, row)
+
+    // The current item: image, first tap row, tap-row count.  set_item also
+    // sets the per-lane column offsets when the tap-column chunk changes.
+    int b = 0, i0 = 0, rr = 0;
+    auto set_item = [&](int it) __attribute__((always_inline)) {
+        b = it / perImage;
         const int rest = it - b * perImage;
         const int cc = rest / c.rowChunks;
-        const int i0 = (rest - cc * c.rowChunks) * c.RR;
-        const int rr = min(c.RR, g.FHe - i0);
+        i0 = (rest - cc * c.rowChunks) * c.RR;
+        rr = min(c.RR, g.FHe - i0);
         if (cc != jChunk) {  // wave-uniform
             jChunk = cc;
             const int j0 = cc * c.CW;
             const int jEnd = min(j0 + c.CW, g.FWe);
-            int dc[2], xc[2];
-            bool di[2], xi[2];
-#pragma unroll
-            for (int p = 0; p < 2; p++) {
-                const int jj = j0 + lane + 64 * p;
+            auto col = [&](int jj, int& dc, bool& di, int& xc, bool& xi) {
                 int tj = g.flip ? g.FWe - 1 - jj : jj;
                 bool on = jj < jEnd;
                 if (g.upFc > 1) { on = on && (tj % g.upFc) == 0; tj /= g.upFc; }
-                dc[p] = min(max(tj, 0), g.FW - 1);
-                di[p] = on;
-                xc[p] = min(jj, g.W - 1);
-                xi[p] = jj < g.W;
-            }
-            dCol0 = dc[0]; dCol1 = dc[1]; dIn0 = di[0]; dIn1 = di[1];
-            xCol0 = xc[0]; xCol1 = xc[1]; xIn0 = xi[0]; xIn1 = xi[1];
+                dc = min(max(tj, 0), g.FW - 1);
+                di = on;
+                xc = min(jj, g.W - 1);
+                xi = jj < g.W;
+            };
+            col(j0 + lane, dCol0, dIn0, xCol0, xIn0);
+            col(j0 + lane + 64, dCol1, dIn1, xCol1, xIn1);
         }
-        __syncthreads();
-        // Staging of the dy tap rows and x rows: one row per wave pass, lanes
-        // across columns (at most 128).  Per batch of UD dy rows and UX x rows
-        // (rows w, w+NW, ... of each set): the rows' offsets and masks (per-lane
-        // tables), then all loads of both sets, from
-        // in-bounds offsets or the out-of-range offset, with no branch between
-        // them, then the LDS stores; rows past the end and lanes past the row
-        // width store into a scratch slot instead of branching.  One batch
-        // covers a block's rows for the common shapes, so staging costs one
-        // memory round trip.
-        constexpr int UD = 64 * MB / NW, UX = 96 / NW;  // UD: a wave's dy rows when RR = 4
-        constexpr int MT = 16 * MB;    // dy planes the MFMAs read (K padded with zeros)
-        const int dRows = MT * rr;     // (il, k); the pitch padding beyond MT is never read
+    };
+
+    // Staging of the current item's dy tap rows and x rows: one row per wave
+    // pass, lanes across columns (at most 128).  Per batch of UD dy rows and
+    // UX x rows (rows w, w+NW, ... of each set): the rows' offsets and masks
+    // (per-lane tables), then all loads of both sets, from in-bounds offsets
+    // or the out-of-range offset, with no branch between them, then the LDS
+    // stores; rows past the end and lanes past the row width store into a
+    // scratch slot instead of branching.  One batch covers a block's rows for
+    // the common shapes, so staging costs one memory round trip.
+    int dDst = -1, xDst = -1;
+    auto load_batch = [&](int bd, int bx, float(&dv0)[UD], float(&dv1)[UD], float(&xv0)[UX],
+                          float(&xv1)[UX]) __attribute__((always_inline)) {
+        const int dRows = MT * rr;  // (il, k); the pitch padding beyond MT is never read
         const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * FHW, (uint32_t)K * FHW * 4u);
         const int vd0 = dIn0 ? 4 * dCol0 : STAGE_OOB, vd1 = dIn1 ? 4 * dCol1 : STAGE_OOB;
-        const int xRowsAll = nCnt * c.xRows;  // (plane, row)
         const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + ((size_t)b * N + n_lo) * HWx, (uint32_t)nCnt * HWx * 4u);
         const int vx0 = xIn0 ? 4 * xCol0 : STAGE_OOB, vx1 = xIn1 ? 4 * xCol1 : STAGE_OOB;
-        for (int bd = wave, bx = wave; bd < dRows || bx < xRowsAll; bd += NW * UD, bx += NW * UX) {
-            // Per-lane row tables of the batch (lane u = the wave's u-th dy row
-            // and u-th x row): byte offsets (flip, up-sampling lattice and
-            // clamps applied), LDS destinations (-1: not live) and in-field
-            // masks, computed in parallel across lanes and read back per row
-            // with readlane / a ballot bit.
-            int dSoff, dDst, xSoff, xDst;
-            uint64_t dMask, xMask;
-            {
-                const int row = bd + NW * lane;
-                const int il = row / MT, k = row - il * MT;
-                int ti = i0 + il;
-                if (g.flip) ti = g.FHe - 1 - ti;
-                bool on = lane < UD && row < dRows && k < K && il < rr;
-                if (g.upFr > 1) {
-                    on = on && (ti % g.upFr) == 0;
-                    ti /= g.upFr;
-                }
-                dSoff = 4 * (min(k, K - 1) * FHW + min(max(ti, 0), g.FH - 1) * g.FW);
-                dDst = (lane < UD && row < dRows) ? il * c.DWp * MTP + k : -1;
-                dMask = __ballot(on);
+        // Per-lane row tables of the batch (lane u = the wave's u-th dy row
+        // and u-th x row): byte offsets (flip, up-sampling lattice and clamps
+        // applied), LDS destinations (-1: not live) and in-field masks,
+        // computed in parallel across lanes and read back per row with
+        // readlane / a ballot bit.
+        int dSoff, xSoff;
+        uint64_t dMask, xMask;
+        {
+            const int row = bd + NW * lane;
+            const int il = row / MT, k = row - il * MT;
+            int ti = i0 + il;
+            if (g.flip) ti = g.FHe - 1 - ti;
+            bool on = lane < UD && row < dRows && k < K && il < rr;
+            if (g.upFr > 1) {
+                on = on && (ti % g.upFr) == 0;
+                ti /= g.upFr;
             }
-            {
-                const int row = bx + NW * lane;
-                const int nl = row / c.xRows, lr = row - nl * c.xRows;
-                const int gy = i0 + lr;
-                const bool live = lane < UX && row < xRowsAll;
-                xSoff = 4 * (min(nl, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W);
-                xDst = live ? nl * xPlane + lr * c.xColsP : -1;
-                xMask = __ballot(live && gy < g.H);
-            }
-            auto drow = [&](int u, int& soff, bool& on) {
-                soff = __builtin_amdgcn_readlane(dSoff, u);
-                on = (dMask >> u) & 1;
-            };
-            auto ddst = [&](int u) { return __builtin_amdgcn_readlane(dDst, u); };
-            auto xrow = [&](int u, int& soff, bool& on) {
-                soff = __builtin_amdgcn_readlane(xSoff, u);
-                on = (xMask >> u) & 1;
-            };
-            auto xdst = [&](int u) { return __builtin_amdgcn_readlane(xDst, u); };
-            float dv0[UD], dv1[UD], xv0[UX], xv1[UX];
-            if (dTwo) load_rows_fn<true, UD>(dyr, drow, vd0, vd1, dv0, dv1);
-            else load_rows_fn<false, UD>(dyr, drow, vd0, vd1, dv0, dv1);
-            if (xTwo) load_rows_fn<true, UX>(xr, xrow, vx0, vx1, xv0, xv1);
-            else load_rows_fn<false, UX>(xr, xrow, vx0, vx1, xv0, xv1);
-            __builtin_amdgcn_sched_barrier(0);  // every load of the batch ahead of the first store
-            if (dTwo) store_rows_fn<true, UD>(ddst, sD, sJunk, lane, c.DWp, MTP, dv0, dv1);
-            else store_rows_fn<false, UD>(ddst, sD, sJunk, lane, c.DWp, MTP, dv0, dv1);
-            if (xTwo) store_rows_fn<true, UX>(xdst, sX, sJunk, lane, c.xCols, 1, xv0, xv1);
-            else store_rows_fn<false, UX>(xdst, sX, sJunk, lane, c.xCols, 1, xv0, xv1);
+            dSoff = 4 * (min(k, K - 1) * FHW + min(max(ti, 0), g.FH - 1) * g.FW);
+            dDst = (lane < UD && row < dRows) ? il * c.DWp * MTP + k : -1;
+            dMask = __ballot(on);
         }
-        __syncthreads();
-        // Wave w reduces tap rows w%4, w%4+4, ... of the staged chunk (tap
-        // columns [jb0, jb1): all of them, or half with 8 waves) for every
-        // output tile of the block's column group.
-        constexpr int CP = NW / 4;
-        const int jq = (c.DWp / 4 + CP - 1) / CP * 4;
-        const int jb0 = (wave / 4) * jq, jb1 = min(c.DWp, jb0 + jq);
-        for (int il = wave & 3; il < rr; il += 4) {
-            const float* dRow = sD + ((size_t)il * c.DWp + g4) * MTp + l16;
-            const float* xRow = sX + il * c.xColsP;
+        {
+            const int row = bx + NW * lane;
+            const int nl = row / c.xRows, lr = row - nl * c.xRows;
+            const int gy = i0 + lr;
+            const bool live = lane < UX && row < xRowsAll;
+            xSoff = 4 * (min(nl, nCnt - 1) * HWx + min(gy, g.H - 1) * g.W);
+            xDst = live ? nl * xPlane + lr * c.xColsP : -1;
+            xMask = __ballot(live && gy < g.H);
+        }
+        auto drow = [&](int u, int& soff, bool& on) {
+            soff = __builtin_amdgcn_readlane(dSoff, u);
+            on = (dMask >> u) & 1;
+        };
+        auto xrow = [&](int u, int& soff, bool& on) {
+            soff = __builtin_amdgcn_readlane(xSoff, u);
+            on = (xMask >> u) & 1;
+        };
+        if (dTwo) load_rows_fn<true, UD>(dyr, drow, vd0, vd1, dv0, dv1);
+        else load_rows_fn<false, UD>(dyr, drow, vd0, vd1, dv0, dv1);
+        if (xTwo) load_rows_fn<true, UX>(xr, xrow, vx0, vx1, xv0, xv1);
+        else load_rows_fn<false, UX>(xr, xrow, vx0, vx1, xv0, xv1);
+    };
+    auto store_batch = [&](float* bufD, float* bufX, const float(&dv0)[UD], const float(&dv1)[UD],
+                           const float(&xv0)[UX], const float(&xv1)[UX]) __attribute__((always_inline)) {
+        auto ddst = [&](int u) { return __builtin_amdgcn_readlane(dDst, u); };
+        auto xdst = [&](int u) { return __builtin_amdgcn_readlane(xDst, u); };
+        if (dTwo) store_rows_fn<true, UD>(ddst, bufD, sJunk, lane, c.DWp, MTP, dv0, dv1);
+        else store_rows_fn<false, UD>(ddst, bufD, sJunk, lane, c.DWp, MTP, dv0, dv1);
+        if (xTwo) store_rows_fn<true, UX>(xdst, bufX, sJunk, lane, c.xCols, 1, xv0, xv1);
+        else store_rows_fn<false, UX>(xdst, bufX, sJunk, lane, c.xCols, 1, xv0, xv1);
+    };
+
+    // MFMAs of a staged item with rrI tap rows.  The waves sharing a set of
+    // row blocks (all NW, or a quad with MSPLIT) form RW tap-row lanes (RW =
+    // 4, 2 or 1 as RR allows) times column parts: wave (il0, part) reduces
+    // tap rows il0, il0 + RW, ... over tap columns [jb0, jb1).
+    const int RW = c.RR >= 4 ? 4 : (c.RR >= 2 ? 2 : 1);
+    const int lgRW = RW == 4 ? 2 : (RW == 2 ? 1 : 0);
+    constexpr int QW = MSPLIT ? 4 : NW;
+    const int wq = MSPLIT ? (wave & 3) : wave;
+    const int parts = QW >> lgRW;
+    const int il0 = wq & (RW - 1), part = wq >> lgRW;
+    const int jq = (c.DWp / 4 + parts - 1) / parts * 4;
+    const int jb0 = part * jq, jb1 = min(c.DWp, jb0 + jq);
+    auto mfma_item = [&](const float* bufD, const float* bufX, int rrI) __attribute__((always_inline)) {
+        for (int il = il0; il < rrI; il += RW) {
+            const float* dRow = bufD + ((size_t)il * c.DWp + g4) * MTp + l16;
+            const float* xRow = bufX + il * c.xColsP;
             for (int jb = jb0; jb < jb1; jb += 4) {
-                float av[MB];
+                float av[MW];
 #pragma unroll
-                for (int mb = 0; mb < MB; mb++) av[mb] = dRow[(size_t)jb * MTp + mb * 16];
+                for (int mb = 0; mb < MW; mb++) av[mb] = dRow[(size_t)jb * MTp + (mb0 + mb) * 16];
                 float bv[CBG];
 #pragma unroll
                 for (int q = 0; q < CBG; q++) bv[q] = xRow[xoff[q] + jb];
 #pragma unroll
-                for (int mb = 0; mb < MB; mb++)
+                for (int mb = 0; mb < MW; mb++)
 #pragma unroll
                     for (int q = 0; q < CBG; q++)
                         acc[mb][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[mb], bv[q], acc[mb][q], 0, 0, 0);
             }
         }
+    };
+
+    for (int it = it0; it < it1; it++) {
+        set_item(it);
+        __syncthreads();  // the previous item's MFMAs are done with the staging area
+        const int dRows = MT * rr;
+        for (int bd = wave, bx = wave; bd < dRows || bx < xRowsAll; bd += NW * UD, bx += NW * UX) {
+            float dv0[UD], dv1[UD], xv0[UX], xv1[UX];
+            load_batch(bd, bx, dv0, dv1, xv0, xv1);
+            __builtin_amdgcn_sched_barrier(0);  // every load of the batch ahead of the first store
+            store_batch(sD, sX, dv0, dv1, xv0, xv1);
+        }
+        __syncthreads();
+        mfma_item(sD, sX, rr);
     }
     // The waves' sums meet in LDS in a fixed order and every wave then writes
     // a share of the block's partial dW, ws[s][k][j].
     // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
-    constexpr int E = MB * CBG * 4;  // accumulator floats per lane
+    constexpr int E = MW * CBG * 4;  // accumulator floats per lane
     __syncthreads();
     float* red = smem;
     float* out = ws + (size_t)s * K * Ncols;
-    if constexpr (NW == 4) {
-        // (w0 + w2) + (w1 + w3): waves 2 and 3 park their sums, waves 0 and 1
-        // add them and park the results, then wave w writes element v = w of
-        // every accumulator.
-        if (wave >= 2) {
+    if constexpr (NW == 4 || MSPLIT) {
+        // Per wave quad, over its waves r = wave % 4, (r0 + r2) + (r1 + r3):
+        // waves 2 and 3 park their sums, waves 0 and 1 add them and park the
+        // results, then wave r writes element v = r of every accumulator.
+        const int r = wave & 3;
+        float* rq = red + (size_t)(wave >> 2) * 2 * E * 64;
+        if (r >= 2) {
 #pragma unroll
-            for (int mb = 0; mb < MB; mb++)
+            for (int mb = 0; mb < MW; mb++)
 #pragma unroll
                 for (int q = 0; q < CBG; q++)
 #pragma unroll
                     for (int v = 0; v < 4; v++)
-                        red[(size_t)((wave - 2) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
+                        rq[(size_t)((r - 2) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
         }
         __syncthreads();
-        if (wave < 2) {
+        if (r < 2) {
 #pragma unroll
-            for (int mb = 0; mb < MB; mb++)
+            for (int mb = 0; mb < MW; mb++)
 #pragma unroll
                 for (int q = 0; q < CBG; q++)
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
-                        float* r = red + (size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane;
-                        *r = acc[mb][q][v] + *r;
+                        float* p = rq + (size_t)(r * E + (mb * CBG + q) * 4 + v) * 64 + lane;
+                        *p = acc[mb][q][v] + *p;
                     }
         }
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < MB * CBG; t++) {
+        for (int t = 0; t < MW * CBG; t++) {
             const int mb = t / CBG, q = t - mb * CBG;
-            const int e = 4 * t + wave;
-            const float val = red[(size_t)e * 64 + lane] + red[(size_t)(E + e) * 64 + lane];
+            const int e = 4 * t + r;
+            const float val = rq[(size_t)e * 64 + lane] + rq[(size_t)(E + e) * 64 + lane];
             const int j = colBase + q * 16 + l16;
-            const int k = mb * 16 + 4 * g4 + wave;
+            const int k = (mb0 + mb) * 16 + 4 * g4 + r;
             if (j < Ncols && k < K) out[(size_t)k * Ncols + j] = val;
         }
     } else {
         // ((w0 + w4) + (w2 + w6)) + ((w1 + w5) + (w3 + w7)): waves 4..7 park
         // their sums in slots 0..3, waves 0..3 add theirs into slot w, then
-        // the eight waves write accumulator elements e = w, w + 8, ...
+        // the eight waves write tiles t = w, w + 8, ...  An accumulator is
+        // parked whole (slot, tile, lane: 16 bytes a lane), so every LDS
+        // access is one 16-byte access a lane.
+        constexpr int T = MW * CBG;
+        floatx4* red4 = reinterpret_cast<floatx4*>(smem);
         if (wave >= 4) {
 #pragma unroll
-            for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-                for (int q = 0; q < CBG; q++)
-#pragma unroll
-                    for (int v = 0; v < 4; v++)
-                        red[(size_t)((wave - 4) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
+            for (int t = 0; t < T; t++) red4[(size_t)((wave - 4) * T + t) * 64 + lane] = acc[t / CBG][t % CBG];
         }
         __syncthreads();
         if (wave < 4) {
 #pragma unroll
-            for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-                for (int q = 0; q < CBG; q++)
-#pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        float* r = red + (size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane;
-                        *r = acc[mb][q][v] + *r;
-                    }
+            for (int t = 0; t < T; t++) {
+                floatx4* p = red4 + (size_t)(wave * T + t) * 64 + lane;
+                *p = acc[t / CBG][t % CBG] + *p;
+            }
         }
         __syncthreads();
-        for (int e = wave; e < E; e += NW) {
-            const int t = e >> 2, v = e & 3;
+        for (int t = wave; t < T; t += NW) {
+            const floatx4* p = red4 + (size_t)t * 64 + lane;
+            const floatx4 val = (p[0] + p[(size_t)2 * T * 64]) + (p[(size_t)T * 64] + p[(size_t)3 * T * 64]);
             const int mb = t / CBG, q = t - mb * CBG;
-            const float* r = red + (size_t)e * 64 + lane;
-            const float val = (r[0] + r[(size_t)2 * E * 64]) + (r[(size_t)E * 64] + r[(size_t)3 * E * 64]);
             const int j = colBase + q * 16 + l16;
-            const int k = mb * 16 + 4 * g4 + v;
-            if (j < Ncols && k < K) out[(size_t)k * Ncols + j] = val;
+            if (j < Ncols) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int k = mb * 16 + 4 * g4 + v;
+                    if (k < K) out[(size_t)k * Ncols + j] = val[v];
+                }
+            }
         }
     }
 }

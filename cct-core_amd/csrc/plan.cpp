@@ -325,9 +325,11 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     if ((long)m.N * g.H * g.W * 4 >= (1L << 31) || (long)c.K * g.FH * g.FW * 4 >= (1L << 31)) return false;
     // nw = 0 (heuristic): 4 waves, or 8 when the grid is one block per CU
     // (decided below, once the split count is known).
-    if (nw != 0 && nw != 4 && (nw != 8 || (c.MB == 4 && c.CBW == 9))) return false;  // <4, 9, 8> would spill
+    if (nw != 0 && nw != 4 && nw != 8) return false;
     c.NW = nw ? nw : 4;
-    const size_t red = (size_t)(c.NW / 2) * c.MB * c.CBW * 4 * 64 * sizeof(float);  // cross-wave reduction buffer
+    // Cross-wave reduction buffer: two parked sums per wave quad (8 waves with
+    // MB < 4: four, each of all MB row blocks).
+    const size_t red = (size_t)(c.NW == 8 && c.MB != 4 ? 4 : 2) * c.MB * c.CBW * 4 * 64 * sizeof(float);
     // Tap rows per item: as many as fit 80 KB (two blocks per CU); but at
     // least 4, one per wave, when the tap rows allow it and 160 KB holds them.
     // x rows pitched at 4 mod 32 words and planes at 16 mod 32: the B-operand
@@ -358,13 +360,13 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     c.rowChunks = (g.FHe + c.RR - 1) / c.RR;
     const long items = (long)m.B * c.colChunks * c.rowChunks;
     if (partials <= 0) partials = items * c.colGroups >= 8192 ? 512 : 256;  // blocks: 1-2 per CU
-    long want = (partials + c.colGroups - 1) / c.colGroups;
+    const long want = (partials + c.colGroups - 1) / c.colGroups;
     c.splits = (int)std::max(1L, std::min(items, want));
     if (items * (c.splits + 1) >= (1L << 31)) return false;  // the kernel's 32-bit item arithmetic
-    // One block per CU: 8 waves, two per SIMD, so one wave's MFMAs overlap
-    // the other's LDS reads (c2 FilterAdjoint 20.1 -> 18.4 us).
+    // 8 waves, two per SIMD, so one wave's MFMAs overlap the other's LDS
+    // reads (c2 FilterAdjoint 20.1 -> 18.4 us, c3 495 -> 436 us).
     FaCfg c8;
-    if (nw == 0 && (long)c.colGroups * c.splits <= 256 && choose_fa_igemm(d, g, m, &c8, cbw, partials, cw, 8) &&
+    if (nw == 0 && choose_fa_igemm(d, g, m, &c8, cbw, partials, cw, 8) &&
         c8.splits == c.splits && c8.RR == c.RR && c8.CW == c.CW) {
         *out = c8;
         return true;

@@ -89,6 +89,11 @@ __device__ __forceinline__ void stage_rows_fn(__amdgpu_buffer_rsrc_t src, RowFn 
 // IU rows (row(u) as above) into v0 / v1, store_rows_fn stores them (dst(u)
 // as above).  A __builtin_amdgcn_sched_barrier(0) between the last load and
 // the first store keeps the loads together.
+//
+// The second pass's load of a row comes first, so the one- and two-pass forms
+// both end with a load into v0: a caller that picks the form at run time then
+// joins on one array element (a join on v0 / v1 elements would keep the
+// arrays in scratch memory).
 template <bool TWO, int IU, class RowFn>
 __device__ __forceinline__ void load_rows_fn(__amdgpu_buffer_rsrc_t src, RowFn row, int vcol0, int vcol1,
                                              float (&v0)[IU], float (&v1)[IU]) {
@@ -97,8 +102,8 @@ __device__ __forceinline__ void load_rows_fn(__amdgpu_buffer_rsrc_t src, RowFn r
         int soff;
         bool on;
         row(u, soff, on);
-        v0[u] = buffer_load_f32(src, on ? vcol0 : STAGE_OOB, soff);
         if (TWO) v1[u] = buffer_load_f32(src, on ? vcol1 : STAGE_OOB, soff);
+        v0[u] = buffer_load_f32(src, on ? vcol0 : STAGE_OOB, soff);
     }
 }
 
