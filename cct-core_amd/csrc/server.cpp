@@ -342,13 +342,22 @@ struct cogconv_server_s {
 
     void run() {
         (void)hipSetDevice(cfg.device);
+        std::vector<char> busy;  // per stream: an earlier launch on it is not done
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            // Completions.
+            // Completions.  A stream runs its launches in issue order, so once
+            // one is not done the later ones on that stream are skipped: a
+            // poll costs one event query per busy stream, not one per launch.
+            busy.assign(streams.size(), 0);
             for (size_t i = 0; i < inflight.size();) {
                 Launch* L = inflight[i];
+                if (busy[L->stream]) {
+                    i++;
+                    continue;
+                }
                 const hipError_t q = hipEventQuery(L->ev);
                 if (q == hipErrorNotReady) {
+                    busy[L->stream] = 1;
                     i++;
                     continue;
                 }
@@ -448,7 +457,7 @@ extern "C" int cogconv_server_create(const cogconv_server_config* config, cogcon
         }
         s->streams.push_back(st);
     }
-    const int nw = config->streams >= 2 ? 2 : 1;
+    const int nw = std::min(config->streams, 3);
     for (int i = 0; i < nw; i++) s->workers.emplace_back([s] { s->run(); });
     *server = s;
     return COGCONV_OK;
