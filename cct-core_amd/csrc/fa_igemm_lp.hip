@@ -14,11 +14,13 @@
 // reads: dy rows are staged [tap row][plane][column]; x rows are staged once
 // per output column C, shifted left by C*dn, so that the x samples of output
 // (n, R, C) under tap column t sit at column t of copy C.  Operands arrive in
-// FP32 and are rounded while staged.  A block (4 wave64) owns a column group
-// of CBG 16-column blocks and all MB row blocks; wave w reduces tap rows w,
-// w+4, .. of each item (image, tap-column chunk, tap-row chunk); the four
-// waves' sums meet in LDS and one partial dW per block is written, added
-// afterwards in a fixed order (launch_reduce_partials).
+// FP32 and are rounded while staged.  A block (NW = 4 wave64, or 8 when MB =
+// 4) owns a column group of CBG 16-column blocks and all MB row blocks; wave w
+// reduces tap rows w%4, w%4+4, .. of each item (image, tap-column chunk,
+// tap-row chunk), and with 8 waves the two wave quads (one wave of each per
+// SIMD) own two row blocks each; the waves' sums meet in LDS and one partial
+// dW per block is written, added afterwards in a fixed order
+// (launch_reduce_partials).
 #include "lp.h"
 #include "stage.h"
 
@@ -30,8 +32,8 @@ hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long 
 // 2^31 bytes), so the buffer unit's range check answers the load with 0.
 constexpr int FA_OOB = (int)0x80000000u;
 
-template <int MB, int CBG, int PREC>
-__global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
+template <int MB, int CBG, int PREC, int NW>
+__global__ __launch_bounds__(64 * NW) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
                                                      const float* __restrict__ DY, float* __restrict__ ws) {
     typedef typename LpType<PREC>::T T;
     typedef typename LpType<PREC>::V V;
@@ -85,9 +87,12 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
 #pragma unroll
     for (int q = 0; q < CBG; q++) xoff[q] = reinterpret_cast<const int*>(fa_lp_smem)[q * 16 + l16] + 8 * g4;
 
-    floatx4 acc[MB][CBG];
+    static_assert(NW == 4 || (NW == 8 && MB == 4), "8 waves split four row blocks between two quads");
+    constexpr int MW = NW == 8 ? MB / 2 : MB;  // row blocks a wave accumulates
+    const int mb0 = NW == 8 ? (wave >> 2) * MW : 0;
+    floatx4 acc[MW][CBG];
 #pragma unroll
-    for (int i = 0; i < MB; i++)
+    for (int i = 0; i < MW; i++)
 #pragma unroll
         for (int q = 0; q < CBG; q++) acc[i][q] = floatx4{0.f, 0.f, 0.f, 0.f};
 
@@ -146,14 +151,14 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
         // every load, then the rounding stores.
         // Passes per batch: 64 / 96 samples in flight per lane (half that for the
         // two-term split, whose rounding doubles the registers a sample holds).
-        constexpr int DUD = SPLIT ? 4 : 8, DUX = SPLIT ? 6 : 12;
+        constexpr int DUD = (SPLIT ? 16 : 32) / NW, DUX = (SPLIT ? 24 : 48) / NW;
         const int dRows = MT * rr;
         const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * FHW, (uint32_t)K * FHW * 4u);
-        for (int pb = wave * RPP; pb < dRows; pb += 4 * RPP * DUD) {
+        for (int pb = wave * RPP; pb < dRows; pb += NW * RPP * DUD) {
             int tSoff, tDst, tKl;
             uint64_t tOk;
             {
-                const int p0 = pb + 4 * RPP * lane;
+                const int p0 = pb + NW * RPP * lane;
                 const int il = p0 / MT, k0 = p0 - il * MT;
                 int ti = i0 + il;
                 if (g.flip) ti = g.FHe - 1 - ti;
@@ -204,10 +209,10 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
         // passes of a batch come from a per-lane table as above.
         const int xPasses = nCnt * c.xRows * QP;
         const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + ((size_t)b * N + n_lo) * HWx, (uint32_t)nCnt * HWx * 4u);
-        for (int tb = wave; tb < xPasses; tb += 4 * DUX) {
+        for (int tb = wave; tb < xPasses; tb += NW * DUX) {
             int tSoff, tDst, tQ;
             {
-                const int t = tb + 4 * lane;
+                const int t = tb + NW * lane;
                 const int u = t / QP, q = t - u * QP;
                 const int nl = u / c.xRows, r = u - nl * c.xRows;
                 tSoff = 4 * (min(nl, nCnt - 1) * HWx + min(i0 + r, g.H - 1) * g.W);
@@ -251,24 +256,24 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
             }
         }
         __syncthreads();
-        for (int il = wave; il < rr; il += 4) {
+        for (int il = wave & 3; il < rr; il += 4) {
             const T* dRow = sD + ((size_t)il * MT + l16) * DP + 8 * g4;
             const T* xRow = sX + (size_t)il * OW * DP;
             for (int jb = 0; jb < c.DWp; jb += 32) {
-                V av[MB], bv[CBG];
+                V av[MW], bv[CBG];
 #pragma unroll
-                for (int mb = 0; mb < MB; mb++) av[mb] = *reinterpret_cast<const V*>(dRow + mb * 16 * DP + jb);
+                for (int mb = 0; mb < MW; mb++) av[mb] = *reinterpret_cast<const V*>(dRow + (mb0 + mb) * 16 * DP + jb);
 #pragma unroll
                 for (int q = 0; q < CBG; q++) bv[q] = *reinterpret_cast<const V*>(xRow + xoff[q] + jb);
                 if (SPLIT) {
-                    V al[MB], bl[CBG];
+                    V al[MW], bl[CBG];
 #pragma unroll
-                    for (int mb = 0; mb < MB; mb++)
-                        al[mb] = *reinterpret_cast<const V*>(dRow + dTerm + mb * 16 * DP + jb);
+                    for (int mb = 0; mb < MW; mb++)
+                        al[mb] = *reinterpret_cast<const V*>(dRow + dTerm + (mb0 + mb) * 16 * DP + jb);
 #pragma unroll
                     for (int q = 0; q < CBG; q++) bl[q] = *reinterpret_cast<const V*>(xRow + xTerm + xoff[q] + jb);
 #pragma unroll
-                    for (int mb = 0; mb < MB; mb++)
+                    for (int mb = 0; mb < MW; mb++)
 #pragma unroll
                         for (int q = 0; q < CBG; q++) {
                             acc[mb][q] = lp_mfma<PREC>(al[mb], bv[q], acc[mb][q]);
@@ -276,50 +281,52 @@ __global__ __launch_bounds__(256) void k_fa_igemm_lp(Geom g, PlaneMap m, FaCfg c
                         }
                 }
 #pragma unroll
-                for (int mb = 0; mb < MB; mb++)
+                for (int mb = 0; mb < MW; mb++)
 #pragma unroll
                     for (int q = 0; q < CBG; q++) acc[mb][q] = lp_mfma<PREC>(av[mb], bv[q], acc[mb][q]);
             }
         }
     }
-    // The four waves' sums meet in LDS in a fixed order, (w0 + w2) + (w1 + w3):
-    // waves 2 and 3 park their sums, waves 0 and 1 add them and park the
-    // results, then every wave writes a quarter of the block's partial dW,
-    // ws[s][k][j] (wave w the accumulator element v = w).
+    // Per wave quad (all four waves, or each quad of an 8-wave block), over
+    // its waves r = wave % 4, the sums meet in LDS in a fixed order,
+    // (r0 + r2) + (r1 + r3): waves 2 and 3 park their sums, waves 0 and 1 add
+    // them and park the results, then wave r writes element v = r of every
+    // accumulator of the block's partial dW, ws[s][k][j].
     // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
-    constexpr int E = MB * CBG * 4;
+    constexpr int E = MW * CBG * 4;
     __syncthreads();
-    float* red = reinterpret_cast<float*>(fa_lp_smem);
-    if (wave >= 2) {
+    const int r = wave & 3;
+    float* red = reinterpret_cast<float*>(fa_lp_smem) + (size_t)(wave >> 2) * 2 * E * 64;
+    if (r >= 2) {
 #pragma unroll
-        for (int mb = 0; mb < MB; mb++)
+        for (int mb = 0; mb < MW; mb++)
 #pragma unroll
             for (int q = 0; q < CBG; q++)
 #pragma unroll
                 for (int v = 0; v < 4; v++)
-                    red[(size_t)((wave - 2) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
+                    red[(size_t)((r - 2) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
     }
     __syncthreads();
-    if (wave < 2) {
+    if (r < 2) {
 #pragma unroll
-        for (int mb = 0; mb < MB; mb++)
+        for (int mb = 0; mb < MW; mb++)
 #pragma unroll
             for (int q = 0; q < CBG; q++)
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
-                    float* r = red + (size_t)(wave * E + (mb * CBG + q) * 4 + v) * 64 + lane;
-                    *r = acc[mb][q][v] + *r;
+                    float* p = red + (size_t)(r * E + (mb * CBG + q) * 4 + v) * 64 + lane;
+                    *p = acc[mb][q][v] + *p;
                 }
     }
     __syncthreads();
     float* out = ws + (size_t)s * K * Ncols;
 #pragma unroll
-    for (int t = 0; t < MB * CBG; t++) {
+    for (int t = 0; t < MW * CBG; t++) {
         const int mb = t / CBG, q = t - mb * CBG;
-        const int e = 4 * t + wave;
+        const int e = 4 * t + r;
         const float val = red[(size_t)e * 64 + lane] + red[(size_t)(E + e) * 64 + lane];
         const int j = colBase + q * 16 + l16;
-        const int k = mb * 16 + 4 * g4 + wave;
+        const int k = (mb0 + mb) * 16 + 4 * g4 + r;
         if (j < Ncols && k < K) out[(size_t)k * Ncols + j] = val;
     }
 }
@@ -333,14 +340,27 @@ size_t fa_igemm_lp_lds_bytes(const FaCfg& c, int OW, int precision) {
     return stage > red ? stage : red;
 }
 
+template <int MB, int PREC, int NW>
+static void fa_lp_launch3(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy, float* ws,
+                          hipStream_t s, dim3 grid) {
+    const dim3 block(64 * NW);
+    switch (c.CBW) {
+    case 3: hipLaunchKernelGGL((k_fa_igemm_lp<MB, 3, PREC, NW>), grid, block, c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    case 6: hipLaunchKernelGGL((k_fa_igemm_lp<MB, 6, PREC, NW>), grid, block, c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    default: hipLaunchKernelGGL((k_fa_igemm_lp<MB, 9, PREC, NW>), grid, block, c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    }
+}
+
 template <int MB, int PREC>
 static hipError_t fa_lp_launch2(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
                                 float* ws, hipStream_t s, dim3 grid) {
-    switch (c.CBW) {
-    case 3: hipLaunchKernelGGL((k_fa_igemm_lp<MB, 3, PREC>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
-    case 6: hipLaunchKernelGGL((k_fa_igemm_lp<MB, 6, PREC>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
-    default: hipLaunchKernelGGL((k_fa_igemm_lp<MB, 9, PREC>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    if constexpr (MB == 4) {
+        if (c.NW == 8) {
+            fa_lp_launch3<MB, PREC, 8>(g, m, c, x, dy, ws, s, grid);
+            return hipGetLastError();
+        }
     }
+    fa_lp_launch3<MB, PREC, 4>(g, m, c, x, dy, ws, s, grid);
     return hipGetLastError();
 }
 

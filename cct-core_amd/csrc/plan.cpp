@@ -378,7 +378,7 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
 // BF16 / FP16 / split-BF16 MFMA split-reduction GEMM for
 // FilterAdjointBlockReduceSum, BorderValid (k_fa_igemm_lp).
 bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out, int cbw = 0,
-                        long partials = 0, int cw = 0) {
+                        long partials = 0, int cw = 0, int nw = 0) {
     if (d->mode != COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM || d->border != COGCONV_BORDER_VALID || d->in0.dims != 2)
         return false;
     FaCfg c;
@@ -386,6 +386,10 @@ bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m,
     c.K = m.T / m.B;
     if (c.K > 64) return false;
     c.MB = (c.K + 15) / 16;
+    // 8 waves (two wave quads owning two row blocks each) for MB = 4, the
+    // default there; 4 waves otherwise.
+    if (nw == 8 && c.MB != 4) return false;
+    c.NW = nw ? nw : (c.MB == 4 ? 8 : 4);
     const int P = g.OH * g.OW;
     const long ncols = (long)m.N * P;
     const long colBlocks = (ncols + 15) / 16;
@@ -393,10 +397,13 @@ bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m,
     c.colGroups = (int)((colBlocks + c.CBW - 1) / c.CBW);
     c.nCntMax = std::min(m.N, (16 * c.CBW + P - 1) / P + 1);
     // Tap columns per chunk: 32, 64 or 128 (whole 32-deep MFMA slices; 8
-    // columns per staging lane).
-    c.CW = g.FWe <= 32 ? 32 : (g.FWe <= 64 ? 64 : 128);
+    // columns per staging lane).  The heuristic stops at 64: 128-column chunks
+    // stage twice the x copies per item (c3 BF16 FilterAdjoint 530 us at 128,
+    // 379 us at 64).
+    const int widest = g.FWe <= 32 ? 32 : (g.FWe <= 64 ? 64 : 128);
+    c.CW = std::min(widest, 64);
     if (cw) {
-        if (cw > c.CW) return false;  // no wider than the heuristic chunk
+        if (cw > widest) return false;  // no wider than the dy rows need
         c.CW = cw;
     }
     c.DWp = c.CW;
@@ -604,13 +611,14 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         FaCfg flp;
         if (choose_fa_igemm_lp(desc, g, m, &flp)) {
             add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &flp);
-            for (int cw : {0, 64, 32})
+            for (int cw : {0, 128, 32})
                 for (int cbw : {3, 6, 9})
-                    for (long parts : {256L, 512L}) {
-                        FaCfg c2;
-                        if (choose_fa_igemm_lp(desc, g, m, &c2, cbw, parts, cw))
-                            add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &c2);
-                    }
+                    for (long parts : {256L, 512L})
+                        for (int nw : {4, 8}) {
+                            FaCfg c2;
+                            if (choose_fa_igemm_lp(desc, g, m, &c2, cbw, parts, cw, nw))
+                                add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &c2);
+                        }
             return;
         }
         IgemmCfg lp;
@@ -890,10 +898,10 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
     }
     case KK_FA_IGEMM_LP: {
         static const char* pn[] = {"fp32", "bf16", "fp16", "split_bf16"};
-        snprintf(buf, len, "fa_igemm_%s %s out=%dx%dx%d MB=%d CBW=%d RR=%d CW=%d groups=%d splits=%d lds=%zu",
+        snprintf(buf, len, "fa_igemm_%s %s out=%dx%dx%d MB=%d CBW=%d RR=%d CW=%d groups=%d splits=%d waves=%d lds=%zu",
                  pn[plan->desc.precision], mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->fcfg.MB,
                  plan->fcfg.CBW, plan->fcfg.RR, plan->fcfg.CW, plan->fcfg.colGroups, plan->fcfg.splits,
-                 plan->fcfg.lds_bytes);
+                 plan->fcfg.NW, plan->fcfg.lds_bytes);
         break;
     }
     case KK_FA_IGEMM:

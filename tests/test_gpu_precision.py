@@ -159,6 +159,30 @@ def test_filter_adjoint_precision_accuracy(precision, orient, sampling, step, xs
         assert err <= TOL[precision], (NAMES[precision], wide, err)
 
 
+@pytest.mark.parametrize("precision", (L.BF16, L.SPLIT_BF16))
+def test_filter_adjoint_precision_every_candidate(precision):
+    """Every candidate configuration of a K = 56 FilterAdjoint (four row blocks,
+    the last one padded): the 4-wave kernel and the 8-wave one whose wave quads
+    own two row blocks each, against the FP64 oracle."""
+    rng = np.random.default_rng(5600 + precision)
+    B, N, H, W, K, dh, dw = 2, 6, 30, 40, 56, 27, 36
+    x = inputs(rng, (B * N, H, W), False)
+    dy = inputs(rng, (B * K, dh, dw), False)
+    od, cd = desc_pair(O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, B, x.shape, dy.shape, precision=precision)
+    ref = O.conv(od, x, dy, threads=16)
+    waves = set()
+    for v in range(L.variant_count(cd)):
+        plan = L.Plan(cd, v)
+        desc = plan.description
+        plan.close()
+        print("CASE fa candidate", NAMES[precision], v, desc, flush=True)
+        waves.add(desc.split("waves=")[1].split()[0])
+        got = run_gpu(cd, x, dy, variant=v)
+        err = normwise(got, ref)
+        assert err <= TOL[precision], (v, desc, err)
+    assert waves == {"4", "8"}
+
+
 def test_filter_adjoint_precision_c3_slice():
     """B32 128x128x64 / dy 126x126x64 on a 4-image slice, BF16 / FP16 / split."""
     rng = np.random.default_rng(34)
