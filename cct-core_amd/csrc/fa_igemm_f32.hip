@@ -334,45 +334,59 @@ __global__ __launch_bounds__(64 * NW) void k_fa_igemm_f32(Geom g, PlaneMap m, Fa
 }
 
 // Sum `splits` partial arrays of `total` floats in a fixed order, in one
-// launch.  A block owns 64 consecutive outputs; its G = 16 waves each add a
-// contiguous run of ceil(splits / 16) partials (loads batched 16 at a time),
-// and the 16 wave sums are then added in wave order.  The order depends on
-// `splits` only, so results do not depend on scheduling.
-constexpr int RED_G = 16;
+// launch.  A block owns OPB (64 or 32) consecutive outputs and 16 waves; with
+// OPB = 32 each wave's two half-waves take different partials, so a block has
+// G = 16 * 64 / OPB partial groups.  Group gi adds a contiguous run of
+// ceil(splits / G) partials (loads batched 16 or 8 at a time), and the G group sums
+// are then added in group order.  The order depends on `splits` and OPB only,
+// so results do not depend on scheduling.  OPB = 32 doubles the blocks when
+// the outputs are few (a c2 weight gradient has 4608).
+constexpr int RED_W = 16;  // waves per reduction block
 
-__global__ __launch_bounds__(64 * RED_G) void k_reduce_partials(const float* __restrict__ ws, float* __restrict__ out,
+template <int OPB>
+__global__ __launch_bounds__(64 * RED_W) void k_reduce_partials(const float* __restrict__ ws, float* __restrict__ out,
                                                                   int splits, long total) {
-    __shared__ float part[RED_G][64];
-    const int lane = threadIdx.x & 63;
-    const int grp = threadIdx.x >> 6;
-    const long idx = (long)blockIdx.x * 64 + lane;
+    constexpr int G = RED_W * 64 / OPB;
+    __shared__ float part[G][OPB];
+    const int o = threadIdx.x % OPB;
+    const int gi = threadIdx.x / OPB;
+    const long idx = (long)blockIdx.x * OPB + o;
     const long safe = idx < total ? idx : total - 1;
-    const int per = (splits + RED_G - 1) / RED_G;
-    const int s0 = grp * per;
+    const int per = (splits + G - 1) / G;
+    const int s0 = gi * per;
     const int s1 = min(splits, s0 + per);
+    constexpr int LB = OPB == 32 ? 8 : 16;  // loads in flight per lane
     float acc = 0.f;
-    for (int sb = s0; sb < s1; sb += 16) {
-        float v[16];
+    for (int sb = s0; sb < s1; sb += LB) {
+        float v[LB];
 #pragma unroll
-        for (int i = 0; i < 16; i++) v[i] = ws[(size_t)min(sb + i, splits - 1) * total + safe];
+        for (int i = 0; i < LB; i++) v[i] = ws[(size_t)min(sb + i, splits - 1) * total + safe];
 #pragma unroll
-        for (int i = 0; i < 16; i++)
+        for (int i = 0; i < LB; i++)
             if (sb + i < s1) acc += v[i];
     }
-    part[grp][lane] = acc;
+    part[gi][o] = acc;
     __syncthreads();
-    if (grp == 0 && idx < total) {
-        float t = part[0][lane];
+    if (gi == 0 && idx < total) {
+        float t = part[0][o];
 #pragma unroll
-        for (int gi = 1; gi < RED_G; gi++) t += part[gi][lane];
+        for (int k = 1; k < G; k++) t += part[k][o];
         out[idx] = t;
     }
 }
 
 hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long total, hipStream_t s) {
-    const long blocks = (total + 63) / 64;
-    if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)blocks), dim3(64 * RED_G), 0, s, ws, out, splits, total);
+    if (total <= 0) return hipErrorInvalidConfiguration;
+    if (total < 64L * 256) {
+        const long blocks = (total + 31) / 32;
+        hipLaunchKernelGGL(k_reduce_partials<32>, dim3((unsigned)blocks), dim3(64 * RED_W), 0, s, ws, out, splits,
+                           total);
+    } else {
+        const long blocks = (total + 63) / 64;
+        if (blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+        hipLaunchKernelGGL(k_reduce_partials<64>, dim3((unsigned)blocks), dim3(64 * RED_W), 0, s, ws, out, splits,
+                           total);
+    }
     return hipGetLastError();
 }
 
