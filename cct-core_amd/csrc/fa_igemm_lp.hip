@@ -87,9 +87,12 @@ __global__ __launch_bounds__(64 * NW) void k_fa_igemm_lp(Geom g, PlaneMap m, FaC
 #pragma unroll
     for (int q = 0; q < CBG; q++) xoff[q] = reinterpret_cast<const int*>(fa_lp_smem)[q * 16 + l16] + 8 * g4;
 
-    static_assert(NW == 4 || (NW == 8 && MB == 4), "8 waves split four row blocks between two quads");
-    constexpr int MW = NW == 8 ? MB / 2 : MB;  // row blocks a wave accumulates
-    const int mb0 = NW == 8 ? (wave >> 2) * MW : 0;
+    // 8 waves: with MB = 4 the two wave quads own two row blocks each
+    // (MSPLIT); otherwise every wave accumulates all MB row blocks and the
+    // waves split tap rows and 32-column MFMA slices between them.
+    constexpr bool MSPLIT = NW == 8 && MB == 4;
+    constexpr int MW = MSPLIT ? MB / 2 : MB;  // row blocks a wave accumulates
+    const int mb0 = MSPLIT ? (wave >> 2) * MW : 0;
     floatx4 acc[MW][CBG];
 #pragma unroll
     for (int i = 0; i < MW; i++)
@@ -256,10 +259,20 @@ __global__ __launch_bounds__(64 * NW) void k_fa_igemm_lp(Geom g, PlaneMap m, FaC
             }
         }
         __syncthreads();
-        for (int il = wave & 3; il < rr; il += 4) {
+        // Row lanes RW x column parts: 4 x 1 (4 waves, or an MSPLIT quad);
+        // for 8 waves otherwise 4 x 2 when the chunk has two or more 32-column
+        // slices, else 8 x 1.
+        constexpr int QW = MSPLIT ? 4 : NW;
+        const int RW = (QW == 8 && c.DWp < 64) ? 8 : 4;
+        const int wq = MSPLIT ? (wave & 3) : wave;
+        const int parts = QW / RW;
+        const int il0 = wq % RW, part = wq / RW;
+        const int jq = (c.DWp / 32 + parts - 1) / parts * 32;
+        const int jb0 = part * jq, jb1 = min(c.DWp, jb0 + jq);
+        for (int il = il0; il < rr; il += RW) {
             const T* dRow = sD + ((size_t)il * MT + l16) * DP + 8 * g4;
             const T* xRow = sX + (size_t)il * OW * DP;
-            for (int jb = 0; jb < c.DWp; jb += 32) {
+            for (int jb = jb0; jb < jb1; jb += 32) {
                 V av[MW], bv[CBG];
 #pragma unroll
                 for (int mb = 0; mb < MW; mb++) av[mb] = *reinterpret_cast<const V*>(dRow + (mb0 + mb) * 16 * DP + jb);
@@ -287,47 +300,82 @@ __global__ __launch_bounds__(64 * NW) void k_fa_igemm_lp(Geom g, PlaneMap m, FaC
             }
         }
     }
-    // Per wave quad (all four waves, or each quad of an 8-wave block), over
-    // its waves r = wave % 4, the sums meet in LDS in a fixed order,
-    // (r0 + r2) + (r1 + r3): waves 2 and 3 park their sums, waves 0 and 1 add
-    // them and park the results, then wave r writes element v = r of every
-    // accumulator of the block's partial dW, ws[s][k][j].
+    // The waves' sums meet in LDS in a fixed order, then every wave writes a
+    // share of the block's partial dW, ws[s][k][j].
     // D[i][j] of a 16x16 MFMA block lives in lane (j + 16*(i/4)), element i%4.
-    constexpr int E = MW * CBG * 4;
-    __syncthreads();
-    const int r = wave & 3;
-    float* red = reinterpret_cast<float*>(fa_lp_smem) + (size_t)(wave >> 2) * 2 * E * 64;
-    if (r >= 2) {
-#pragma unroll
-        for (int mb = 0; mb < MW; mb++)
-#pragma unroll
-            for (int q = 0; q < CBG; q++)
-#pragma unroll
-                for (int v = 0; v < 4; v++)
-                    red[(size_t)((r - 2) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
-    }
-    __syncthreads();
-    if (r < 2) {
-#pragma unroll
-        for (int mb = 0; mb < MW; mb++)
-#pragma unroll
-            for (int q = 0; q < CBG; q++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    float* p = red + (size_t)(r * E + (mb * CBG + q) * 4 + v) * 64 + lane;
-                    *p = acc[mb][q][v] + *p;
-                }
-    }
     __syncthreads();
     float* out = ws + (size_t)s * K * Ncols;
+    if constexpr (NW == 4 || MSPLIT) {
+        // Per wave quad (all four waves, or each quad of an MSPLIT block), over
+        // its waves r = wave % 4, (r0 + r2) + (r1 + r3): waves 2 and 3 park
+        // their sums, waves 0 and 1 add them and park the results, then wave
+        // r writes element v = r of every accumulator.
+        constexpr int E = MW * CBG * 4;
+        const int r = wave & 3;
+        float* red = reinterpret_cast<float*>(fa_lp_smem) + (size_t)(wave >> 2) * 2 * E * 64;
+        if (r >= 2) {
 #pragma unroll
-    for (int t = 0; t < MW * CBG; t++) {
-        const int mb = t / CBG, q = t - mb * CBG;
-        const int e = 4 * t + r;
-        const float val = red[(size_t)e * 64 + lane] + red[(size_t)(E + e) * 64 + lane];
-        const int j = colBase + q * 16 + l16;
-        const int k = (mb0 + mb) * 16 + 4 * g4 + r;
-        if (j < Ncols && k < K) out[(size_t)k * Ncols + j] = val;
+            for (int mb = 0; mb < MW; mb++)
+#pragma unroll
+                for (int q = 0; q < CBG; q++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        red[(size_t)((r - 2) * E + (mb * CBG + q) * 4 + v) * 64 + lane] = acc[mb][q][v];
+        }
+        __syncthreads();
+        if (r < 2) {
+#pragma unroll
+            for (int mb = 0; mb < MW; mb++)
+#pragma unroll
+                for (int q = 0; q < CBG; q++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        float* p = red + (size_t)(r * E + (mb * CBG + q) * 4 + v) * 64 + lane;
+                        *p = acc[mb][q][v] + *p;
+                    }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < MW * CBG; t++) {
+            const int mb = t / CBG, q = t - mb * CBG;
+            const int e = 4 * t + r;
+            const float val = red[(size_t)e * 64 + lane] + red[(size_t)(E + e) * 64 + lane];
+            const int j = colBase + q * 16 + l16;
+            const int k = (mb0 + mb) * 16 + 4 * g4 + r;
+            if (j < Ncols && k < K) out[(size_t)k * Ncols + j] = val;
+        }
+    } else {
+        // ((w0 + w4) + (w2 + w6)) + ((w1 + w5) + (w3 + w7)): waves 4..7 park
+        // whole accumulators in slots 0..3 (16 bytes a lane), waves 0..3 add
+        // theirs into slot w, then the eight waves write tiles t = w, w + 8, ...
+        constexpr int T4 = MW * CBG;
+        floatx4* red4 = reinterpret_cast<floatx4*>(fa_lp_smem);
+        if (wave >= 4) {
+#pragma unroll
+            for (int t = 0; t < T4; t++) red4[(size_t)((wave - 4) * T4 + t) * 64 + lane] = acc[t / CBG][t % CBG];
+        }
+        __syncthreads();
+        if (wave < 4) {
+#pragma unroll
+            for (int t = 0; t < T4; t++) {
+                floatx4* p = red4 + (size_t)(wave * T4 + t) * 64 + lane;
+                *p = acc[t / CBG][t % CBG] + *p;
+            }
+        }
+        __syncthreads();
+        for (int t = wave; t < T4; t += NW) {
+            const floatx4* p = red4 + (size_t)t * 64 + lane;
+            const floatx4 val = (p[0] + p[(size_t)2 * T4 * 64]) + (p[(size_t)T4 * 64] + p[(size_t)3 * T4 * 64]);
+            const int mb = t / CBG, q = t - mb * CBG;
+            const int j = colBase + q * 16 + l16;
+            if (j < Ncols) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int k = mb * 16 + 4 * g4 + v;
+                    if (k < K) out[(size_t)k * Ncols + j] = val[v];
+                }
+            }
+        }
     }
 }
 
@@ -336,7 +384,9 @@ size_t fa_igemm_lp_lds_bytes(const FaCfg& c, int OW, int precision) {
     const size_t dp = (size_t)c.DWp + 8;
     const size_t stage = (size_t)terms * 2 *
                          ((size_t)c.RR * 16 * c.MB * dp + (size_t)c.nCntMax * c.xRows * OW * dp + 64 * 8);
-    const size_t red = (size_t)2 * c.MB * c.CBW * 4 * 64 * sizeof(float);
+    // Cross-wave sums: two parked sums per wave quad; four (of all MB row
+    // blocks) for 8 waves with MB < 4.
+    const size_t red = (size_t)(c.NW == 8 && c.MB != 4 ? 4 : 2) * c.MB * c.CBW * 4 * 64 * sizeof(float);
     return stage > red ? stage : red;
 }
 
@@ -354,13 +404,8 @@ static void fa_lp_launch3(const Geom& g, const PlaneMap& m, const FaCfg& c, cons
 template <int MB, int PREC>
 static hipError_t fa_lp_launch2(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
                                 float* ws, hipStream_t s, dim3 grid) {
-    if constexpr (MB == 4) {
-        if (c.NW == 8) {
-            fa_lp_launch3<MB, PREC, 8>(g, m, c, x, dy, ws, s, grid);
-            return hipGetLastError();
-        }
-    }
-    fa_lp_launch3<MB, PREC, 4>(g, m, c, x, dy, ws, s, grid);
+    if (c.NW == 8) fa_lp_launch3<MB, PREC, 8>(g, m, c, x, dy, ws, s, grid);
+    else fa_lp_launch3<MB, PREC, 4>(g, m, c, x, dy, ws, s, grid);
     return hipGetLastError();
 }
 

@@ -386,15 +386,21 @@ bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m,
     c.K = m.T / m.B;
     if (c.K > 64) return false;
     c.MB = (c.K + 15) / 16;
-    // 8 waves (two wave quads owning two row blocks each) for MB = 4, the
-    // default there; 4 waves otherwise.
-    if (nw == 8 && c.MB != 4) return false;
-    c.NW = nw ? nw : (c.MB == 4 ? 8 : 4);
+    // 8 waves (the default: two per SIMD; c2 BF16 25.6 -> 23.9 us, c3 414 ->
+    // 379 us) or 4.
+    if (nw != 0 && nw != 4 && nw != 8) return false;
+    c.NW = nw ? nw : 8;
     const int P = g.OH * g.OW;
     const long ncols = (long)m.N * P;
     const long colBlocks = (ncols + 15) / 16;
     c.CBW = cbw ? cbw : (colBlocks <= 3 ? 3 : (colBlocks <= 6 ? 6 : 9));
     c.colGroups = (int)((colBlocks + c.CBW - 1) / c.CBW);
+    // The split-BF16 8-wave kernel with three row blocks and nine column
+    // blocks (all accumulators in every wave) would spill registers.
+    if (c.NW == 8 && d->precision == COGCONV_SPLIT_BF16 && c.MB == 3 && c.CBW == 9) {
+        if (nw == 8) return false;
+        c.NW = 4;
+    }
     c.nCntMax = std::min(m.N, (16 * c.CBW + P - 1) / P + 1);
     // Tap columns per chunk: 32, 64 or 128 (whole 32-deep MFMA slices; 8
     // columns per staging lane).  The heuristic stops at 64: 128-column chunks

@@ -160,12 +160,15 @@ def test_filter_adjoint_precision_accuracy(precision, orient, sampling, step, xs
 
 
 @pytest.mark.parametrize("precision", (L.BF16, L.SPLIT_BF16))
-def test_filter_adjoint_precision_every_candidate(precision):
-    """Every candidate configuration of a K = 56 FilterAdjoint (four row blocks,
-    the last one padded): the 4-wave kernel and the 8-wave one whose wave quads
-    own two row blocks each, against the FP64 oracle."""
-    rng = np.random.default_rng(5600 + precision)
-    B, N, H, W, K, dh, dw = 2, 6, 30, 40, 56, 27, 36
+@pytest.mark.parametrize("K,dw", [(56, 36), (40, 36), (20, 28)])
+def test_filter_adjoint_precision_every_candidate(precision, K, dw):
+    """Every candidate configuration of a FilterAdjoint with four, three and two
+    row blocks (the last one padded) against the FP64 oracle: the 4-wave kernel
+    and the 8-wave ones (wave quads owning two row blocks each for K = 56;
+    otherwise waves splitting tap rows and 32-column slices, 8 x 1 for the
+    one-slice dy rows of dw = 28)."""
+    rng = np.random.default_rng(5600 + precision + K)
+    B, N, H, W, dh = 2, 6, 30, dw + 4, 27
     x = inputs(rng, (B * N, H, W), False)
     dy = inputs(rng, (B * K, dh, dw), False)
     od, cd = desc_pair(O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, B, x.shape, dy.shape, precision=precision)
@@ -180,7 +183,7 @@ def test_filter_adjoint_precision_every_candidate(precision):
         got = run_gpu(cd, x, dy, variant=v)
         err = normwise(got, ref)
         assert err <= TOL[precision], (v, desc, err)
-    assert waves == {"4", "8"}
+    assert waves == {"4", "8"}, waves
 
 
 def test_filter_adjoint_precision_c3_slice():
