@@ -403,11 +403,12 @@ bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m,
     }
     c.nCntMax = std::min(m.N, (16 * c.CBW + P - 1) / P + 1);
     // Tap columns per chunk: 32, 64 or 128 (whole 32-deep MFMA slices; 8
-    // columns per staging lane).  The heuristic stops at 64: 128-column chunks
-    // stage twice the x copies per item (c3 BF16 FilterAdjoint 530 us at 128,
-    // 379 us at 64).
+    // columns per staging lane).  The heuristic stops at 64 (32 for the
+    // two-term split-BF16 staging): wider chunks stage more x copies per item
+    // (c3 BF16 FilterAdjoint 530 us at 128, 379 us at 64; c3 split-BF16
+    // 872 us at 64, 593 us at 32).
     const int widest = g.FWe <= 32 ? 32 : (g.FWe <= 64 ? 64 : 128);
-    c.CW = std::min(widest, 64);
+    c.CW = std::min(widest, d->precision == COGCONV_SPLIT_BF16 ? 32 : 64);
     if (cw) {
         if (cw > widest) return false;  // no wider than the dy rows need
         c.CW = cw;
@@ -617,7 +618,7 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         FaCfg flp;
         if (choose_fa_igemm_lp(desc, g, m, &flp)) {
             add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &flp);
-            for (int cw : {0, 128, 32})
+            for (int cw : {0, 128, 64, 32})
                 for (int cbw : {3, 6, 9})
                     for (long parts : {256L, 512L})
                         for (int nw : {4, 8}) {
