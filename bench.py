@@ -91,7 +91,7 @@ def tuned_plan(L, desc, args):
         if args.tune_cache:
             L.tune_cache_load(args.tune_cache)
         # Warm caches, as in the timed steps (the inputs are reused every step).
-        r = L.autotune(desc, warmup_steps=2, run_steps=20, flush_bytes=0)
+        r = L.autotune(desc, warmup_steps=5, run_steps=60, flush_bytes=0)
         log("tuned:", r["description"], "%.2f us" % r["avg_us"], "(cached)" if r["from_cache"] else
             "(%d candidates)" % r["variants"])
         if args.tune_cache:
