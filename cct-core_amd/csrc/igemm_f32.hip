@@ -115,9 +115,10 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
         // a scratch slot instead of branching.
         const int nrows = c.RC * TRp;
         // Rows a wave stages per batch: 20 (one batch for 16 planes of a
-        // 3x3 tile's 10 rows) where the registers allow two blocks per CU,
-        // 12 for four row blocks.
-        constexpr int IU = MB == 4 ? 12 : 20;
+        // 3x3 tile's 10 rows) where the registers allow two blocks per CU;
+        // 12 for four row blocks and for the up-sampled phase kernels (2x
+        // 7x7 Zero: 700 us at 12, 727 us at 20).
+        constexpr int IU = (MB == 4 || PH) ? 12 : 20;
         const int vc0 = colIn0 ? 4 * colOff0 : STAGE_OOB, vc1 = colIn1 ? 4 * colOff1 : STAGE_OOB;
         for (int base = wave; base < nrows; base += 8 * IU) {
             // Per-lane row table of the batch (lane u = the wave's u-th item):
