@@ -303,6 +303,9 @@ bool choose_fa_igemm(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Fa
     c.MB = (c.K + 15) / 16;
     const int P = g.OH * g.OW;
     const long ncols = (long)m.N * P;
+    // The kernel splits output columns j < N*OH*OW into (plane, row, column)
+    // with fast_div, which is exact below 2^22.
+    if (ncols >= (1L << 22)) return false;
     const long colBlocks = (ncols + 15) / 16;
     c.CBW = cbw ? cbw : (colBlocks <= 3 ? 3 : (colBlocks <= 6 ? 6 : 9));
     c.colGroups = (int)((colBlocks + c.CBW - 1) / c.CBW);

@@ -68,6 +68,23 @@ struct Arena {
         }
         return false;
     }
+    // Last fit from the top of the region: long-lived blocks (packed banks)
+    // gather at the end, away from the request blocks that come and go.
+    bool alloc_high(size_t bytes, size_t* off) {
+        bytes = round(bytes ? bytes : 1);
+        for (auto it = free_.rbegin(); it != free_.rend(); ++it) {
+            if (it->second >= bytes) {
+                const size_t start = it->first, len = it->second;
+                *off = start + len - bytes;
+                free_.erase(start);
+                if (len > bytes) free_[start] = len - bytes;
+                in_use += bytes;
+                peak = std::max(peak, in_use);
+                return true;
+            }
+        }
+        return false;
+    }
     void release(size_t off, size_t bytes) {
         bytes = round(bytes ? bytes : 1);
         in_use -= bytes;
@@ -100,6 +117,7 @@ struct Req {
     double flops;
     double t_submit = 0, t_done = 0;
     bool done = false;
+    int waiters = 0;  // threads inside cogconv_server_wait on this request
     int status = COGCONV_OK;
     std::string error;
 };
@@ -146,6 +164,7 @@ struct cogconv_server_s {
     // operand 2 (the low-precision frame kernels): arena offset, kept for the
     // server's life; plans that do not pack map to SIZE_MAX.
     std::map<std::pair<cogconv_plan, int>, size_t> packed;
+    size_t reserved = 0;  // arena bytes held by packed banks (taken from the top of the arena)
     // Worker threads (two when there are two streams or more): each admits
     // requests, issues launches and retires completions.  All shared state is
     // guarded by mu; issue() drops the lock only around runtime calls
@@ -235,7 +254,7 @@ struct cogconv_server_s {
                 auto it = packed.find(key);
                 if (it == packed.end()) {
                     size_t o = 0;
-                    if (!arena.alloc(ws, &o)) return false;
+                    if (!arena.alloc_high(ws, &o)) return false;
                     int32_t did = 0;
                     const int pst = unlocked(lk, [&] {
                         int r = cogconv_plan_pack_operand(plan, in1, arena.base + o,
@@ -250,6 +269,7 @@ struct cogconv_server_s {
                     }
                     it = packed.find(key);  // another worker may have packed it meanwhile
                     if (it != packed.end() || !did) arena.release(o, ws);
+                    else reserved += Arena::round(ws);
                     if (it == packed.end()) it = packed.emplace(key, did ? o : SIZE_MAX).first;
                 }
                 pk = it->second;
@@ -380,8 +400,10 @@ struct cogconv_server_s {
                 delete L;
                 inflight.erase(inflight.begin() + i);
                 cv_done.notify_all();
+                cv_work.notify_all();
             }
             // Admissions, in arrival order, while slots and budget allow.
+            bool blocked = false;
             // The requests of a launch leave `pending` before it is issued (the
             // other worker must not take them while mu is dropped) and go back
             // to its front if the budget cannot hold them yet.
@@ -408,13 +430,30 @@ struct cogconv_server_s {
                     ok = issue(g, lk);
                 }
                 issuing--;
+                // Waiters on issuing (drain) and on launch slots (other workers).
+                cv_done.notify_all();
+                cv_work.notify_all();
                 if (!ok) {
+                    if (inflight.empty() && issuing == 0) {
+                        // Nothing holds request memory and the request still does
+                        // not fit (the free space is split around packed banks):
+                        // it never will, so it fails instead of waiting.
+                        fail_reqs(g, COGCONV_ERR_BUDGET, "request does not fit the admission budget");
+                        continue;
+                    }
                     pending.push_front(g[0]);
-                    break;  // wait for memory
+                    blocked = true;  // wait for memory
+                    break;
                 }
             }
+            if (!pending.empty() && (int)inflight.size() + issuing >= cfg.max_inflight) blocked = true;
             if (stop && pending.empty() && inflight.empty() && issuing == 0) break;
-            if (inflight.empty()) {
+            if (inflight.empty() && blocked) {
+                // Admission is held up by launches another worker is issuing
+                // with mu dropped: release mu until that worker notifies (its
+                // issuing-- above), or briefly, and try again.
+                cv_work.wait_for(lk, std::chrono::microseconds(50));
+            } else if (inflight.empty()) {
                 cv_work.wait(lk, [&] { return stop || !pending.empty() || !inflight.empty(); });
             } else {
                 // Launches in flight: poll again at once (a timed sleep rounds up to
@@ -559,11 +598,13 @@ extern "C" int cogconv_server_submit(cogconv_server s, const cogconv_request* re
         const size_t need = resident ? Arena::round(ws)
                                      : Arena::round(q->in0_bytes) + Arena::round(q->out_bytes) + Arena::round(ws) +
                                            (request->bank < 0 ? Arena::round(q->in1_bytes) : 0);
-        if (need > s->arena.size) {
-            std::lock_guard<std::mutex> lk(s->mu);
+        std::lock_guard<std::mutex> lk(s->mu);
+        const size_t room = s->arena.size - s->reserved;
+        if (need > room) {
             s->rejected++;
             delete q;
-            return fail(COGCONV_ERR_BUDGET, "request needs %zu bytes, budget is %zu", need, s->arena.size);
+            return fail(COGCONV_ERR_BUDGET, "request needs %zu bytes, budget is %zu (%zu held by packed banks)", need,
+                        s->arena.size, s->reserved);
         }
     }
     {
@@ -607,7 +648,11 @@ extern "C" int cogconv_server_wait(cogconv_server s, uint64_t ticket) {
     bool done = false;
     int e = ticket_status(s, ticket, &done);
     if (e) return e;
-    s->cv_done.wait(lk, [&] { return s->reqs[ticket]->done; });
+    // The request stays alive while it has waiters (stats_reset skips it).
+    Req* q = s->reqs.find(ticket)->second;
+    q->waiters++;
+    s->cv_done.wait(lk, [&] { return q->done; });
+    q->waiters--;
     return ticket_status(s, ticket, &done);
 }
 
@@ -663,7 +708,7 @@ extern "C" int cogconv_server_stats_reset(cogconv_server s) {
     s->first_submit = s->last_complete = s->gflop = 0;
     s->arena.peak = s->arena.in_use;
     for (auto it = s->reqs.begin(); it != s->reqs.end();) {
-        if (it->second->done) {
+        if (it->second->done && it->second->waiters == 0) {
             delete it->second;
             it = s->reqs.erase(it);
         } else {
