@@ -1,17 +1,25 @@
 """Benchmark of the Field convolution path on one MI355X.
 
-Workload (default): the FP32 CNN-layer triple, batch 16 images 64x64 with 16
-planes, 32 filters 3x3x16 (He-normal, std sqrt(2/144)), BorderValid layer:
-  ProjectFrame + blockReduceSum(16), BorderValid    y  = PF(x, W)        64x64 -> 62x62, 16x32 planes
-  BackProjectFrame + blockReduceSum(32), BorderFull dx = BPF(dy, W)      62x62 -> 64x64, 16x16 planes
-  FilterAdjoint + blockReduceSum(16), BorderValid   dW = FA(x, dy)       -> 3x3, 512 planes
-dy is uniform[-1,1).  One step runs the three ops once; inputs are resident in
-HBM before the timed region starts.
+Default workload: large fields, FP32, with the BF16-MFMA form beside it.
+Batch 8 images 256x256 with 32 planes; one step runs, back to back on one stream:
+  ProjectFrame + blockReduceSum(32), BorderClamp, 32 filters 7x7x32     -> 256x256x32 per image
+  ProjectFrame + blockReduceSum(32), BorderClamp, 32 filters 9x9x32     -> 256x256x32 per image
+  ProjectFrame + blockReduceSum(32), UpsampleInputConvolution(2), BorderZero, the 7x7 bank
+                                                                          -> 512x512x32 per image
+  PlaneByPlane 1024x1024 ScalarField (*) 11x11 Gaussian-plus-noise filter, BorderClamp
+(the reference rejects up-sampling with BorderClamp, so the up-sampled op runs
+BorderZero).  Inputs are seeded and resident in HBM before the timed region.
 
-Prints one JSON line: metric/value (whole-job GFLOP/s of the triple), the
-roofline of the dominant kernel (timed with HIP events on the stream the kernels
-run on), and the CPU baseline (the oracle's FP32 restatement on a bounded sample
-of the same workload, OpenMP on the host cores).
+Prints one JSON line: `value` = FP32 GFLOP/s of the timed steps; `bf16` = the
+same step with the BF16-MFMA plans where they are offered (the two Clamp
+ProjectFrames; the others stay FP32); per-op medians of >= 20 launches, each
+after a 512 MiB cache-flush write, timed with HIP events on the op's stream;
+the roofline of the dominant FP32 kernel; the CPU baseline (the oracle's FP32
+restatement, OpenMP, on a bounded sample of the same workload, on every host
+core the process may use and on one core); and the CNN-layer triple's per-op
+medians as extra keys.
+
+Other workloads: --workload cnn-triple | precision | serving.
 """
 import argparse
 import json
@@ -28,61 +36,89 @@ sys.path.insert(0, os.path.join(ROOT, "cct-core_amd"))
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector and matrix)
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16/FP16 MFMA
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+METRIC = "conv GFLOP/s & % roofline (FP32 vs BF16-MFMA); p50/p99 request latency"
+FLUSH_BYTES = 512 << 20    # beyond the 256 MB last-level cache
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_inputs(seed=16, B=16, N=16, K=32, H=64):
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+# ---------------------------------------------------------------- inputs
+
+def large_inputs(seed=4, B=8, N=32, K=32, H=256):
+    """x: B*N planes of HxH uniform[-1,1); 7x7 and 9x9 He-normal banks of K
+    filters over N planes; the 1024x1024 image and its 11x11 filter (a
+    sigma-2 Gaussian plus N(0, 0.01) noise)."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (B * N, H, H)).astype(np.float32)
+    banks = {F: (rng.standard_normal((K * N, F, F)) * np.sqrt(2.0 / (N * F * F))).astype(np.float32) for F in (7, 9)}
+    yy, xx = np.mgrid[-5:6, -5:6]
+    gauss = np.exp(-(yy ** 2 + xx ** 2) / (2 * 2.0 ** 2))
+    pbp_filter = (gauss / gauss.sum() + rng.normal(0, 0.01, gauss.shape)).astype(np.float32)[None]
+    img = rng.uniform(0, 1, (1, 1024, 1024)).astype(np.float32)
+    return x, banks, img, pbp_filter
+
+
+def large_ops(L, precision, B=8, N=32, K=32, H=256):
+    """(name, desc, operand-1 key, operand-2 key) of one step of the large-field
+    workload.  BF16 is offered for the Clamp ProjectFrames; the up-sampled
+    ProjectFrame and PlaneByPlane run FP32 in both steps."""
+    pn = {L.FP32: "fp32", L.BF16: "bf16"}[precision]
+    ops = []
+    for F in (7, 9):
+        ops.append(("projectFrame+BRS clamp %dx%d %s" % (F, F, pn),
+                    L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_CLAMP, L.CROSS_CORRELATION,
+                                L.NO_SAMPLING, 1, B, (2, H, H, B * N), (2, F, F, K * N), precision), "x", F))
+    ops.append(("projectFrame+BRS up2 zero 7x7 fp32",
+                L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CROSS_CORRELATION,
+                            L.UPSAMPLE_INPUT, 2, B, (2, H, H, B * N), (2, 7, 7, K * N)), "x", 7))
+    ops.append(("planeByPlane clamp 1024x1024 11x11 fp32",
+                L.make_desc(L.PLANE_BY_PLANE, L.BORDER_CLAMP, L.CONVOLUTION, L.NO_SAMPLING, 1, 1,
+                            (2, 1024, 1024, 1), (2, 11, 11, 1)), "img", "pbp"))
+    return ops
+
+
+def triple_inputs(seed=16, B=16, N=16, K=32, H=64):
     rng = np.random.default_rng(seed)
     x = rng.uniform(-1, 1, (B * N, H, H)).astype(np.float32)
     w = (rng.standard_normal((K * N, 3, 3)) * np.sqrt(2.0 / (N * 9))).astype(np.float32)
     dy = rng.uniform(-1, 1, (B * K, H - 2, H - 2)).astype(np.float32)
-    return x, w, dy
+    dyf = rng.uniform(-1, 1, (B * K, H, H)).astype(np.float32)
+    return x, w, dy, dyf
 
 
-def triple_descs(L, B=16, N=16, K=32, H=64, precision=0):
+def triple_descs(L, B=16, N=16, K=32, H=64, precision=0, extras=False):
+    """The CNN-layer triple (Valid layer): PF+BRS Valid, BPF+BRS Full (its
+    adjoint), FA+BRS Valid (the weight gradient).  With `extras`, also PF+BRS
+    with BorderCyclic and with DownsampleOutputConvolution(2) (BorderZero)."""
     f = lambda planes, s: (2, s, s, planes)  # noqa: E731
-    pf = L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_VALID, L.CROSS_CORRELATION, L.NO_SAMPLING, 1, B,
-                     f(B * N, H), f(K * N, 3), precision)
-    bpf = L.make_desc(L.BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_FULL, L.CONVOLUTION, L.NO_SAMPLING, 1, B,
-                      f(B * K, H - 2), f(K * N, 3), precision)
-    fa = L.make_desc(L.FILTER_ADJOINT_BLOCK_REDUCE_SUM, L.BORDER_VALID, L.CROSS_CORRELATION, L.NO_SAMPLING, 1, B,
-                     f(B * N, H), f(B * K, H - 2), precision)
-    return [("projectFrame+BRS", pf), ("backProjectFrame+BRS", bpf), ("filterAdjoint+BRS", fa)]
+    ops = [("projectFrame+BRS", L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_VALID, L.CROSS_CORRELATION,
+                                            L.NO_SAMPLING, 1, B, f(B * N, H), f(K * N, 3), precision), "x", "w"),
+           ("backProjectFrame+BRS", L.make_desc(L.BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_FULL, L.CONVOLUTION,
+                                                L.NO_SAMPLING, 1, B, f(B * K, H - 2), f(K * N, 3), precision),
+            "dy", "w"),
+           ("filterAdjoint+BRS", L.make_desc(L.FILTER_ADJOINT_BLOCK_REDUCE_SUM, L.BORDER_VALID, L.CROSS_CORRELATION,
+                                             L.NO_SAMPLING, 1, B, f(B * N, H), f(B * K, H - 2), precision),
+            "x", "dy")]
+    if extras:
+        ops.append(("projectFrame+BRS cyclic", L.make_desc(
+            L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_CYCLIC, L.CROSS_CORRELATION, L.NO_SAMPLING, 1, B,
+            f(B * N, H), f(K * N, 3), precision), "x", "w"))
+        ops.append(("projectFrame+BRS down2", L.make_desc(
+            L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CROSS_CORRELATION, L.DOWNSAMPLE_OUTPUT, 2, B,
+            f(B * N, H), f(K * N, 3), precision), "x", "w"))
+    return ops
 
 
-def cpu_baseline(sample_images=16, budget_s=10.0):
-    """Oracle FP32 restatement (reference accumulation order), OpenMP over output
-    planes, on `sample_images` of the batch; median GFLOP/s over repetitions."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    threads = max(1, min(cores, 16))
-    B, N, K, H = sample_images, 16, 32, 64
-    x, w, dy = make_inputs(B=B)
-    ops = [(O.make_desc(O.PF_BRS, O.VALID, O.XCORR, O.NONE, 1, B, x.shape, w.shape), x, w),
-           (O.make_desc(O.BPF_BRS, O.FULL, O.CONV, O.NONE, 1, B, dy.shape, w.shape), dy, w),
-           (O.make_desc(O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, B, x.shape, dy.shape), x, dy)]
-    flops = 2.0 * B * N * K * 9 * (62 * 62 + 64 * 64 + 62 * 62)
-    for d, a, k in ops:  # warm-up
-        O.conv(d, a, k, prec="f32", threads=threads)
-    times = []
-    t_all = time.perf_counter()
-    while len(times) < 5 or (time.perf_counter() - t_all < budget_s and len(times) < 100):
-        t0 = time.perf_counter()
-        for d, a, k in ops:
-            O.conv(d, a, k, prec="f32", threads=threads)
-        times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    return {"value": round(flops / med / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": "%d of 16 images of the same triple (%.3g GFLOP), oracle FP32 restatement, median of %d reps"
-                      % (B, flops / 1e9, len(times))}
-
+# ---------------------------------------------------------------- timing
 
 def tuned_plan(L, desc, args):
     """The plan for `desc`, with its kernel configuration chosen by the
@@ -99,6 +135,203 @@ def tuned_plan(L, desc, args):
     return L.Plan(desc)
 
 
+class Op:
+    """One planned op with its buffers."""
+
+    def __init__(self, L, name, plan, in0, in1):
+        self.name, self.plan, self.in0, self.in1 = name, plan, in0, in1
+        self.out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
+        self.ws = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
+
+    def launch(self, stream):
+        self.plan.launch(self.in0, self.in1, self.out, self.ws, stream)
+
+    def close(self):
+        self.out.close()
+        if self.ws:
+            self.ws.close()
+
+
+def flushed_median_ms(L, op, stream, flush, reps=20, warm=2):
+    """Median over `reps` launches of `op`, each timed alone with HIP events on
+    `stream` after a write of the whole flush buffer (so no operand is left in
+    the L2 or the last-level cache)."""
+    for _ in range(warm):
+        op.launch(stream)
+    e0, e1 = L.Event(), L.Event()
+    times = []
+    for _ in range(reps):
+        flush.zero(stream)
+        e0.record(stream)
+        op.launch(stream)
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_ms(e1))
+    e0.close()
+    e1.close()
+    return statistics.median(times), min(times), max(times)
+
+
+def timed_steps(L, ops, stream, steps, warmup):
+    """`warmup` untimed steps, then exactly `steps` steps (every op once, back
+    to back on `stream`) bracketed by device synchronisations; returns ms per
+    step (the larger of the event and wall-clock spans)."""
+    for _ in range(warmup):
+        for op in ops:
+            op.launch(stream)
+    L.synchronize()
+    t0, t1 = L.Event(), L.Event()
+    wall0 = time.perf_counter()
+    t0.record(stream)
+    for _ in range(steps):
+        for op in ops:
+            op.launch(stream)
+    t1.record(stream)
+    t1.synchronize()
+    L.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = max(t0.elapsed_ms(t1), wall * 1e3) / steps
+    t0.close()
+    t1.close()
+    return ms
+
+
+def roofline(plan, ms, peak_tflops, name, reps, traffic_file):
+    """Roofline object for one launch of `plan` taking `ms`: compute-bound when
+    its algorithmic FLOP/byte is above the ridge of `peak_tflops` and HBM."""
+    ai = plan.flops / plan.bytes
+    ridge = peak_tflops * 1e12 / (PEAK_HBM_GBS * 1e9)
+    if ai >= ridge:
+        achieved = plan.flops / (ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak_tflops, "unit": "TFLOP/s",
+                "frac": round(achieved / peak_tflops, 4), "traffic": None}
+    else:
+        achieved = plan.bytes / (ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None}
+    roof["kernel"] = name + ": " + plan.description
+    path = os.path.join(ROOT, "profiles", traffic_file)
+    if os.path.exists(path):
+        with open(path) as fh:
+            t = json.load(fh).get(name)
+        if t:
+            roof["traffic"] = t["hbm_bytes_per_launch"]
+            roof["traffic_source"] = ("profiles/%s (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
+                                      "copy-calibrated)" % traffic_file)
+    roof["algorithmic_flops"] = plan.flops
+    roof["algorithmic_bytes"] = plan.bytes
+    roof["arithmetic_intensity"] = round(ai, 1)
+    roof["avg_launch_ms"] = round(ms, 5)
+    roof["timing"] = ("median of %d launches of this op alone on its stream, HIP events, each after a "
+                      "512 MiB cache-flush write" % reps)
+    return roof
+
+
+def copy_calibration(L):
+    """Known-size reference for the HBM counters: fill 1 GiB, then copy its
+    first 64 MiB (long evicted from the 256 MB last-level cache by the fill)
+    three times."""
+    src, dst = L.DeviceBuffer(1 << 30), L.DeviceBuffer(64 << 20)
+    src.zero()
+    L.synchronize()
+    for _ in range(3):
+        L.check(L.lib().cogconv_d2d_async(dst.ptr, src.ptr, 64 << 20, None))
+    L.synchronize()
+    src.close()
+    dst.close()
+
+
+# ---------------------------------------------------------------- CPU baseline
+
+def cpu_baseline_large(budget_s=20.0):
+    """The oracle's FP32 restatement (the reference's per-output accumulation
+    order), OpenMP over output rows, on a bounded sample of the large-field
+    step: image 0 of the batch for the 7x7 / 9x9 Clamp and the up-sampled
+    ProjectFrames plus the whole PlaneByPlane, on every core the process may
+    use; and on one core, image 0 with the first 2 of the 32 filters.  Median
+    GFLOP/s (algorithmic FLOPs, as on the GPU) over >= 5 repetitions after one
+    warm-up."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    x, banks, img, pbp = large_inputs()
+    N, H = 32, 256
+    x0 = x[:N]
+
+    def sample(k):
+        ops, flops = [], 0.0
+        for F in (7, 9):
+            w = banks[F][:k * N]
+            ops.append((O.make_desc(O.PF_BRS, O.CLAMP, O.XCORR, O.NONE, 1, 1, x0.shape, w.shape), x0, w))
+            flops += 2.0 * k * N * H * H * F * F
+        w7 = banks[7][:k * N]
+        ops.append((O.make_desc(O.PF_BRS, O.ZERO, O.XCORR, O.UP, 2, 1, x0.shape, w7.shape), x0, w7))
+        flops += 2.0 * k * N * (2 * H) * (2 * H) * 49 / 4.0
+        ops.append((O.make_desc(O.PBP, O.CLAMP, O.CONV, O.NONE, 1, 1, img.shape, pbp.shape), img, pbp))
+        flops += 2.0 * 1024 * 1024 * 121
+        return ops, flops
+
+    def run(ops, flops, threads, budget):
+        for d, a, k in ops:
+            O.conv(d, a, k, prec="f32", threads=threads)
+        times = []
+        t_all = time.perf_counter()
+        while len(times) < 5 or (time.perf_counter() - t_all < budget and len(times) < 20):
+            t0 = time.perf_counter()
+            for d, a, k in ops:
+                O.conv(d, a, k, prec="f32", threads=threads)
+            times.append(time.perf_counter() - t0)
+        return flops / statistics.median(times) / 1e9, len(times)
+
+    cores = host_cores()
+    ops, flops = sample(32)
+    v_all, n_all = run(ops, flops, cores, budget_s / 2)
+    ops1, flops1 = sample(2)
+    v_one, n_one = run(ops1, flops1, 1, budget_s / 2)
+    return {"value": round(v_all, 3), "unit": "GFLOP/s", "cores": cores, "kind": "port",
+            "sample": "image 0 of 8 of the FP32 step (7x7 and 9x9 Clamp PF+BRS, Up2 7x7 PF+BRS, and the whole "
+                      "PlaneByPlane; %.3g GFLOP), oracle FP32 restatement, OpenMP over (output plane, 8-row band) on every core "
+                      "in the process's affinity mask, median of %d reps" % (flops / 1e9, n_all),
+            "one_core": {"value": round(v_one, 3), "unit": "GFLOP/s", "cores": 1,
+                         "sample": "image 0, filters 0-1 of 32 of the same ops (%.3g GFLOP), median of %d reps"
+                                   % (flops1 / 1e9, n_one)},
+            "note": "the oracle materialises the zero-inserted operand of the up-sampled op (4x the multiply-adds "
+                    "of the reference's kernel, which skips zero taps); GFLOP/s counts algorithmic FLOPs"}
+
+
+# ---------------------------------------------------------------- PMC summaries
+
+def pmc_traffic(fetch_csv, write_csv, calib_kernel="__amd_rocclr_copyBuffer", calib_bytes=64 << 20):
+    """HBM bytes per dispatch per kernel from two rocprofv3 counter-collection
+    CSVs (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what
+    the same counter reads for the 64 MiB copy of --copy-calibration."""
+    import csv
+
+    def read(path, counter):
+        per = {}
+        with open(path) as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                per.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+        return per
+    fetch, write = read(fetch_csv, "FETCH_SIZE"), read(write_csv, "WRITE_SIZE")
+    cal = [k for k in fetch if calib_kernel in k]
+    if not cal:
+        raise SystemExit("no copy kernel %r in %s (run the profiled command with --copy-calibration)"
+                         % (calib_kernel, fetch_csv))
+    # The calibration copy reads and writes calib_bytes; the median dispatch of the copy kernel.
+    ff = calib_bytes / 1024.0 / statistics.median(fetch[cal[0]])
+    wf = calib_bytes / 1024.0 / statistics.median(write[[k for k in write if calib_kernel in k][0]])
+    out = {"_calibration": {"fetch_factor": round(ff, 4), "write_factor": round(wf, 4),
+                            "method": "64 MiB device copy from a cold 1 GiB source, bench.py --copy-calibration"}}
+    for k in fetch:
+        if calib_kernel in k or k not in write:
+            continue
+        fk = statistics.median(fetch[k])
+        wk = statistics.median(write[k])
+        out[k] = {"fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+                  "hbm_bytes_per_dispatch": int(round(1024.0 * (fk * ff + wk * wf)))}
+    return out
 def precision_workload(args):
     """B32 128x128x64 planes, 64 filters 3x3x64: ProjectFrame+BRS and
     BackProjectFrame+BRS (BorderZero) and FilterAdjoint+BRS (BorderValid, dy
@@ -190,89 +423,148 @@ def precision_workload(args):
     print(json.dumps(line), flush=True)
 
 
-def large_workload(args):
-    """Large fields: batch 8 images 256x256x32, 32 filters 7x7x32 and 9x9x32,
-    ProjectFrame+BRS with BorderClamp in FP32 and BF16; the same 7x7 bank on
-    the 2x up-sampled input (UpsampleInputConvolution(2), BorderZero, the
-    reference rejects Up + Clamp); PlaneByPlane 1024x1024 scalar field with
-    an 11x11 Gaussian-plus-noise filter, BorderClamp.  GFLOP/s from HIP
-    events; BF16 errors against the FP64 oracle on a one-image slice."""
-    import cogconv._lib as L
+def large_errors(L, x, banks, N=32, H=256):
+    """Normwise and rel-L2 errors of the Clamp ProjectFrames (FP32 and BF16)
+    against the FP64 oracle on image 0."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    L.init(0)
-    rng = np.random.default_rng(4)
-    B, N, K, H = 8, 32, 32, 256
-    x = rng.uniform(-1, 1, (B * N, H, H)).astype(np.float32)
-    banks = {F: (rng.standard_normal((K * N, F, F)) * np.sqrt(2.0 / (N * F * F))).astype(np.float32) for F in (7, 9)}
-    yy, xx = np.mgrid[-5:6, -5:6]
-    gauss = np.exp(-(yy ** 2 + xx ** 2) / (2 * 2.0 ** 2))
-    pf = (gauss / gauss.sum() + rng.normal(0, 0.01, gauss.shape)).astype(np.float32)[None]
-    img = rng.uniform(0, 1, (1, 1024, 1024)).astype(np.float32)
-    bx, bimg, bpf = (L.DeviceBuffer.from_array(a) for a in (x, img, pf))
-    bb = {F: L.DeviceBuffer.from_array(w) for F, w in banks.items()}
-    stream = L.Stream()
-    ops = []
-    for F in (7, 9):
-        for prec, pn in ((L.FP32, "fp32"), (L.BF16, "bf16")):
-            ops.append(("projectFrame+BRS clamp %dx%d %s" % (F, F, pn),
-                        L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_CLAMP, L.CROSS_CORRELATION,
-                                    L.NO_SAMPLING, 1, B, (2, H, H, B * N), (2, F, F, K * N), prec), bx, bb[F]))
-    ops.append(("projectFrame+BRS up2 zero 7x7 fp32",
-                L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CROSS_CORRELATION,
-                            L.UPSAMPLE_INPUT, 2, B, (2, H, H, B * N), (2, 7, 7, K * N)), bx, bb[7]))
-    ops.append(("planeByPlane clamp 1024x1024 11x11 fp32",
-                L.make_desc(L.PLANE_BY_PLANE, L.BORDER_CLAMP, L.CONVOLUTION, L.NO_SAMPLING, 1, 1,
-                            (2, 1024, 1024, 1), (2, 11, 11, 1)), bimg, bpf))
-    res = {}
-    for name, desc, a, k in ops:
-        plan = tuned_plan(L, desc, args)
-        out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
-        ws = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
-        for _ in range(2):
-            plan.launch(a, k, out, ws, stream)
-        e0, e1 = L.Event(), L.Event()
-        reps = 10
-        e0.record(stream)
-        for _ in range(reps):
-            plan.launch(a, k, out, ws, stream)
-        e1.record(stream)
-        e1.synchronize()
-        ms = e0.elapsed_ms(e1) / reps
-        res[name] = {"ms": round(ms, 4), "gflops": round(plan.flops / (ms * 1e-3) / 1e9, 1),
-                     "gflop": round(plan.flops / 1e9, 3), "kernel": plan.description}
-        log(name, plan.description, "%.4f ms" % ms)
-        out.close()
-        if ws:
-            ws.close()
     errs = {}
     xs = x[:N]
+    a = L.DeviceBuffer.from_array(xs)
     for F in (7, 9):
         od = O.make_desc(O.PF_BRS, O.CLAMP, O.XCORR, O.NONE, 1, 1, xs.shape, banks[F].shape)
-        ref = O.conv(od, xs, banks[F], threads=16)
+        ref = O.conv(od, xs, banks[F], threads=host_cores())
+        wb = L.DeviceBuffer.from_array(banks[F])
         for prec, pn in ((L.FP32, "fp32"), (L.BF16, "bf16")):
             plan = L.Plan(L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_CLAMP, L.CROSS_CORRELATION,
-                                      L.NO_SAMPLING, 1, 1, (2, H, H, N), (2, F, F, K * N), prec))
-            a = L.DeviceBuffer.from_array(xs)
-            out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
-            wsb = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
-            plan.launch(a, bb[F], out, wsb, None)
+                                      L.NO_SAMPLING, 1, 1, (2, H, H, N), (2, F, F, banks[F].shape[0]), prec))
+            op = Op(L, "", plan, a, wb)
+            op.launch(None)
             L.synchronize()
-            got = out.download(plan.out_shape).astype(np.float64)
+            got = op.out.download(plan.out_shape).astype(np.float64)
             d = np.abs(got - ref)
-            errs["%dx%d %s" % (F, F, pn)] = {"max_rel_normwise": float(d.max() / np.abs(ref).max()),
+            errs["%dx%d %s" % (F, F, pn)] = {"max_abs": float(d.max()),
+                                           "max_rel_normwise": float(d.max() / np.abs(ref).max()),
                                            "rel_l2": float(np.sqrt((d ** 2).sum() / (ref ** 2).sum()))}
-            for b_ in (a, out, wsb):
-                if b_ is not None:
-                    b_.close()
-    line = {"metric": "conv GFLOP/s & % roofline (FP32 vs BF16-MFMA); p50/p99 request latency",
-            "workload": "large fields: batch 8 images 256x256x32, 32 filters 7x7x32 / 9x9x32 (BorderClamp; "
-                        "Up2 with BorderZero), PlaneByPlane 1024x1024 11x11 BorderClamp",
-            "results": res, "errors_vs_fp64_one_image": errs,
-            "peaks": {"fp32_tflops": PEAK_FP32_TFLOPS, "bf16_tflops": PEAK_BF16_TFLOPS, "hbm_gbs": PEAK_HBM_GBS}}
+            op.close()
+            plan.close()
+        wb.close()
+    a.close()
+    return errs
+
+
+def triple_extras(L, args, stream, flush):
+    """Per-op flushed medians of the FP32 CNN-layer triple (batch 16 images
+    64x64x16, 32 filters 3x3x16) and of its BorderCyclic and Down2
+    ProjectFrames."""
+    x, w, dy, _ = triple_inputs()
+    bufs = {"x": L.DeviceBuffer.from_array(x), "w": L.DeviceBuffer.from_array(w), "dy": L.DeviceBuffer.from_array(dy)}
+    res = {}
+    for name, desc, k0, k1 in triple_descs(L, extras=True):
+        plan = tuned_plan(L, desc, args)
+        op = Op(L, name, plan, bufs[k0], bufs[k1])
+        med, lo, hi = flushed_median_ms(L, op, stream, flush)
+        res[name] = {"ms": round(med, 5), "min_ms": round(lo, 5), "max_ms": round(hi, 5),
+                     "gflops": round(plan.flops / (med * 1e-3) / 1e9, 1),
+                     "frac_fp32_peak": round(plan.flops / (med * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                     "kernel": plan.description}
+        log("cnn triple", name, plan.description, "%.4f ms" % med)
+        op.close()
+        plan.close()
+    for b in bufs.values():
+        b.close()
+    return res
+
+
+def large_workload(args):
+    import cogconv._lib as L
+    L.init(0)
+    props = L.device_props(0)
+    log("device:", props)
+    if args.copy_calibration:
+        copy_calibration(L)
+    x, banks, img, pbp = large_inputs()
+    bufs = {"x": L.DeviceBuffer.from_array(x), 7: L.DeviceBuffer.from_array(banks[7]),
+            9: L.DeviceBuffer.from_array(banks[9]), "img": L.DeviceBuffer.from_array(img),
+            "pbp": L.DeviceBuffer.from_array(pbp)}
+    stream = L.Stream()
+    flush = L.DeviceBuffer(FLUSH_BYTES)
+    steps = {}
+    for prec, pn in ((L.FP32, "fp32"), (L.BF16, "bf16")):
+        ops = []
+        for name, desc, k0, k1 in large_ops(L, prec):
+            plan = tuned_plan(L, desc, args)
+            ops.append(Op(L, name, plan, bufs[k0], bufs[k1]))
+            log("plan:", name, plan.description, "GFLOP %.3f" % (plan.flops / 1e9))
+        ms = timed_steps(L, ops, stream, args.steps, args.warmup)
+        per_op = {}
+        for op in ops:
+            med, lo, hi = flushed_median_ms(L, op, stream, flush, reps=args.op_reps)
+            per_op[op.name] = {"ms": round(med, 5), "min_ms": round(lo, 5), "max_ms": round(hi, 5),
+                               "gflops": round(op.plan.flops / (med * 1e-3) / 1e9, 1),
+                               "gflop": round(op.plan.flops / 1e9, 3), "kernel": op.plan.description}
+            log(pn, op.name, op.plan.description, "%.4f ms" % med)
+        steps[pn] = {"ops": ops, "ms_per_step": ms, "flops": sum(op.plan.flops for op in ops), "per_op": per_op}
+    fp32, bf16 = steps["fp32"], steps["bf16"]
+    value = fp32["flops"] / (fp32["ms_per_step"] * 1e-3) / 1e9
+    bf16_value = bf16["flops"] / (bf16["ms_per_step"] * 1e-3) / 1e9
+    # The dominant kernel: the FP32 op with the longest median launch.
+    dom = max(fp32["ops"], key=lambda op: fp32["per_op"][op.name]["ms"])
+    roof = roofline(dom.plan, fp32["per_op"][dom.name]["ms"], PEAK_FP32_TFLOPS, dom.name, args.op_reps,
+                    "traffic_large.json")
+    # BF16 beside it: the BF16 op with the longest median launch, against dense BF16 MFMA.
+    bdom = max([op for op in bf16["ops"] if "bf16" in op.name], key=lambda op: bf16["per_op"][op.name]["ms"])
+    broof = roofline(bdom.plan, bf16["per_op"][bdom.name]["ms"], PEAK_BF16_TFLOPS, bdom.name, args.op_reps,
+                     "traffic_large.json")
+    ratios = {}
+    for F in (7, 9):
+        a = fp32["per_op"]["projectFrame+BRS clamp %dx%d fp32" % (F, F)]["ms"]
+        b = bf16["per_op"]["projectFrame+BRS clamp %dx%d bf16" % (F, F)]["ms"]
+        ratios["clamp %dx%d" % (F, F)] = round(a / b, 2)
+    errs = large_errors(L, x, banks)
+    extras = None if args.no_extras else triple_extras(L, args, stream, flush)
+    for st in steps.values():
+        for op in st["ops"]:
+            op.close()
+            op.plan.close()
+    cpu = None if args.no_cpu else cpu_baseline_large(budget_s=args.cpu_budget)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fp32["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform[-1,1) fields, He-normal filter banks, Gaussian-plus-noise 11x11 filter)",
+        "kernel_choice": "heuristic (--no-tune)" if args.no_tune else "autotuned per op (cogconv_autotune)",
+        "config": {"workload": "large fields: batch 8 images 256x256x32, 32 filters 7x7x32 and 9x9x32, "
+                               "ProjectFrame+blockReduceSum BorderClamp; the 7x7 bank on the 2x up-sampled input "
+                               "(BorderZero); PlaneByPlane 1024x1024 ScalarField with an 11x11 filter, BorderClamp",
+                   "batch": 8, "image": [256, 256, 32], "filters": [[32, 7, 7, 32], [32, 9, 9, 32]],
+                   "plane_by_plane": [1024, 1024, 11, 11], "flops_per_step": fp32["flops"],
+                   "step": "the four ops back to back on one stream"},
+        "per_op": fp32["per_op"],
+        "bf16": {"value": round(bf16_value, 2), "unit": "GFLOP/s", "ms_per_step": round(bf16["ms_per_step"], 5),
+                 "step": "the Clamp ProjectFrames as BF16-MFMA plans (FP32 accumulation); the up-sampled op and "
+                         "PlaneByPlane stay FP32",
+                 "per_op": {k: v for k, v in bf16["per_op"].items() if "bf16" in k},
+                 "bf16_over_fp32_per_op": ratios, "roofline": broof},
+        "errors_vs_fp64_image0": errs,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "cnn_triple_per_op": extras,
+        "peaks": {"fp32_tflops": PEAK_FP32_TFLOPS, "bf16_tflops": PEAK_BF16_TFLOPS, "hbm_gbs": PEAK_HBM_GBS},
+        "device": props["name"] + " " + props["arch"],
+    }
+    flush.close()
+    for b in bufs.values():
+        b.close()
     print(json.dumps(line), flush=True)
-
-
 RATE_SWEEP = (4000, 8000, 12000, 16000, 20000, 24000, 28000, 32000, 40000, 48000, 56000, 64000, 72000, 80000,
               96000)  # offered req/s, resident transport
 P99_TARGETS_MS = (0.25, 0.5, 1.0, 2.0)
@@ -393,175 +685,103 @@ def serving_workload(args):
     print(json.dumps(line), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-budget", type=float, default=10.0)
-    ap.add_argument("--schedule", default="serial", choices=("serial", "layer", "concurrent"),
-                    help="serial (default): the three ops back to back on one stream; layer: ProjectFrame, then "
-                         "BackProjectFrame and FilterAdjoint on two streams, as a layer's forward and backward order "
-                         "them; concurrent: all three on three streams")
-    ap.add_argument("--concurrent", action="store_true", help="same as --schedule concurrent")
-    ap.add_argument("--workload", default="cnn-triple", choices=("cnn-triple", "precision", "serving", "large"))
-    ap.add_argument("--requests", type=int, default=3000, help="requests of the serving workload")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="use the heuristic kernel configuration instead of autotuning each op's descriptor")
-    ap.add_argument("--tune-cache", default=None,
-                    help="autotuning cache file: loaded before planning, saved after (cached samples are not re-timed)")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run three 64 MiB device-to-device copies (a known-size reference for HBM counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "precision":
-        precision_workload(args)
-        return
-    if args.workload == "serving":
-        serving_workload(args)
-        return
-    if args.workload == "large":
-        large_workload(args)
-        return
+def cpu_baseline_triple(sample_images=16, budget_s=10.0):
+    """Oracle FP32 restatement, OpenMP over output planes on every core the
+    process may use, on `sample_images` of the CNN-layer triple's batch."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = host_cores()
+    B, N, K = sample_images, 16, 32
+    x, w, dy, _ = triple_inputs(B=B)
+    ops = [(O.make_desc(O.PF_BRS, O.VALID, O.XCORR, O.NONE, 1, B, x.shape, w.shape), x, w),
+           (O.make_desc(O.BPF_BRS, O.FULL, O.CONV, O.NONE, 1, B, dy.shape, w.shape), dy, w),
+           (O.make_desc(O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, B, x.shape, dy.shape), x, dy)]
+    flops = 2.0 * B * N * K * 9 * (62 * 62 + 64 * 64 + 62 * 62)
+    for d, a, k in ops:
+        O.conv(d, a, k, prec="f32", threads=threads)
+    times = []
+    t_all = time.perf_counter()
+    while len(times) < 5 or (time.perf_counter() - t_all < budget_s and len(times) < 100):
+        t0 = time.perf_counter()
+        for d, a, k in ops:
+            O.conv(d, a, k, prec="f32", threads=threads)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": round(flops / med / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": "%d of 16 images of the same triple (%.3g GFLOP), oracle FP32 restatement, median of %d reps"
+                      % (B, flops / 1e9, len(times))}
 
+
+def cnn_triple_workload(args):
+    """The FP32 CNN-layer triple: batch 16 images 64x64x16, 32 filters 3x3x16
+    (He-normal), PF+BRS Valid, BPF+BRS Full, FA+BRS Valid; one step runs the
+    three ops once."""
     import cogconv._lib as L
     L.init(0)
     props = L.device_props(0)
     log("device:", props)
-
     if args.copy_calibration:
-        # Known-size reference for the HBM counters: fill 1 GiB, then copy its
-        # first 64 MiB (long evicted from the 256 MB Infinity Cache by the fill)
-        # three times.
-        src, dst = L.DeviceBuffer(1 << 30), L.DeviceBuffer(64 << 20)
-        src.zero()
-        L.synchronize()
-        for _ in range(3):
-            L.check(L.lib().cogconv_d2d_async(dst.ptr, src.ptr, 64 << 20, None))
-        L.synchronize()
-        src.close()
-        dst.close()
-
-    x, w, dy = make_inputs()
-    bx, bw, bdy = (L.DeviceBuffer.from_array(a) for a in (x, w, dy))
+        copy_calibration(L)
+    x, w, dy, _ = triple_inputs()
+    bufs = {"x": L.DeviceBuffer.from_array(x), "w": L.DeviceBuffer.from_array(w), "dy": L.DeviceBuffer.from_array(dy)}
     ops = []
-    for name, desc in triple_descs(L):
+    for name, desc, k0, k1 in triple_descs(L):
         plan = tuned_plan(L, desc, args)
-        out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
-        ws = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
-        ins = {"projectFrame+BRS": (bx, bw), "backProjectFrame+BRS": (bdy, bw), "filterAdjoint+BRS": (bx, bdy)}[name]
-        ops.append({"name": name, "plan": plan, "in": ins, "out": out, "ws": ws})
+        ops.append(Op(L, name, plan, bufs[k0], bufs[k1]))
         log("plan:", name, plan.description, "GFLOP %.4f" % (plan.flops / 1e9))
-    flops_step = sum(op["plan"].flops for op in ops)
-    streams = [L.Stream() for _ in ops]
-    main_stream = streams[0]
-
-    def launch(i, stream):
-        op = ops[i]
-        op["plan"].launch(op["in"][0], op["in"][1], op["out"], op["ws"], stream)
-
-    # Phase 1: each op alone on one stream, timed with HIP events on that
-    # stream (average launch duration; this feeds the roofline).
-    iso_reps = max(20, args.steps // 4)
-    per_op_ms = []
-    for i in range(len(ops)):
-        for _ in range(5):
-            launch(i, main_stream)
-        e0, e1 = L.Event(), L.Event()
-        e0.record(main_stream)
-        for _ in range(iso_reps):
-            launch(i, main_stream)
-        e1.record(main_stream)
-        e1.synchronize()
-        per_op_ms.append(e0.elapsed_ms(e1) / iso_reps)
-
-    # Phase 2: the timed steps.  The three ops read only the step's inputs (PF:
-    # x, W; BPF: dy, W; FA: x, dy).  "layer" orders them as a layer's training
-    # step does: the forward ProjectFrame first, then the two backward ops, which
-    # do not depend on each other, on two streams; "serial" runs all three back
-    # to back on one stream; "concurrent" forks all three.
+    flops_step = sum(op.plan.flops for op in ops)
+    stream = L.Stream()
+    flush = L.DeviceBuffer(FLUSH_BYTES)
     schedule = "concurrent" if args.concurrent else args.schedule
-    fork = [L.Event() for _ in range(args.steps + args.warmup)]
-    done = [[L.Event() for _ in ops] for _ in range(args.steps + args.warmup)]
-
-    def run_step(k):
-        if schedule == "serial":
-            for i in range(len(ops)):
-                launch(i, main_stream)
-            return
-        first = 1 if schedule == "layer" else 0
-        for i in range(first):
-            launch(i, main_stream)
-        fork[k].record(main_stream)
-        for i in range(first, len(ops)):
-            st = streams[i]
-            if i:
-                L.check(L.lib().cogconv_stream_wait_event(st.handle, fork[k].handle))
-            launch(i, st)
-            if i:
-                done[k][i].record(st)
-        for i in range(max(1, first), len(ops)):
-            L.check(L.lib().cogconv_stream_wait_event(main_stream.handle, done[k][i].handle))
-
-    for k in range(args.warmup):
-        run_step(k)
-    L.synchronize()
-    K = args.steps
-    t_start, t_end = L.Event(), L.Event()
-    wall0 = time.perf_counter()
-    t_start.record(main_stream)
-    for k in range(K):
-        run_step(args.warmup + k)
-    t_end.record(main_stream)
-    t_end.synchronize()
-    L.synchronize()
-    wall = time.perf_counter() - wall0
-    total_ms = t_start.elapsed_ms(t_end)
-    ms_per_step = max(total_ms, wall * 1e3) / K
-    value = flops_step / (ms_per_step * 1e-3) / 1e9
-
-    dom = int(np.argmax(per_op_ms))
-    dplan = ops[dom]["plan"]
-    ai = dplan.flops / dplan.bytes
-    ridge = PEAK_FP32_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9)
-    if ai >= ridge:
-        achieved = dplan.flops / (per_op_ms[dom] * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None}
+    if schedule == "serial":
+        ms_per_step = timed_steps(L, ops, stream, args.steps, args.warmup)
     else:
-        achieved = dplan.bytes / (per_op_ms[dom] * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None}
-    roof["kernel"] = ops[dom]["name"] + ": " + dplan.description
-    tfile = os.path.join(ROOT, "profiles", "traffic_cnn_triple.json")
-    if os.path.exists(tfile):
-        with open(tfile) as fh:
-            t = json.load(fh).get(ops[dom]["name"])
-        if t:
-            roof["traffic"] = t["hbm_bytes_per_launch"]
-            roof["traffic_source"] = "profiles/traffic_cnn_triple.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
-    roof["algorithmic_flops"] = dplan.flops
-    roof["algorithmic_bytes"] = dplan.bytes
-    roof["avg_launch_ms"] = round(per_op_ms[dom], 5)
-    roof["timing"] = "HIP events around %d back-to-back launches of this op alone on its stream" % iso_reps
+        # "layer": the forward ProjectFrame first, then the two backward ops
+        # (independent of each other) on two streams; "concurrent": all three forked.
+        streams = [stream] + [L.Stream() for _ in ops[1:]]
+        n = args.steps + args.warmup
+        fork = [L.Event() for _ in range(n)]
+        done = [[L.Event() for _ in ops] for _ in range(n)]
 
-    cpu = None if args.no_cpu else cpu_baseline(budget_s=args.cpu_budget)
+        def run_step(k):
+            first = 1 if schedule == "layer" else 0
+            for i in range(first):
+                ops[i].launch(stream)
+            fork[k].record(stream)
+            for i in range(first, len(ops)):
+                if i:
+                    L.check(L.lib().cogconv_stream_wait_event(streams[i].handle, fork[k].handle))
+                ops[i].launch(streams[i])
+                if i:
+                    done[k][i].record(streams[i])
+            for i in range(max(1, first), len(ops)):
+                L.check(L.lib().cogconv_stream_wait_event(stream.handle, done[k][i].handle))
+        for k in range(args.warmup):
+            run_step(k)
+        L.synchronize()
+        t0, t1 = L.Event(), L.Event()
+        wall0 = time.perf_counter()
+        t0.record(stream)
+        for k in range(args.steps):
+            run_step(args.warmup + k)
+        t1.record(stream)
+        t1.synchronize()
+        L.synchronize()
+        ms_per_step = max(t0.elapsed_ms(t1), (time.perf_counter() - wall0) * 1e3) / args.steps
+    per_op = {}
+    for op in ops:
+        med, lo, hi = flushed_median_ms(L, op, stream, flush, reps=args.op_reps)
+        per_op[op.name] = {"ms": round(med, 5), "min_ms": round(lo, 5), "max_ms": round(hi, 5),
+                           "gflops": round(op.plan.flops / (med * 1e-3) / 1e9, 1), "kernel": op.plan.description}
+    value = flops_step / (ms_per_step * 1e-3) / 1e9
+    dom = max(ops, key=lambda op: per_op[op.name]["ms"])
+    roof = roofline(dom.plan, per_op[dom.name]["ms"], PEAK_FP32_TFLOPS, dom.name, args.op_reps,
+                    "traffic_cnn_triple.json")
+    cpu = None if args.no_cpu else cpu_baseline_triple(budget_s=args.cpu_budget)
     line = {
-        "metric": "conv GFLOP/s & % roofline (FP32 vs BF16-MFMA); p50/p99 request latency",
-        "value": round(value, 2),
-        "unit": "GFLOP/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
+        "metric": METRIC, "value": round(value, 2), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded uniform[-1,1) x and dy, He-normal filters)",
         "kernel_choice": "heuristic (--no-tune)" if args.no_tune else "autotuned per op (cogconv_autotune)",
         "config": {"workload": "FP32 CNN-layer triple: PF+BRS Valid, BPF+BRS Full, FA+BRS Valid; batch 16 "
@@ -570,13 +790,46 @@ def main():
                    "step": {"layer": "ProjectFrame, then BackProjectFrame and FilterAdjoint on two streams",
                             "serial": "ops back to back on one stream",
                             "concurrent": "three ops on three streams"}[schedule]},
-        "per_op_isolated": {op["name"]: {"ms": round(t, 5), "gflops": round(op["plan"].flops / (t * 1e-3) / 1e9, 1),
-                                         "kernel": op["plan"].description} for op, t in zip(ops, per_op_ms)},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "device": props["name"] + " " + props["arch"],
+        "per_op": per_op, "roofline": roof, "cpu_baseline": cpu, "device": props["name"] + " " + props["arch"],
     }
     print(json.dumps(line), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--op-reps", type=int, default=20, help="flushed launches per op for the per-op medians")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="large workload: skip the CNN-triple per-op keys")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--schedule", default="serial", choices=("serial", "layer", "concurrent"),
+                    help="cnn-triple: serial (default) runs the three ops back to back on one stream; layer runs "
+                         "ProjectFrame, then BackProjectFrame and FilterAdjoint on two streams; concurrent forks all "
+                         "three")
+    ap.add_argument("--concurrent", action="store_true", help="same as --schedule concurrent")
+    ap.add_argument("--workload", default="large", choices=("large", "cnn-triple", "precision", "serving"))
+    ap.add_argument("--requests", type=int, default=3000, help="requests of the serving workload")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the heuristic kernel configuration instead of autotuning each op's descriptor")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuning cache file: loaded before planning, saved after (cached samples are not re-timed)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run three 64 MiB device-to-device copies (a known-size reference for HBM counters)")
+    ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="no GPU work: print HBM bytes per dispatch per kernel from two rocprofv3 counter CSVs "
+                         "(FETCH_SIZE, WRITE_SIZE) of a run made with --copy-calibration")
+    args = ap.parse_args()
+    if args.pmc_traffic:
+        print(json.dumps(pmc_traffic(*args.pmc_traffic), indent=1))
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    {"large": large_workload, "cnn-triple": cnn_triple_workload, "precision": precision_workload,
+     "serving": serving_workload}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -209,13 +209,14 @@ static inline float border_read(const float* a, int U_r, int U_c, int y, int x, 
 }
 
 /*
- * S(a, f) for one plane pair.  a is rows x cols, f is frows x fcols (both
- * before sampling).  upA/upF are the zero-insertion factors of a / f, dn the
- * output step.  out has oh x ow elements, double (prec 0) or float (prec 1).
+ * S(a, f) for one plane pair, output rows R0 <= R < R1.  a is rows x cols, f
+ * is frows x fcols (both before sampling).  upA/upF are the zero-insertion
+ * factors of a / f, dn the output step.  out receives (R1 - R0) x ow elements,
+ * double (prec 0) or float (prec 1).
  */
-void oracle_scalar(const float* a, int rows, int cols, const float* f, int frows, int fcols, int border, int orient,
-                   int upA_r, int upA_c, int upF_r, int upF_c, int dn_r, int dn_c, int oh, int ow, int prec,
-                   void* out) {
+static void scalar_rows(const float* a, int rows, int cols, const float* f, int frows, int fcols, int border,
+                        int orient, int upA_r, int upA_c, int upF_r, int upF_c, int dn_r, int dn_c, int R0, int R1,
+                        int ow, int prec, void* out) {
     const float* A = a;
     float* Aup = NULL;
     if (upA_r > 1 || upA_c > 1) { Aup = upsample_plane(a, rows, cols, upA_r, upA_c); A = Aup; }
@@ -234,7 +235,7 @@ void oracle_scalar(const float* a, int rows, int cols, const float* f, int frows
     if (border == OB_FULL) { h_r = F_r - 1; h_c = F_c - 1; }
     else if (border == OB_VALID) { h_r = 0; h_c = 0; }
     else { h_r = (F_r - 1) / 2; h_c = (F_c - 1) / 2; }
-    for (int R = 0; R < oh; R++) {
+    for (int R = R0; R < R1; R++) {
         for (int C = 0; C < ow; C++) {
             int y0 = R * dn_r - h_r, x0 = C * dn_c - h_c;
             if (prec == 0) {
@@ -245,7 +246,7 @@ void oracle_scalar(const float* a, int rows, int cols, const float* f, int frows
                         float v = border_read(A, U_r, U_c, y0 + i, x0 + j, border, &in);
                         s += (double)t[(size_t)i * F_c + j] * (double)v;
                     }
-                ((double*)out)[(size_t)R * ow + C] = s;
+                ((double*)out)[(size_t)(R - R0) * ow + C] = s;
             } else {
                 float rowSum = 0.0f;
                 for (int i = 0; i < F_r; i++) {
@@ -257,13 +258,20 @@ void oracle_scalar(const float* a, int rows, int cols, const float* f, int frows
                     }
                     rowSum += colSum;
                 }
-                ((float*)out)[(size_t)R * ow + C] = rowSum;
+                ((float*)out)[(size_t)(R - R0) * ow + C] = rowSum;
             }
         }
     }
     free(t);
     free(Aup);
     free(Fup);
+}
+
+void oracle_scalar(const float* a, int rows, int cols, const float* f, int frows, int fcols, int border, int orient,
+                   int upA_r, int upA_c, int upF_r, int upF_c, int dn_r, int dn_c, int oh, int ow, int prec,
+                   void* out) {
+    scalar_rows(a, rows, cols, f, frows, fcols, border, orient, upA_r, upA_c, upF_r, upF_c, dn_r, dn_c, 0, oh, ow,
+                prec, out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -334,33 +342,41 @@ int oracle_conv(const oracle_desc* d, const float* in0, const float* in1, void* 
     size_t inA = (size_t)d->in0_rows * d->in0_cols, inF = (size_t)d->in1_rows * d->in1_cols;
     size_t op = (size_t)ot.rows * ot.cols;
     int R = reduction_length(d);
+    /* Work items are (output plane, band of up to 8 output rows), so that a
+     * few output planes still spread over many threads; every element sums its
+     * reduction terms in the same order whatever the banding. */
+    const int band = 8, bands = (ot.rows + band - 1) / band;
+    const long items = (long)ot.planes * bands;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
-    for (int o = 0; o < ot.planes; o++) {
-        void* tmp = malloc(op * (prec == 0 ? sizeof(double) : sizeof(float)));
+    for (long it = 0; it < items; it++) {
+        const int o = (int)(it / bands), R0 = (int)(it % bands) * band;
+        const int R1 = R0 + band < ot.rows ? R0 + band : ot.rows;
+        const size_t n = (size_t)(R1 - R0) * ot.cols, off = (size_t)o * op + (size_t)R0 * ot.cols;
+        void* tmp = malloc(n * (prec == 0 ? sizeof(double) : sizeof(float)));
         if (prec == 0) {
-            double* acc = (double*)out + (size_t)o * op;
-            for (size_t q = 0; q < op; q++) acc[q] = 0.0;
+            double* acc = (double*)out + off;
+            for (size_t q = 0; q < n; q++) acc[q] = 0.0;
             for (int r = 0; r < R; r++) {
                 int pa, pf;
                 oracle_plane_pair(d, o, r, &pa, &pf);
-                oracle_scalar(in0 + (size_t)pa * inA, d->in0_rows, d->in0_cols, in1 + (size_t)pf * inF, d->in1_rows,
-                              d->in1_cols, d->border, d->orient, upA_r, upA_c, upF_r, upF_c, dn_r, dn_c, ot.rows,
-                              ot.cols, 0, tmp);
-                for (size_t q = 0; q < op; q++) acc[q] += ((double*)tmp)[q];
+                scalar_rows(in0 + (size_t)pa * inA, d->in0_rows, d->in0_cols, in1 + (size_t)pf * inF, d->in1_rows,
+                            d->in1_cols, d->border, d->orient, upA_r, upA_c, upF_r, upF_c, dn_r, dn_c, R0, R1,
+                            ot.cols, 0, tmp);
+                for (size_t q = 0; q < n; q++) acc[q] += ((double*)tmp)[q];
             }
         } else {
-            float* acc = (float*)out + (size_t)o * op;
-            for (size_t q = 0; q < op; q++) acc[q] = 0.0f;
+            float* acc = (float*)out + off;
+            for (size_t q = 0; q < n; q++) acc[q] = 0.0f;
             for (int r = 0; r < R; r++) {
                 int pa, pf;
                 oracle_plane_pair(d, o, r, &pa, &pf);
-                oracle_scalar(in0 + (size_t)pa * inA, d->in0_rows, d->in0_cols, in1 + (size_t)pf * inF, d->in1_rows,
-                              d->in1_cols, d->border, d->orient, upA_r, upA_c, upF_r, upF_c, dn_r, dn_c, ot.rows,
-                              ot.cols, 1, tmp);
-                for (size_t q = 0; q < op; q++) acc[q] += ((float*)tmp)[q];
+                scalar_rows(in0 + (size_t)pa * inA, d->in0_rows, d->in0_cols, in1 + (size_t)pf * inF, d->in1_rows,
+                            d->in1_cols, d->border, d->orient, upA_r, upA_c, upF_r, upF_c, dn_r, dn_c, R0, R1,
+                            ot.cols, 1, tmp);
+                for (size_t q = 0; q < n; q++) acc[q] += ((float*)tmp)[q];
             }
         }
         free(tmp);
