@@ -62,7 +62,7 @@ class DevicePropsC(ctypes.Structure):
 
 class ServerConfigC(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int32),
-                ("streams", ctypes.c_int32), ("max_stack", ctypes.c_int32)]
+                ("streams", ctypes.c_int32), ("max_stack", ctypes.c_int32), ("graph_after", ctypes.c_int32)]
 
 
 class RequestC(ctypes.Structure):
@@ -75,7 +75,8 @@ REQUEST_RESIDENT = 1
 
 class ServerStatsC(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
-                ("launches", ctypes.c_uint64), ("stacked", ctypes.c_uint64), ("p50_ms", ctypes.c_double),
+                ("launches", ctypes.c_uint64), ("stacked", ctypes.c_uint64), ("graph_launches", ctypes.c_uint64),
+                ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
                 ("first_submit_s", ctypes.c_double), ("last_complete_s", ctypes.c_double),
                 ("requests_per_s", ctypes.c_double), ("gflop", ctypes.c_double), ("peak_bytes", ctypes.c_uint64)]
@@ -89,6 +90,38 @@ class TuneOptionsC(ctypes.Structure):
 class TuneResultC(ctypes.Structure):
     _fields_ = [("variant", ctypes.c_int32), ("variants", ctypes.c_int32), ("avg_us", ctypes.c_double),
                 ("from_cache", ctypes.c_int32), ("description", ctypes.c_char * 192)]
+
+
+# Elementwise epilogue ops (cogconv.h COGCONV_EPI_*)
+(EPI_ADD_CONST, EPI_SUB_CONST, EPI_RSUB_CONST, EPI_MUL_CONST, EPI_DIV_CONST, EPI_RDIV_CONST, EPI_MAX_CONST,
+ EPI_MIN_CONST) = range(1, 9)
+EPI_NEG, EPI_ABS, EPI_SQ, EPI_SQRT, EPI_EXP, EPI_LOG, EPI_TANH, EPI_SIGNUM = range(16, 24)
+(EPI_ADD_FIELD, EPI_SUB_FIELD, EPI_RSUB_FIELD, EPI_MUL_FIELD, EPI_DIV_FIELD, EPI_RDIV_FIELD, EPI_MAX_FIELD,
+ EPI_MIN_FIELD) = range(32, 40)
+EPI_MAX_OPS, EPI_MAX_OPERANDS = 8, 4
+
+
+class EpilogueOpC(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int32), ("value", ctypes.c_float), ("operand", ctypes.c_int32)]
+
+
+class EpilogueC(ctypes.Structure):
+    _fields_ = [("n_ops", ctypes.c_int32), ("ops", EpilogueOpC * EPI_MAX_OPS), ("n_operands", ctypes.c_int32),
+                ("operands", FieldTypeC * EPI_MAX_OPERANDS)]
+
+
+def make_epilogue(ops, operand_types=()):
+    """ops: [(op, value, operand slot)]; operand_types: FieldTypeC or tuples, one per slot."""
+    if len(ops) > EPI_MAX_OPS or len(operand_types) > EPI_MAX_OPERANDS:
+        raise ValueError("epilogue too long")
+    e = EpilogueC()
+    e.n_ops = len(ops)
+    for i, (op, value, slot) in enumerate(ops):
+        e.ops[i] = EpilogueOpC(op, value, slot)
+    e.n_operands = len(operand_types)
+    for i, t in enumerate(operand_types):
+        e.operands[i] = t if isinstance(t, FieldTypeC) else field_type(*t)
+    return e
 
 
 _P = ctypes.c_void_p
@@ -135,6 +168,23 @@ SIGNATURES = {
     "cogconv_launch": [_P, _P, _P, _P, _P, _P],
     "cogconv_plan_pack_operand": [_P, _P, _P, _P, _P],
     "cogconv_launch_packed": [_P, _P, _P, _P, _P],
+    "cogconv_plan_table_supported": [_P, ctypes.POINTER(ctypes.c_int32)],
+    "cogconv_launch_table": [_P, _P, _P, _P, _P, ctypes.c_int32, _P],
+    "cogconv_plan_set_epilogue": [_P, ctypes.POINTER(EpilogueC)],
+    "cogconv_launch_epilogue": [_P, _P, _P, _P, _P, ctypes.POINTER(_P), _P],
+    "cogconv_elementwise": [_P, _P, ctypes.POINTER(FieldTypeC), ctypes.POINTER(EpilogueC), ctypes.POINTER(_P), _P],
+    "cogconv_block_reduce_sum_epilogue": [_P, _P, ctypes.POINTER(FieldTypeC), ctypes.c_int32,
+                                          ctypes.POINTER(EpilogueC), ctypes.POINTER(_P), _P],
+    "cogconv_function_create": [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(_P)],
+    "cogconv_function_destroy": [_P],
+    "cogconv_function_name": [_P, ctypes.c_char_p, _SZ],
+    "cogconv_function_optimal_thread_count": [_P, ctypes.POINTER(ctypes.c_int32)],
+    "cogconv_function_input_count": [_P, ctypes.POINTER(ctypes.c_int32)],
+    "cogconv_function_output_count": [_P, ctypes.POINTER(ctypes.c_int32)],
+    "cogconv_function_io": [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_char_p, _SZ, ctypes.POINTER(FieldTypeC)],
+    "cogconv_function_evaluate": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)],
+    "cogconv_separable": [_P, ctypes.POINTER(FieldTypeC), _P, ctypes.c_int32, _P, ctypes.c_int32, ctypes.c_int32,
+                          ctypes.c_int32, _P, ctypes.POINTER(EpilogueC), ctypes.POINTER(_P), _P],
     "cogconv_block_reduce_sum": [_P, _P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _P],
     "cogconv_plan_variant_count": [ctypes.POINTER(DescC), ctypes.POINTER(ctypes.c_int32)],
     "cogconv_plan_create_variant": [ctypes.POINTER(DescC), ctypes.c_int32, ctypes.POINTER(_P)],
@@ -237,7 +287,8 @@ class DeviceBuffer:
         return out
 
     def zero(self, stream=None):
-        check(lib().cogconv_fill_zero(self.ptr, self.nbytes, stream))
+        s = stream.handle if isinstance(stream, Stream) else stream
+        check(lib().cogconv_fill_zero(self.ptr, self.nbytes, s))
 
     def close(self):
         if getattr(self, "ptr", None):
@@ -385,6 +436,36 @@ class Plan:
         s = stream.handle if isinstance(stream, Stream) else stream
         check(lib().cogconv_launch_packed(self.handle, ptr(in0), ptr(out), ptr(workspace), s))
 
+    def set_epilogue(self, epilogue):
+        """cogconv_plan_set_epilogue (an EpilogueC, or None to remove it)."""
+        check(lib().cogconv_plan_set_epilogue(self.handle, ctypes.byref(epilogue) if epilogue is not None else None))
+        self.epilogue_operands = epilogue.n_operands if epilogue is not None else 0
+
+    def launch_epilogue(self, in0, in1, out, workspace, operands, stream=None):
+        """cogconv_launch_epilogue: operands are the device buffers of the epilogue's field slots."""
+        def ptr(x):
+            return x.ptr if isinstance(x, DeviceBuffer) else x
+        s = stream.handle if isinstance(stream, Stream) else stream
+        arr = (_P * max(1, len(operands)))(*[ptr(o) for o in operands])
+        check(lib().cogconv_launch_epilogue(self.handle, ptr(in0), ptr(in1), ptr(out), ptr(workspace), arr, s))
+
+    def table_supported(self):
+        v = ctypes.c_int32()
+        check(lib().cogconv_plan_table_supported(self.handle, ctypes.byref(v)))
+        return bool(v.value)
+
+    def launch_table(self, pairs, device_table, in1=None, workspace=None, packed=False, stream=None):
+        """cogconv_launch_table: pairs = [(in0, out)] one per image of the plan's batch
+        (DeviceBuffers or device addresses); device_table: a DeviceBuffer of 16 bytes a pair."""
+        def ptr(x):
+            return x.ptr if isinstance(x, DeviceBuffer) else x
+        host = (ctypes.c_void_p * (2 * len(pairs)))(*[ptr(v) for pr in pairs for v in pr])
+        s = stream.handle if isinstance(stream, Stream) else stream
+        check(lib().cogconv_launch_table(self.handle, host, ptr(device_table), ptr(in1), ptr(workspace),
+                                         1 if packed else 0, s))
+        if stream is None:
+            synchronize()  # the pageable host table must outlive the copy
+
     def close(self):
         if getattr(self, "handle", None):
             lib().cogconv_plan_destroy(self.handle)
@@ -426,11 +507,121 @@ def tune_cache_load(path):
     check(lib().cogconv_tune_cache_load(os.fsencode(path)))
 
 
-def block_reduce_sum(inp, out, points, in_planes, factor, stream=None):
+def _ptr(x):
+    return x.ptr if isinstance(x, DeviceBuffer) else x
+
+
+def _stream(stream):
+    return stream.handle if isinstance(stream, Stream) else stream
+
+
+def _ptr_array(operands):
+    return (_P * max(1, len(operands)))(*[_ptr(o) for o in operands])
+
+
+def block_reduce_sum(inp, out, points, in_planes, factor, stream=None, epilogue=None, operands=(), in_type=None):
+    """cogconv_block_reduce_sum, or cogconv_block_reduce_sum_epilogue with an EpilogueC
+    (`in_type`: the input's FieldType, whose field shape the epilogue's operands follow)."""
+    if epilogue is None:
+        check(lib().cogconv_block_reduce_sum(_ptr(inp), _ptr(out), points, in_planes, factor, _stream(stream)))
+        return
+    t = in_type if isinstance(in_type, FieldTypeC) else field_type(*(in_type or (2, 1, int(points), int(in_planes))))
+    check(lib().cogconv_block_reduce_sum_epilogue(_ptr(inp), _ptr(out), ctypes.byref(t), factor,
+                                                  ctypes.byref(epilogue), _ptr_array(operands), _stream(stream)))
+
+
+class SeparablePlan:
+    """cogconv_separable on a fixed field type, filter lengths, border and orientation."""
+
+    def __init__(self, ftype, row_taps, column_taps, border, orientation):
+        self.ftype = ftype if isinstance(ftype, FieldTypeC) else field_type(*ftype)
+        self.row_taps, self.column_taps = int(row_taps), int(column_taps)
+        self.border, self.orientation = border, orientation
+        self.epilogue = None
+
+    def set_epilogue(self, epilogue):
+        self.epilogue = epilogue
+
+    def launch(self, inp, row_filter, column_filter, out, operands=(), stream=None):
+        check(lib().cogconv_separable(_ptr(inp), ctypes.byref(self.ftype), _ptr(row_filter), self.row_taps,
+                                      _ptr(column_filter), self.column_taps, self.border, self.orientation, _ptr(out),
+                                      ctypes.byref(self.epilogue) if self.epilogue is not None else None,
+                                      _ptr_array(operands), _stream(stream)))
+
+
+def elementwise(inp, out, ftype, epilogue, operands=(), stream=None):
+    """cogconv_elementwise: out = epilogue(inp) over a field of type `ftype`."""
     def ptr(x):
         return x.ptr if isinstance(x, DeviceBuffer) else x
     s = stream.handle if isinstance(stream, Stream) else stream
-    check(lib().cogconv_block_reduce_sum(ptr(inp), ptr(out), points, in_planes, factor, s))
+    t = ftype if isinstance(ftype, FieldTypeC) else field_type(*ftype)
+    arr = (_P * max(1, len(operands)))(*[ptr(o) for o in operands])
+    check(lib().cogconv_elementwise(ptr(inp), ptr(out), ctypes.byref(t), ctypes.byref(epilogue), arr, s))
+
+
+class Function:
+    """A compiled graph restored from a file and run through the reentrant
+    evaluate() Function Interface (cogconv_function_*)."""
+
+    def __init__(self, path, device=0):
+        p = ctypes.c_void_p()
+        check(lib().cogconv_function_create(str(path).encode(), device, ctypes.byref(p)))
+        self.handle = p.value
+        buf = ctypes.create_string_buffer(256)
+        check(lib().cogconv_function_name(self.handle, buf, 256))
+        self.name = buf.value.decode()
+        self.inputs = self._io(0)
+        self.outputs = self._io(1)
+
+    def _io(self, output):
+        n = ctypes.c_int32()
+        fn = lib().cogconv_function_output_count if output else lib().cogconv_function_input_count
+        check(fn(self.handle, ctypes.byref(n)))
+        res = []
+        for i in range(n.value):
+            buf = ctypes.create_string_buffer(256)
+            t = FieldTypeC()
+            check(lib().cogconv_function_io(self.handle, output, i, buf, 256, ctypes.byref(t)))
+            res.append((buf.value.decode(), t.as_tuple()))
+        return res
+
+    @staticmethod
+    def _shape(t):
+        d, r, c, p, o = t
+        spatial = (r, c) if d == 2 else ((c,) if d == 1 else ())
+        return spatial if (o == 0 and p == 1) else (p,) + spatial
+
+    def optimal_thread_count(self):
+        n = ctypes.c_int32()
+        check(lib().cogconv_function_optimal_thread_count(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def evaluate(self, inputs):
+        """inputs: arrays in input order (planar float32); returns the outputs in order."""
+        if len(inputs) != len(self.inputs):
+            raise ValueError("expected %d inputs, got %d" % (len(self.inputs), len(inputs)))
+        ins = []
+        for a, (name, t) in zip(inputs, self.inputs):
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            if a.size != t[1] * t[2] * t[3]:
+                raise ValueError("input %s has %d values, its field type holds %d" % (name, a.size, t[1] * t[2] * t[3]))
+            ins.append(a)
+        outs = [np.empty(self._shape(t), np.float32) for _, t in self.outputs]
+        ip = (_P * max(1, len(ins)))(*[a.ctypes.data for a in ins])
+        op = (_P * max(1, len(outs)))(*[o.ctypes.data for o in outs])
+        check(lib().cogconv_function_evaluate(self.handle, ip, op))
+        return outs
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().cogconv_function_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HostBuffer:
@@ -460,8 +651,8 @@ class HostBuffer:
 class Server:
     """The serving loop of libcogconv (cogconv_server_*)."""
 
-    def __init__(self, budget_bytes=1 << 30, max_inflight=8, streams=4, max_stack=8, device=0):
-        cfg = ServerConfigC(device, budget_bytes, max_inflight, streams, max_stack)
+    def __init__(self, budget_bytes=1 << 30, max_inflight=8, streams=4, max_stack=8, device=0, graph_after=4):
+        cfg = ServerConfigC(device, budget_bytes, max_inflight, streams, max_stack, graph_after)
         p = ctypes.c_void_p()
         check(lib().cogconv_server_create(ctypes.byref(cfg), ctypes.byref(p)))
         self.handle = p.value

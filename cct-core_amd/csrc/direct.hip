@@ -8,6 +8,7 @@
 // the 2nd operand, output of filter size).  The kernels here are written for
 // gfx950: 64-wide wavefronts, one wavefront per 64 output columns, the input
 // tile and filter chunk staged in LDS, and wavefront shuffles for reductions.
+#include "epi.h"
 #include "internal.h"
 #include "stage.h"
 
@@ -67,7 +68,7 @@ __device__ __forceinline__ float read_tap(const float* __restrict__ f, const Geo
 // staged in LDS.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_direct(Geom g, PlaneMap m, DirectCfg c, const float* __restrict__ A,
-                                                const float* __restrict__ F, float* __restrict__ O) {
+                                                const float* __restrict__ F, float* __restrict__ O, Epi e) {
     extern __shared__ float smem[];
     float* st = smem;
     float* sf = smem + (size_t)c.ldsRows * c.ldsCols;
@@ -128,18 +129,18 @@ __global__ __launch_bounds__(256) void k_direct(Geom g, PlaneMap m, DirectCfg c,
     if (col < g.OW) {
         float* out = O + (size_t)o * g.OH * g.OW + col;
         const int row = orow0 + rg * 4;
-        if (row + 0 < g.OH) out[(size_t)(row + 0) * g.OW] = acc0;
-        if (row + 1 < g.OH) out[(size_t)(row + 1) * g.OW] = acc1;
-        if (row + 2 < g.OH) out[(size_t)(row + 2) * g.OW] = acc2;
-        if (row + 3 < g.OH) out[(size_t)(row + 3) * g.OW] = acc3;
+        if (row + 0 < g.OH) out[(size_t)(row + 0) * g.OW] = epi(e, acc0, o, (row + 0) * g.OW + col);
+        if (row + 1 < g.OH) out[(size_t)(row + 1) * g.OW] = epi(e, acc1, o, (row + 1) * g.OW + col);
+        if (row + 2 < g.OH) out[(size_t)(row + 2) * g.OW] = epi(e, acc2, o, (row + 2) * g.OW + col);
+        if (row + 3 < g.OH) out[(size_t)(row + 3) * g.OW] = epi(e, acc3, o, (row + 3) * g.OW + col);
     }
 }
 
 hipError_t launch_direct(const Geom& g, const PlaneMap& m, const DirectCfg& c, const float* a, const float* f,
-                         float* out, hipStream_t s) {
+                         float* out, hipStream_t s, const Epi& e) {
     const long blocks = (long)c.tilesC * c.tilesR * m.nOut;
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(k_direct, dim3((unsigned)blocks), dim3(64 * c.RG), c.lds_bytes, s, g, m, c, a, f, out);
+    hipLaunchKernelGGL(k_direct, dim3((unsigned)blocks), dim3(64 * c.RG), c.lds_bytes, s, g, m, c, a, f, out, e);
     return hipGetLastError();
 }
 
@@ -177,7 +178,7 @@ __host__ __device__ inline int rt_tap_pitch(int fw) { return (fw + 3) & ~3; }
 
 template <int FW, int RPT>
 __global__ __launch_bounds__(256) void k_direct_rt(Geom g, PlaneMap m, DirectCfg c, const float* __restrict__ A,
-                                                   const float* __restrict__ F, float* __restrict__ O) {
+                                                   const float* __restrict__ F, float* __restrict__ O, Epi e) {
     extern __shared__ float4 smem4[];
     float* smem = reinterpret_cast<float*>(smem4);
     constexpr int ROWS = 8 * RPT;         // output rows per tile
@@ -312,9 +313,11 @@ __global__ __launch_bounds__(256) void k_direct_rt(Geom g, PlaneMap m, DirectCfg
 #pragma unroll
         for (int a = 0; a < RPT; a++) {
             const int row = orow0 + RPT * rg + a;
+            const int pt = row * g.OW + col;
             if (row < g.OH && col < g.OW)
                 *reinterpret_cast<float4*>(out + (size_t)row * g.OW + col) =
-                    make_float4(acc[a][0], acc[a][1], acc[a][2], acc[a][3]);
+                    make_float4(epi(e, acc[a][0], o, pt), epi(e, acc[a][1], o, pt + 1), epi(e, acc[a][2], o, pt + 2),
+                                epi(e, acc[a][3], o, pt + 3));
         }
     } else {
 #pragma unroll
@@ -323,7 +326,7 @@ __global__ __launch_bounds__(256) void k_direct_rt(Geom g, PlaneMap m, DirectCfg
             if (row >= g.OH) continue;
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                if (col + q < g.OW) out[(size_t)row * g.OW + col + q] = acc[a][q];
+                if (col + q < g.OW) out[(size_t)row * g.OW + col + q] = epi(e, acc[a][q], o, row * g.OW + col + q);
         }
     }
 }
@@ -351,11 +354,11 @@ void direct_rt_config(const Geom& g, int rpt, DirectCfg& c) {
 
 template <int RPT>
 static hipError_t launch_rt(const Geom& g, const PlaneMap& m, const DirectCfg& c, const float* a, const float* f,
-                            float* out, hipStream_t s, dim3 grid) {
+                            float* out, hipStream_t s, dim3 grid, const Epi& e) {
     const dim3 block(256);
     switch (g.FW) {
 #define RT_CASE(W) \
-    case W: hipLaunchKernelGGL((k_direct_rt<W, RPT>), grid, block, c.lds_bytes, s, g, m, c, a, f, out); break;
+    case W: hipLaunchKernelGGL((k_direct_rt<W, RPT>), grid, block, c.lds_bytes, s, g, m, c, a, f, out, e); break;
         RT_CASE(1) RT_CASE(2) RT_CASE(3) RT_CASE(4) RT_CASE(5) RT_CASE(6) RT_CASE(7) RT_CASE(8) RT_CASE(9)
         RT_CASE(10) RT_CASE(11) RT_CASE(12) RT_CASE(13) RT_CASE(14) RT_CASE(15)
 #undef RT_CASE
@@ -365,14 +368,14 @@ static hipError_t launch_rt(const Geom& g, const PlaneMap& m, const DirectCfg& c
 }
 
 hipError_t launch_direct_rt(const Geom& g, const PlaneMap& m, const DirectCfg& c, const float* a, const float* f,
-                            float* out, hipStream_t s) {
+                            float* out, hipStream_t s, const Epi& e) {
     const long blocks = (long)c.tilesC * c.tilesR * m.nOut;
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     if (!direct_rt_supported(g) || c.ldsRows != 8 * c.RG + g.FH - 1) return hipErrorInvalidValue;
     const dim3 grid((unsigned)blocks);
-    if (c.RG == 4) return launch_rt<4>(g, m, c, a, f, out, s, grid);
-    if (c.RG == 2) return launch_rt<2>(g, m, c, a, f, out, s, grid);
-    if (c.RG == 1) return launch_rt<1>(g, m, c, a, f, out, s, grid);
+    if (c.RG == 4) return launch_rt<4>(g, m, c, a, f, out, s, grid, e);
+    if (c.RG == 2) return launch_rt<2>(g, m, c, a, f, out, s, grid, e);
+    if (c.RG == 1) return launch_rt<1>(g, m, c, a, f, out, s, grid, e);
     return hipErrorInvalidValue;
 }
 
@@ -438,16 +441,18 @@ __global__ __launch_bounds__(256) void k_small_output(Geom g, PlaneMap m, SmallC
     }
 }
 
-__global__ void k_sum_splits(const float* __restrict__ partial, float* __restrict__ out, int splits, long total) {
+__global__ void k_sum_splits(const float* __restrict__ partial, float* __restrict__ out, int splits, long total,
+                             long points, Epi e) {
     for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
         float t = 0.f;
         for (int s = 0; s < splits; s++) t += partial[(size_t)s * total + idx];
-        out[idx] = t;
+        const long p = idx / points;
+        out[idx] = epi(e, t, (int)p, (int)(idx - p * points));
     }
 }
 
 hipError_t launch_small_output(const Geom& g, const PlaneMap& m, const SmallCfg& c, const float* a, const float* f,
-                               float* out, float* workspace, hipStream_t s) {
+                               float* out, float* workspace, hipStream_t s, const Epi& ep) {
     const long gx = (long)m.nOut * c.splits;
     if (gx <= 0 || gx > 0x7fffffffL || c.qchunks > 65535) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(k_small_output, dim3((unsigned)gx, c.qchunks), dim3(256), 0, s, g, m, c, a, f, workspace);
@@ -455,7 +460,8 @@ hipError_t launch_small_output(const Geom& g, const PlaneMap& m, const SmallCfg&
     if (e != hipSuccess) return e;
     const long total = (long)m.nOut * g.OH * g.OW;
     const long blocks = std::min((total + 255) / 256, 65536L);
-    hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)blocks), dim3(256), 0, s, workspace, out, c.splits, total);
+    hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)blocks), dim3(256), 0, s, workspace, out, c.splits, total,
+                       (long)g.OH * g.OW, ep);
     return hipGetLastError();
 }
 
@@ -465,23 +471,40 @@ hipError_t launch_small_output(const Geom& g, const PlaneMap& m, const SmallCfg&
 // take consecutive pixels, so reads and writes are coalesced.
 // ---------------------------------------------------------------------------
 __global__ void k_block_reduce_sum(const float* __restrict__ in, float* __restrict__ out, long points, int outPlanes,
-                                   int factor) {
+                                   int factor, Epi e) {
     const long total = points * outPlanes;
     for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
         const long p = idx / points, q = idx - p * points;
         const float* src = in + (size_t)p * factor * points + q;
         float t = 0.f;
         for (int j = 0; j < factor; j++) t += src[(size_t)j * points];
-        out[idx] = t;
+        out[idx] = epi(e, t, (int)p, (int)q);
     }
 }
 
 hipError_t launch_block_reduce_sum(const float* in, float* out, int64_t points, int out_planes, int factor,
-                                   hipStream_t s) {
+                                   hipStream_t s, const Epi& e) {
     const long total = (long)points * out_planes;
     const long blocks = std::min((total + 255) / 256, 65536L);
     hipLaunchKernelGGL(k_block_reduce_sum, dim3((unsigned)blocks), dim3(256), 0, s, in, out, (long)points, out_planes,
-                       factor);
+                       factor, e);
+    return hipGetLastError();
+}
+
+// The elementwise pass of an epilogue over a stored field (in == out allowed).
+__global__ void k_elementwise(const float* in, float* out, long points, int planes, Epi e) {
+    const long total = points * planes;
+    for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+        const long p = idx / points;
+        out[idx] = epi(e, in[idx], (int)p, (int)(idx - p * points));
+    }
+}
+
+hipError_t launch_elementwise(const float* in, float* out, int64_t points, int planes, const Epi& e, hipStream_t s) {
+    const long total = (long)points * planes;
+    if (total <= 0) return hipSuccess;
+    const long blocks = std::min((total + 255) / 256, 65536L);
+    hipLaunchKernelGGL(k_elementwise, dim3((unsigned)blocks), dim3(256), 0, s, in, out, (long)points, planes, e);
     return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@
 // waves' sums meet in LDS.  Blocks split the (image, tap-row chunk) space; each writes one partial
 // dW, and the partials are added in a fixed order by a two-level reduction, so
 // results do not depend on scheduling.
+#include "epi.h"
 #include "stage.h"
 
 namespace cogconv {
@@ -345,7 +346,7 @@ constexpr int RED_W = 16;  // waves per reduction block
 
 template <int OPB>
 __global__ __launch_bounds__(64 * RED_W) void k_reduce_partials(const float* __restrict__ ws, float* __restrict__ out,
-                                                                  int splits, long total) {
+                                                                  int splits, long total, long points, Epi e) {
     constexpr int G = RED_W * 64 / OPB;
     __shared__ float part[G][OPB];
     const int o = threadIdx.x % OPB;
@@ -371,21 +372,23 @@ __global__ __launch_bounds__(64 * RED_W) void k_reduce_partials(const float* __r
         float t = part[0][o];
 #pragma unroll
         for (int k = 1; k < G; k++) t += part[k][o];
-        out[idx] = t;
+        const long p = idx / points;
+        out[idx] = epi(e, t, (int)p, (int)(idx - p * points));
     }
 }
 
-hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long total, hipStream_t s) {
+hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long total, long points, hipStream_t s,
+                                  const Epi& e) {
     if (total <= 0) return hipErrorInvalidConfiguration;
     if (total < 64L * 256) {
         const long blocks = (total + 31) / 32;
         hipLaunchKernelGGL(k_reduce_partials<32>, dim3((unsigned)blocks), dim3(64 * RED_W), 0, s, ws, out, splits,
-                           total);
+                           total, points, e);
     } else {
         const long blocks = (total + 63) / 64;
         if (blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
         hipLaunchKernelGGL(k_reduce_partials<64>, dim3((unsigned)blocks), dim3(64 * RED_W), 0, s, ws, out, splits,
-                           total);
+                           total, points, e);
     }
     return hipGetLastError();
 }
@@ -410,7 +413,7 @@ static hipError_t launch_mb(const Geom& g, const PlaneMap& m, const FaCfg& c, co
 }
 
 hipError_t launch_fa_igemm_f32(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
-                               float* out, float* ws, hipStream_t s) {
+                               float* out, float* ws, hipStream_t s, const Epi& ep) {
     const long blocks = (long)c.colGroups * c.splits;
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)blocks);
@@ -423,7 +426,7 @@ hipError_t launch_fa_igemm_f32(const Geom& g, const PlaneMap& m, const FaCfg& c,
     }
     if (e != hipSuccess) return e;
     const long total = (long)c.K * m.N * g.OH * g.OW;
-    return launch_reduce_partials(ws, out, c.splits, total, s);
+    return launch_reduce_partials(ws, out, c.splits, total, (long)g.OH * g.OW, s, ep);
 }
 
 }  // namespace cogconv

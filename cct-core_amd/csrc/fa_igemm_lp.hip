@@ -26,7 +26,6 @@
 
 namespace cogconv {
 
-hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long total, hipStream_t s);
 
 // Lane byte offset with bit 31 set: past every staged buffer (each is under
 // 2^31 bytes), so the buffer unit's range check answers the load with 0.
@@ -421,7 +420,7 @@ static hipError_t fa_lp_launch1(const Geom& g, const PlaneMap& m, const FaCfg& c
 }
 
 hipError_t launch_fa_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x,
-                              const float* dy, float* out, float* ws, hipStream_t s) {
+                              const float* dy, float* out, float* ws, hipStream_t s, const Epi& ep) {
     const long blocks = (long)c.colGroups * c.splits;
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)blocks);
@@ -434,7 +433,7 @@ hipError_t launch_fa_igemm_lp(int precision, const Geom& g, const PlaneMap& m, c
     }
     if (e != hipSuccess) return e;
     const long total = (long)c.K * m.N * g.OH * g.OW;
-    return launch_reduce_partials(ws, out, c.splits, total, s);
+    return launch_reduce_partials(ws, out, c.splits, total, (long)g.OH * g.OW, s, ep);
 }
 
 }  // namespace cogconv

@@ -21,6 +21,7 @@
 // filter slice [RC][F*F][16*MB] (each thread copies the F*F contiguous taps of
 // one (plane, filter) pair).  Both staged plane strides are 16 mod 32 words so
 // that the four k-lane groups of an MFMA read different LDS banks.
+#include "epi.h"
 #include "stage.h"
 
 namespace cogconv {
@@ -29,7 +30,8 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 template <int FR, int FC, int MB, bool PH>
 __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg c, Phase ph, const float* __restrict__ X,
-                                                   const float* __restrict__ Wt, float* __restrict__ O) {
+                                                   const float* __restrict__ Wt, float* __restrict__ O,
+                                                   const cogconv_image_io* __restrict__ tab, Epi e) {
     constexpr int TAPS = FR * FC;
     const int TAPSF = ph.FHf * ph.FWf;  // taps of a full filter in the bank
     constexpr int MT = 16 * MB;
@@ -60,6 +62,13 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
     const bool bpf = m.mode == COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM;
     const int L = m.T / m.N;
     const int HW = g.H * g.W;                       // 32-bit offsets: the plan checks the sizes
+    if (tab) {
+        // Request table: image b's planes and output planes are entry b's buffers.
+        // X and O are re-based so that the image-b offsets below land on them.
+        X = reinterpret_cast<const float*>(reinterpret_cast<uintptr_t>(tab[b].in0) - (uintptr_t)b * m.N * HW * 4);
+        O = reinterpret_cast<float*>(reinterpret_cast<uintptr_t>(tab[b].out) -
+                                     (uintptr_t)b * Mout * ph.OHf * ph.OWf * 4);
+    }
     const float* Xb = X + (size_t)b * m.N * HW;
     const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(Xb, (uint32_t)m.N * HW * 4u);  // the plan bounds this by 2^31
     // Per-lane column offsets of the staged tile (two 64-lane passes at most)
@@ -243,6 +252,20 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
                                                    ph.oCol0 + ocol) = make_float4(sum[0], sum[1], sum[2], sum[3]);
                 }
             }
+            // The epilogue rewrites the thread's own stores (still in L2), so the
+            // accumulators above stay in registers whatever its code size.
+            if (e.n) {
+                for (int t = 0; t < MB * 4; t++) {
+                    const int mb = t >> 2, pb = t & 3;
+                    const int mm = m0 + mb * 16 + l16;
+                    const int orow = tr * 8 + 2 * rw + (pb >> 1);
+                    const int ocol = tc * 32 + (pb & 1) * 16 + 4 * g4;
+                    if (mm < Mout && orow < g.OH && ocol < g.OW)
+                        epi_rewrite4(e, O + ((size_t)(b * Mout + mm) * ph.OHf + ph.oRow0 + orow) * ph.OWf + ph.oCol0 +
+                                            ocol,
+                                     b * Mout + mm, (ph.oRow0 + orow) * ph.OWf + ph.oCol0 + ocol);
+                }
+            }
         }
         return;
     }
@@ -272,9 +295,12 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
         for (int item = tid >> 4; item < MT * 8; item += 32) {
             const int mm = item >> 3, row = item & 7;
             const int orow = tr * 8 + row;
-            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
+            if (m0 + mm < Mout && orow < g.OH && ocol < g.OW) {
+                const float2 v = *reinterpret_cast<const float2*>(sO + mm * OP + row * 32 + c2);
+                const int pt = (ph.oRow0 + orow) * ph.OWf + ph.oCol0 + ocol, pl = b * Mout + m0 + mm;
                 *reinterpret_cast<float2*>(O + ((size_t)(b * Mout + m0 + mm) * ph.OHf + ph.oRow0 + orow) * ph.OWf +
-                                           ph.oCol0 + ocol) = *reinterpret_cast<const float2*>(sO + mm * OP + row * 32 + c2);
+                                           ph.oCol0 + ocol) = make_float2(epi(e, v.x, pl, pt), epi(e, v.y, pl, pt + 1));
+            }
         }
     } else {
         const int col = tid & 31;
@@ -284,7 +310,8 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
             const int orow = tr * 8 + row;
             if (m0 + mm < Mout && orow < g.OH && ocol < g.OW)
                 O[((size_t)(b * Mout + m0 + mm) * ph.OHf + ph.oRow0 + ph.oStride * orow) * ph.OWf + ph.oCol0 +
-                  ph.oStride * ocol] = sO[mm * OP + row * 32 + col];
+                  ph.oStride * ocol] = epi(e, sO[mm * OP + row * 32 + col], b * Mout + m0 + mm,
+                                           (ph.oRow0 + ph.oStride * orow) * ph.OWf + ph.oCol0 + ph.oStride * ocol);
         }
     }
 }
@@ -295,11 +322,12 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
 // registers in every launch.
 template <int FR, int FC, bool PH>
 static hipError_t launch_rc(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const Phase& ph, const float* x,
-                            const float* w, float* out, hipStream_t s, dim3 grid) {
+                            const float* w, float* out, hipStream_t s, dim3 grid, const cogconv_image_io* tab,
+                            const Epi& e) {
     switch (c.MB) {
-    case 1: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 1, PH>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
-    case 2: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 2, PH>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
-    default: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 4, PH>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out); break;
+    case 1: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 1, PH>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out, tab, e); break;
+    case 2: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 2, PH>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out, tab, e); break;
+    default: hipLaunchKernelGGL((k_igemm_f32<FR, FC, 4, PH>), grid, dim3(512), c.lds_bytes, s, g, m, c, ph, x, w, out, tab, e); break;
     }
     return hipGetLastError();
 }
@@ -318,43 +346,44 @@ bool igemm_f32_supported(int F) { return igemm_f32_supported_rc(F, F); }
 size_t igemm_f32_exchange_floats(int MB) { return (size_t)16 * MB * 260; }  // >= the [4 waves][MB][4][64] float4 exchange
 
 hipError_t launch_igemm_f32_phase(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const Phase& ph, const float* x,
-                                  const float* w, float* out, hipStream_t s) {
+                                  const float* w, float* out, hipStream_t s, const Epi& e,
+                                  const cogconv_image_io* tab) {
     const long blocks = (long)c.tilesC * c.tilesR * c.mTiles * m.B;
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     const dim3 grid((unsigned)blocks);
     const int key = g.FH * 10 + g.FW;
     if (ph.step == 1) {
         switch (key) {
-        case 11: return launch_rc<1, 1, false>(g, m, c, ph, x, w, out, s, grid);
-        case 33: return launch_rc<3, 3, false>(g, m, c, ph, x, w, out, s, grid);
-        case 55: return launch_rc<5, 5, false>(g, m, c, ph, x, w, out, s, grid);
-        case 77: return launch_rc<7, 7, false>(g, m, c, ph, x, w, out, s, grid);
-        case 99: return launch_rc<9, 9, false>(g, m, c, ph, x, w, out, s, grid);
-        case 22: return launch_rc<2, 2, false>(g, m, c, ph, x, w, out, s, grid);
-        case 44: return launch_rc<4, 4, false>(g, m, c, ph, x, w, out, s, grid);
+        case 11: return launch_rc<1, 1, false>(g, m, c, ph, x, w, out, s, grid, tab, e);
+        case 33: return launch_rc<3, 3, false>(g, m, c, ph, x, w, out, s, grid, tab, e);
+        case 55: return launch_rc<5, 5, false>(g, m, c, ph, x, w, out, s, grid, tab, e);
+        case 77: return launch_rc<7, 7, false>(g, m, c, ph, x, w, out, s, grid, tab, e);
+        case 99: return launch_rc<9, 9, false>(g, m, c, ph, x, w, out, s, grid, tab, e);
+        case 22: return launch_rc<2, 2, false>(g, m, c, ph, x, w, out, s, grid, tab, e);
+        case 44: return launch_rc<4, 4, false>(g, m, c, ph, x, w, out, s, grid, tab, e);
         default: return hipErrorInvalidValue;
         }
     }
     switch (key) {
-    case 11: return launch_rc<1, 1, true>(g, m, c, ph, x, w, out, s, grid);
-    case 22: return launch_rc<2, 2, true>(g, m, c, ph, x, w, out, s, grid);
-    case 33: return launch_rc<3, 3, true>(g, m, c, ph, x, w, out, s, grid);
-    case 44: return launch_rc<4, 4, true>(g, m, c, ph, x, w, out, s, grid);
-    case 55: return launch_rc<5, 5, true>(g, m, c, ph, x, w, out, s, grid);
-    case 12: return launch_rc<1, 2, true>(g, m, c, ph, x, w, out, s, grid);
-    case 21: return launch_rc<2, 1, true>(g, m, c, ph, x, w, out, s, grid);
-    case 23: return launch_rc<2, 3, true>(g, m, c, ph, x, w, out, s, grid);
-    case 32: return launch_rc<3, 2, true>(g, m, c, ph, x, w, out, s, grid);
-    case 34: return launch_rc<3, 4, true>(g, m, c, ph, x, w, out, s, grid);
-    case 43: return launch_rc<4, 3, true>(g, m, c, ph, x, w, out, s, grid);
-    case 45: return launch_rc<4, 5, true>(g, m, c, ph, x, w, out, s, grid);
-    case 54: return launch_rc<5, 4, true>(g, m, c, ph, x, w, out, s, grid);
+    case 11: return launch_rc<1, 1, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 22: return launch_rc<2, 2, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 33: return launch_rc<3, 3, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 44: return launch_rc<4, 4, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 55: return launch_rc<5, 5, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 12: return launch_rc<1, 2, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 21: return launch_rc<2, 1, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 23: return launch_rc<2, 3, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 32: return launch_rc<3, 2, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 34: return launch_rc<3, 4, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 43: return launch_rc<4, 3, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 45: return launch_rc<4, 5, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
+    case 54: return launch_rc<5, 4, true>(g, m, c, ph, x, w, out, s, grid, tab, e);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
-                            float* out, hipStream_t s) {
+                            float* out, hipStream_t s, const Epi& e, const cogconv_image_io* tab) {
     Phase ph;
     ph.i0 = 0;
     ph.j0 = 0;
@@ -367,7 +396,7 @@ hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c,
     ph.oStride = 1;
     ph.OHf = g.OH;
     ph.OWf = g.OW;
-    return launch_igemm_f32_phase(g, m, c, ph, x, w, out, s);
+    return launch_igemm_f32_phase(g, m, c, ph, x, w, out, s, e, tab);
 }
 
 }  // namespace cogconv

@@ -24,6 +24,7 @@
 // filter); x*w ~ xh*wh + xh*wl + xl*wh, each term on the MFMA, all into the
 // same FP32 accumulator.  BF16 has FP32's exponent range, so the split needs
 // no rescaling except for values below ~2^-118 whose low part underflows.
+#include "epi.h"
 #include "lp.h"
 #include "stage.h"
 
@@ -65,7 +66,8 @@ __global__ void k_pack_filters_lp(Geom g, PlaneMap m, int Mout, int Mpad, int Rp
 template <int F, int MB, int PREC>
 __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c, const float* __restrict__ X,
                                                   const typename LpType<PREC>::T* __restrict__ Wp,
-                                                  float* __restrict__ O) {
+                                                  float* __restrict__ O, const cogconv_image_io* __restrict__ tab,
+                                                  Epi e) {
     typedef typename LpType<PREC>::T T;
     typedef typename LpType<PREC>::V V;
     constexpr int TAPS = F * F;
@@ -98,6 +100,12 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     const int y0 = tr * 8 * g.dnR - g.hR;
     const int x0 = tc * 32 * g.dnC - g.hC;
     const int HW = g.H * g.W;
+    if (tab) {
+        // Request table: image b's planes and output planes are entry b's buffers.
+        // X and O are re-based so that the image-b offsets below land on them.
+        X = reinterpret_cast<const float*>(reinterpret_cast<uintptr_t>(tab[b].in0) - (uintptr_t)b * m.N * HW * 4);
+        O = reinterpret_cast<float*>(reinterpret_cast<uintptr_t>(tab[b].out) - (uintptr_t)b * Mout * g.OH * g.OW * 4);
+    }
     const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + (size_t)b * m.N * HW, (uint32_t)m.N * HW * 4u);
     const int Mpad = c.mTiles * MT;
     const int Rpad = c.RC;  // planes padded to a multiple of 32
@@ -402,6 +410,24 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
             }
         }
     }
+    // The epilogue rewrites the thread's own stores (still in L2), so the
+    // accumulators above stay in registers whatever its code size.
+    if (e.n) {
+        for (int t = 0; t < MB * 4; t++) {
+            const int mb = t >> 2, pb = t & 3;
+            const int mm = m0 + mb * 16 + l16;
+            const int orow = tr * 8 + 2 * wave + (pb >> 1);
+            const int ocol = tc * 32 + (pb & 1) * 16 + 4 * g4;
+            if (mm >= Mout || orow >= g.OH || ocol >= g.OW) continue;
+            float* dst = O + ((size_t)(b * Mout + mm) * g.OH + orow) * g.OW + ocol;
+            const int pl = b * Mout + mm, pt = orow * g.OW + ocol;
+            if (vec) {
+                epi_rewrite4(e, dst, pl, pt);
+            } else {
+                for (int v = 0; v < 4 && ocol + v < g.OW; v++) epi_rewrite1(e, dst + v, pl, pt + v);
+            }
+        }
+    }
 }
 
 size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision) {
@@ -419,7 +445,8 @@ size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision) {
 // into ws), LP_RUN_ONLY (the convolution on a bank packed earlier into ws).
 template <int F, int MB, int PREC>
 static hipError_t launch_lp3(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
-                             float* out, void* ws, hipStream_t s, int stage) {
+                             float* out, void* ws, hipStream_t s, int stage, const Epi& ep,
+                             const cogconv_image_io* tab) {
     typedef typename LpType<PREC>::T T;
     T* wp = static_cast<T*>(ws);
     if (stage != LP_RUN_ONLY) {
@@ -432,29 +459,31 @@ static hipError_t launch_lp3(const Geom& g, const PlaneMap& m, const IgemmCfg& c
     }
     const long blocks = (long)c.tilesC * c.tilesR * c.mTiles * m.B;
     hipLaunchKernelGGL((k_igemm_lp<F, MB, PREC>), dim3((unsigned)blocks), dim3(256), c.lds_bytes, s, g, m, c, x, wp,
-                       out);
+                       out, tab, ep);
     return hipGetLastError();
 }
 
 template <int F, int PREC>
 static hipError_t launch_lp2(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
-                             float* out, void* ws, hipStream_t s, int stage) {
+                             float* out, void* ws, hipStream_t s, int stage, const Epi& ep,
+                             const cogconv_image_io* tab) {
     switch (c.MB) {
-    case 1: return launch_lp3<F, 1, PREC>(g, m, c, x, w, out, ws, s, stage);
-    case 2: return launch_lp3<F, 2, PREC>(g, m, c, x, w, out, ws, s, stage);
-    default: return launch_lp3<F, 4, PREC>(g, m, c, x, w, out, ws, s, stage);
+    case 1: return launch_lp3<F, 1, PREC>(g, m, c, x, w, out, ws, s, stage, ep, tab);
+    case 2: return launch_lp3<F, 2, PREC>(g, m, c, x, w, out, ws, s, stage, ep, tab);
+    default: return launch_lp3<F, 4, PREC>(g, m, c, x, w, out, ws, s, stage, ep, tab);
     }
 }
 
 template <int PREC>
 static hipError_t launch_lp1(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
-                             float* out, void* ws, hipStream_t s, int stage) {
+                             float* out, void* ws, hipStream_t s, int stage, const Epi& ep,
+                             const cogconv_image_io* tab) {
     switch (g.FH) {
-    case 1: return launch_lp2<1, PREC>(g, m, c, x, w, out, ws, s, stage);
-    case 3: return launch_lp2<3, PREC>(g, m, c, x, w, out, ws, s, stage);
-    case 5: return launch_lp2<5, PREC>(g, m, c, x, w, out, ws, s, stage);
-    case 7: return launch_lp2<7, PREC>(g, m, c, x, w, out, ws, s, stage);
-    case 9: return launch_lp2<9, PREC>(g, m, c, x, w, out, ws, s, stage);
+    case 1: return launch_lp2<1, PREC>(g, m, c, x, w, out, ws, s, stage, ep, tab);
+    case 3: return launch_lp2<3, PREC>(g, m, c, x, w, out, ws, s, stage, ep, tab);
+    case 5: return launch_lp2<5, PREC>(g, m, c, x, w, out, ws, s, stage, ep, tab);
+    case 7: return launch_lp2<7, PREC>(g, m, c, x, w, out, ws, s, stage, ep, tab);
+    case 9: return launch_lp2<9, PREC>(g, m, c, x, w, out, ws, s, stage, ep, tab);
     default: return hipErrorInvalidValue;
     }
 }
@@ -462,15 +491,16 @@ static hipError_t launch_lp1(const Geom& g, const PlaneMap& m, const IgemmCfg& c
 bool igemm_lp_supported(int F) { return F == 1 || F == 3 || F == 5 || F == 7 || F == 9; }
 
 hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& cfg, const float* x,
-                           const float* w, float* out, void* ws, hipStream_t s, int stage) {
+                           const float* w, float* out, void* ws, hipStream_t s, int stage, const Epi& ep,
+                           const cogconv_image_io* tab, bool x16) {
     const long blocks = (long)cfg.tilesC * cfg.tilesR * cfg.mTiles * m.B;
     if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
     IgemmCfg c = cfg;
-    if (reinterpret_cast<uintptr_t>(x) & 15) c.vec4 = 0;  // 16-byte loads need 16-byte rows
+    if (tab ? !x16 : (reinterpret_cast<uintptr_t>(x) & 15) != 0) c.vec4 = 0;  // 16-byte loads need 16-byte rows
     switch (precision) {
-    case COGCONV_BF16: return launch_lp1<COGCONV_BF16>(g, m, c, x, w, out, ws, s, stage);
-    case COGCONV_FP16: return launch_lp1<COGCONV_FP16>(g, m, c, x, w, out, ws, s, stage);
-    case COGCONV_SPLIT_BF16: return launch_lp1<COGCONV_SPLIT_BF16>(g, m, c, x, w, out, ws, s, stage);
+    case COGCONV_BF16: return launch_lp1<COGCONV_BF16>(g, m, c, x, w, out, ws, s, stage, ep, tab);
+    case COGCONV_FP16: return launch_lp1<COGCONV_FP16>(g, m, c, x, w, out, ws, s, stage, ep, tab);
+    case COGCONV_SPLIT_BF16: return launch_lp1<COGCONV_SPLIT_BF16>(g, m, c, x, w, out, ws, s, stage, ep, tab);
     default: return hipErrorInvalidValue;
     }
 }

@@ -113,6 +113,21 @@ __device__ __forceinline__ int fast_div(int n, int d, float inv_d) {
     return q;
 }
 
+// Elementwise epilogue of a plan (cogconv_plan_set_epilogue): n ops applied in
+// order to every output element (plane, point) at its store.  Field operand
+// `slot` is read at (srcPlanes == 1 ? 0 : plane) * srcPoints + (srcPoints == 1 ?
+// 0 : point): same type as the output, a ScalarField of its field shape, a 0-D
+// field with its planes, or a 0-D scalar (ResultType.scala:29-58).  n == 0: none.
+struct Epi {
+    int n;
+    int op[COGCONV_EPI_MAX_OPS];
+    float val[COGCONV_EPI_MAX_OPS];
+    int slot[COGCONV_EPI_MAX_OPS];
+    const float* src[COGCONV_EPI_MAX_OPERANDS];
+    int srcPlanes[COGCONV_EPI_MAX_OPERANDS];
+    int srcPoints[COGCONV_EPI_MAX_OPERANDS];
+};
+
 // Kernel families.
 enum KernelKind {
     KK_DIRECT = 1,
@@ -123,6 +138,7 @@ enum KernelKind {
     KK_FA_IGEMM_LP = 6,
     KK_IGEMM_UP = 7,
     KK_DIRECT_RT = 8,
+    KK_FFT = 9,
 };
 
 // LDS-tiled direct kernel configuration.
@@ -133,6 +149,13 @@ struct DirectCfg {
     int ldsRows;
     int tilesR, tilesC;
     size_t lds_bytes;
+};
+
+// FFT route of a PlaneByPlane plan: expanded plane Er x Ec; PA / PK image and
+// filter planes, P output planes.
+struct FftCfg {
+    int Er, Ec;
+    int PA, PK, P;
 };
 
 // Split-reduction kernel for small outputs (FilterAdjoint to a filter-sized result).
@@ -202,38 +225,68 @@ enum IgemmVariant {
     IV_F32_FRAME = 1,  // FP32 MFMA, ProjectFrame/BackProjectFrame block-reduce modes
 };
 
-// Kernel launchers (device code in *.hip).
+// The empty epilogue (plain stores).
+inline const Epi kNoEpi = {};
+// Checks an epilogue against an output of type `t` (cogconv_plan_set_epilogue's
+// rules) and builds its device form; `operands` receives its field count.
+int epi_build(const cogconv_field_type& t, const cogconv_epilogue* in, Epi* out, int* operands);
+// Binds the n field operands of `e` to device buffers.
+int epi_bind(Epi* e, int n, const float* const* operands);
+
+// Kernel launchers (device code in *.hip).  `e` is the plan's elementwise
+// epilogue, applied by the kernel that stores the final output.
 // Register-tiled direct kernel: no sampling, FH <= 16, FW <= 15.
 bool direct_rt_supported(const Geom& g);
 void direct_rt_config(const Geom& g, int rpt, DirectCfg& c);
 hipError_t launch_direct_rt(const Geom& g, const PlaneMap& m, const DirectCfg& c, const float* a, const float* f,
-                            float* out, hipStream_t s);
+                            float* out, hipStream_t s, const Epi& e = kNoEpi);
 hipError_t launch_direct(const Geom& g, const PlaneMap& m, const DirectCfg& c, const float* a, const float* f,
-                         float* out, hipStream_t s);
+                         float* out, hipStream_t s, const Epi& e = kNoEpi);
 hipError_t launch_small_output(const Geom& g, const PlaneMap& m, const SmallCfg& c, const float* a, const float* f,
-                               float* out, float* workspace, hipStream_t s);
+                               float* out, float* workspace, hipStream_t s, const Epi& e = kNoEpi);
 bool igemm_f32_supported(int F);
 size_t igemm_f32_exchange_floats(int MB);
 bool igemm_f32_supported_rc(int FR, int FC);
+// `tab` (device memory, one entry per image of the plan's batch): each image's
+// operand-1 planes and output planes come from its entry instead of x / out.
 hipError_t launch_igemm_f32_phase(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const Phase& ph, const float* x,
-                                  const float* w, float* out, hipStream_t s);
+                                  const float* w, float* out, hipStream_t s, const Epi& e = kNoEpi,
+                                  const cogconv_image_io* tab = nullptr);
 hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x, const float* w,
-                            float* out, hipStream_t s);
+                            float* out, hipStream_t s, const Epi& e = kNoEpi, const cogconv_image_io* tab = nullptr);
 hipError_t launch_fa_igemm_f32(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
-                               float* out, float* ws, hipStream_t s);
+                               float* out, float* ws, hipStream_t s, const Epi& e = kNoEpi);
+hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long total, long points, hipStream_t s,
+                                  const Epi& e);
 size_t fa_igemm_lp_lds_bytes(const FaCfg& c, int OW, int precision);
 hipError_t launch_fa_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x,
-                              const float* dy, float* out, float* ws, hipStream_t s);
+                              const float* dy, float* out, float* ws, hipStream_t s, const Epi& e = kNoEpi);
 bool igemm_lp_supported(int F);
 size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision);
 size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision);
 enum LpStage { LP_PACK_AND_RUN = 0, LP_PACK_ONLY = 1, LP_RUN_ONLY = 2 };
 hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x,
-                           const float* w, float* out, void* ws, hipStream_t s, int stage = LP_PACK_AND_RUN);
+                           const float* w, float* out, void* ws, hipStream_t s, int stage = LP_PACK_AND_RUN,
+                           const Epi& e = kNoEpi, const cogconv_image_io* tab = nullptr, bool x16 = false);
 hipError_t launch_block_reduce_sum(const float* in, float* out, int64_t points, int out_planes, int factor,
-                                   hipStream_t s);
+                                   hipStream_t s, const Epi& e = kNoEpi);
+// out = epilogue(in) over `planes` planes of `points` (the unmerged elementwise pass).
+hipError_t launch_elementwise(const float* in, float* out, int64_t points, int planes, const Epi& e, hipStream_t s);
 hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t s);
+// FFT route (fft.hip): good transform lengths, workspace, per-plan hipFFT state.
+int fft_good_length(int m);
+size_t fft_workspace_bytes(const FftCfg& c);
+void* fft_state_new();
+void fft_release(void* state);
+int launch_fft(const Geom& g, const FftCfg& c, void** state, const float* x, const float* f, float* out, void* ws,
+               hipStream_t s, const Epi& e);
 hipError_t launch_fill_value(float* p, size_t n, float v, hipStream_t s);
+
+// Request-table launches (plan.cpp): plans that take one, and the launch on a
+// table already in device memory (x16: every entry's operand 1 is 16-byte aligned).
+bool plan_table_capable(const cogconv_plan_s* plan);
+int launch_table(cogconv_plan plan, const cogconv_image_io* device_table, bool x16, const float* in1, void* workspace,
+                 bool packed, hipStream_t s);
 
 // Candidate configurations of a plan (plan.cpp) and the autotuning cache
 // (tune.cpp): the tuned variant index for `desc` on the current device and the
@@ -254,8 +307,12 @@ struct cogconv_plan_s {
     cogconv::IgemmCfg icfg;
     cogconv::FaCfg fcfg;
     cogconv::PhaseSet ps;
+    cogconv::FftCfg fft;
+    void* fft_state;  // hipFFT handles of an FFT plan (made at its first launch)
     size_t workspace;
     double flops;
     double bytes;
     int variant;  // index into the plan's candidate list (autotuning)
+    cogconv::Epi epi;       // elementwise epilogue (epi.n == 0: none); src[] set per launch
+    int epi_operands;       // field operands the epilogue reads
 };

@@ -509,8 +509,36 @@ struct Variant {
     IgemmCfg i;
     FaCfg f;
     PhaseSet ps;
+    FftCfg fft;
     size_t workspace;
 };
+
+// The FFT route (DynamicConvolutionGenerator.scala:109-148, convolveBy2DFFT
+// L322-476): PlaneByPlane on 2-D fields, BorderZero / Clamp / Cyclic, no
+// sampling, a filter no larger than the field.  The transform covers the
+// field plus twice the filter's half-width in each dimension (the span a
+// circular convolution needs to equal the bordered one; FFTConvolutionUtils.scala:50-97),
+// rounded up to a 2/3/5/7-smooth length; a Cyclic field of such a length is
+// transformed as it is.
+bool choose_fft(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FftCfg* out) {
+    if (d->mode != COGCONV_PLANE_BY_PLANE || d->in0.dims != 2 || d->in1.dims != 2) return false;
+    if (g.border != COGCONV_BORDER_ZERO && g.border != COGCONV_BORDER_CLAMP && g.border != COGCONV_BORDER_CYCLIC)
+        return false;
+    if (g.upAr != 1 || g.upAc != 1 || g.upFr != 1 || g.upFc != 1 || g.dnR != 1 || g.dnC != 1) return false;
+    if (g.FH > g.H || g.FW > g.W || (g.FH % 2) == 0 || (g.FW % 2) == 0) return false;
+    FftCfg c;
+    const int hr = (g.FH - 1) / 2, hc = (g.FW - 1) / 2;
+    c.Er = fft_good_length(g.H + 2 * hr);
+    c.Ec = fft_good_length(g.W + 2 * hc);
+    if (g.border == COGCONV_BORDER_CYCLIC && fft_good_length(g.H) == g.H) c.Er = g.H;
+    if (g.border == COGCONV_BORDER_CYCLIC && fft_good_length(g.W) == g.W) c.Ec = g.W;
+    if ((long)c.Er * c.Ec >= (1L << 30)) return false;
+    c.PA = m.PA;
+    c.PK = m.PFn;
+    c.P = m.nOut;
+    *out = c;
+    return true;
+}
 
 // FP32 implicit GEMM for the frame block-reduce modes on 2x up-sampled input:
 // the zero-inserted input makes output pixel (2R+pr, 2C+pc) see only the taps
@@ -617,6 +645,14 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         v.workspace = 0;
         out.push_back(v);
     };
+    auto add_fft = [&](const FftCfg& c) {
+        Variant v;
+        std::memset(&v, 0, sizeof v);
+        v.kind = KK_FFT;
+        v.fft = c;
+        v.workspace = fft_workspace_bytes(c);
+        out.push_back(v);
+    };
     if (desc->precision != COGCONV_FP32) {
         FaCfg flp;
         if (choose_fa_igemm_lp(desc, g, m, &flp)) {
@@ -658,8 +694,15 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
     }
     PhaseSet up;
     const bool up_ok = choose_igemm_up(desc, g, m, &up);
+    FftCfg fc;
+    const bool fft_ok = choose_fft(desc, g, m, &fc);
+    // The FFT route first for filters the register-tiled kernel does not take and
+    // that are at least 31 x 31 (below that the direct kernels measure faster);
+    // the autotuner times both routes.
+    const bool fft_first = fft_ok && !rt_ok && (long)g.FH * g.FW >= 31L * 31;
     // Heuristic default.
-    if (up_ok) add_up(up);
+    if (fft_first) add_fft(fc);
+    else if (up_ok) add_up(up);
     else if (igemm_ok) add(KK_IGEMM, nullptr, nullptr, &ig, nullptr);
     else if (fa_ok) add(KK_FA_IGEMM, nullptr, nullptr, nullptr, &fa);
     else if (small_out) add(KK_SMALL_OUTPUT, nullptr, &sm, nullptr, nullptr);
@@ -695,11 +738,12 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         const DirectCfg c = choose_direct(g, rg);
         add(KK_DIRECT, &c, nullptr, nullptr, nullptr);
     }
+    if (fft_ok && !fft_first) add_fft(fc);
 }
 
 bool kind_matches_algo(int kind, int algo) {
     switch (algo) {
-    case COGCONV_ALGO_DIRECT: return kind == KK_DIRECT || kind == KK_DIRECT_RT;
+    case COGCONV_ALGO_DIRECT: return kind == KK_DIRECT || kind == KK_DIRECT_RT || kind == KK_FFT;
     case COGCONV_ALGO_SMALL_OUTPUT: return kind == KK_SMALL_OUTPUT;
     case COGCONV_ALGO_IMPLICIT_GEMM:
         return kind == KK_IGEMM || kind == KK_FA_IGEMM || kind == KK_IGEMM_LP || kind == KK_FA_IGEMM_LP ||
@@ -760,6 +804,7 @@ int make_plan(const cogconv_desc* desc, int vi, cogconv_plan* plan) {
             tmp.icfg = vars[tuned].i;
             tmp.fcfg = vars[tuned].f;
             tmp.ps = vars[tuned].ps;
+            tmp.fft = vars[tuned].fft;
             char name[192];
             describe_into(&tmp, name, sizeof name);
             if (tunedName == name) pick = tuned;
@@ -779,8 +824,13 @@ int make_plan(const cogconv_desc* desc, int vi, cogconv_plan* plan) {
     p->icfg = v.i;
     p->fcfg = v.f;
     p->ps = v.ps;
+    p->fft = v.fft;
     p->workspace = v.workspace;
     p->variant = pick;
+    if (p->kind == KK_FFT && !(p->fft_state = fft_state_new())) {
+        delete p;
+        return fail(COGCONV_ERR_OOM, "host allocation failed");
+    }
     // Algorithmic work: every multiply-add whose tap multiplies a stored
     // (non-inserted) sample.  Up-sampled operand 1 leaves FH*FW/(upAr*upAc)
     // such taps per output on average; FilterAdjoint up-sampling of operand 2
@@ -842,6 +892,7 @@ extern "C" int cogconv_plan_variant(cogconv_plan plan, int32_t* variant) {
 
 extern "C" int cogconv_plan_destroy(cogconv_plan plan) {
     clear_error();
+    if (plan) fft_release(plan->fft_state);
     delete plan;
     return COGCONV_OK;
 }
@@ -914,6 +965,10 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
                  plan->fcfg.NW, plan->fcfg.lds_bytes);
         break;
     }
+    case KK_FFT:
+        snprintf(buf, len, "fft %s out=%dx%dx%d filter=%dx%d transform=%dx%d", mode_name(plan->desc.mode),
+                 plan->m.nOut, g.OH, g.OW, g.FH, g.FW, plan->fft.Er, plan->fft.Ec);
+        break;
     case KK_FA_IGEMM:
         snprintf(buf, len, "fa_igemm_f32 %s out=%dx%dx%d MB=%d CBW=%d RR=%d CW=%d groups=%d splits=%d waves=%d lds=%zu",
                  mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->fcfg.MB, plan->fcfg.CBW, plan->fcfg.RR,
@@ -958,43 +1013,214 @@ extern "C" int cogconv_launch_packed(cogconv_plan plan, const float* in0, float*
     return check_hip(e, "convolution launch");
 }
 
+namespace {
+
+// Every kernel of the plan's launch sequence, with epilogue `ep` on the final stores.
+int launch_all(cogconv_plan plan, const float* in0, const float* in1, float* out, void* workspace, hipStream_t s,
+               const Epi& ep) {
+    hipError_t e = hipSuccess;
+    switch (plan->kind) {
+    case KK_DIRECT:
+        e = launch_direct(plan->g, plan->m, plan->dcfg, in0, in1, out, s, ep);
+        break;
+    case KK_DIRECT_RT:
+        e = launch_direct_rt(plan->g, plan->m, plan->dcfg, in0, in1, out, s, ep);
+        break;
+    case KK_SMALL_OUTPUT:
+        e = launch_small_output(plan->g, plan->m, plan->scfg, in0, in1, out, static_cast<float*>(workspace), s, ep);
+        break;
+    case KK_IGEMM:
+        e = launch_igemm_f32(plan->g, plan->m, plan->icfg, in0, in1, out, s, ep);
+        break;
+    case KK_FA_IGEMM:
+        e = launch_fa_igemm_f32(plan->g, plan->m, plan->fcfg, in0, in1, out, static_cast<float*>(workspace), s, ep);
+        break;
+    case KK_IGEMM_LP:
+        e = launch_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->icfg, in0, in1, out, workspace, s,
+                            LP_PACK_AND_RUN, ep);
+        break;
+    case KK_IGEMM_UP:
+        for (int i = 0; i < plan->ps.n && e == hipSuccess; i++)
+            e = launch_igemm_f32_phase(plan->ps.g[i], plan->m, plan->ps.c[i], plan->ps.p[i], in0, in1, out, s, ep);
+        break;
+    case KK_FA_IGEMM_LP:
+        e = launch_fa_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->fcfg, in0, in1, out,
+                               static_cast<float*>(workspace), s, ep);
+        break;
+    case KK_FFT:
+        return launch_fft(plan->g, plan->fft, &plan->fft_state, in0, in1, out, workspace, s, ep);
+    default:
+        return fail(COGCONV_ERR_RUNTIME, "internal: unknown kernel kind %d", plan->kind);
+    }
+    return check_hip(e, "convolution launch");
+}
+
+// The plan's epilogue with its field operands bound to `operands`.
+int bind_epilogue(const cogconv_plan_s* plan, const float* const* operands, Epi* ep) {
+    *ep = plan->epi;
+    return epi_bind(ep, plan->epi_operands, operands);
+}
+
+}  // namespace
+
 extern "C" int cogconv_launch(cogconv_plan plan, const float* in0, const float* in1, float* out, void* workspace,
                               cogconv_stream stream) {
     clear_error();
     if (!plan || !in0 || !in1 || !out) return fail(COGCONV_ERR_INVALID_ARG, "null plan or buffer");
     if (plan->workspace && !workspace)
         return fail(COGCONV_ERR_INVALID_ARG, "plan needs %zu bytes of workspace", plan->workspace);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    hipError_t e = hipSuccess;
-    switch (plan->kind) {
-    case KK_DIRECT:
-        e = launch_direct(plan->g, plan->m, plan->dcfg, in0, in1, out, s);
-        break;
-    case KK_DIRECT_RT:
-        e = launch_direct_rt(plan->g, plan->m, plan->dcfg, in0, in1, out, s);
-        break;
-    case KK_SMALL_OUTPUT:
-        e = launch_small_output(plan->g, plan->m, plan->scfg, in0, in1, out, static_cast<float*>(workspace), s);
-        break;
-    case KK_IGEMM:
-        e = launch_igemm_f32(plan->g, plan->m, plan->icfg, in0, in1, out, s);
-        break;
-    case KK_FA_IGEMM:
-        e = launch_fa_igemm_f32(plan->g, plan->m, plan->fcfg, in0, in1, out, static_cast<float*>(workspace), s);
-        break;
-    case KK_IGEMM_LP:
-        e = launch_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->icfg, in0, in1, out, workspace, s);
-        break;
-    case KK_IGEMM_UP:
-        for (int i = 0; i < plan->ps.n && e == hipSuccess; i++)
-            e = launch_igemm_f32_phase(plan->ps.g[i], plan->m, plan->ps.c[i], plan->ps.p[i], in0, in1, out, s);
-        break;
-    case KK_FA_IGEMM_LP:
-        e = launch_fa_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->fcfg, in0, in1, out,
-                               static_cast<float*>(workspace), s);
-        break;
-    default:
-        return fail(COGCONV_ERR_RUNTIME, "internal: unknown kernel kind %d", plan->kind);
+    if (plan->epi_operands)
+        return fail(COGCONV_ERR_INVALID_ARG, "the plan's epilogue reads field operands: use cogconv_launch_epilogue");
+    return launch_all(plan, in0, in1, out, workspace, reinterpret_cast<hipStream_t>(stream), plan->epi);
+}
+
+bool cogconv::plan_table_capable(const cogconv_plan_s* plan) {
+    return plan->kind == KK_IGEMM_LP || (plan->kind == KK_IGEMM && plan->m.mode != COGCONV_PLANE_BY_PLANE);
+}
+
+int cogconv::launch_table(cogconv_plan plan, const cogconv_image_io* device_table, bool x16, const float* in1,
+                          void* workspace, bool packed, hipStream_t s) {
+    hipError_t e;
+    if (plan->kind == KK_IGEMM_LP)
+        e = launch_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->icfg, nullptr, in1, nullptr, workspace, s,
+                            packed ? LP_RUN_ONLY : LP_PACK_AND_RUN, plan->epi, device_table, x16);
+    else
+        e = launch_igemm_f32(plan->g, plan->m, plan->icfg, nullptr, in1, nullptr, s, plan->epi, device_table);
+    return check_hip(e, "table launch");
+}
+
+extern "C" int cogconv_plan_table_supported(cogconv_plan plan, int32_t* supported) {
+    clear_error();
+    if (!plan || !supported) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    *supported = plan_table_capable(plan) ? 1 : 0;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_launch_table(cogconv_plan plan, const cogconv_image_io* host_table,
+                                    cogconv_image_io* device_table, const float* in1, void* workspace, int32_t packed,
+                                    cogconv_stream stream) {
+    clear_error();
+    if (!plan || !host_table || !device_table) return fail(COGCONV_ERR_INVALID_ARG, "null plan or table");
+    if (!plan_table_capable(plan))
+        return fail(COGCONV_ERR_UNSUPPORTED, "table launches are offered for the MFMA frame plans (no up-sampling)");
+    if (packed && plan->kind != KK_IGEMM_LP) return fail(COGCONV_ERR_INVALID_ARG, "plan does not pack operand 2");
+    if (!packed && !in1) return fail(COGCONV_ERR_INVALID_ARG, "null operand 2");
+    if ((plan->workspace || packed) && !workspace)
+        return fail(COGCONV_ERR_INVALID_ARG, "plan needs %zu bytes of workspace", plan->workspace);
+    if (plan->epi_operands)
+        return fail(COGCONV_ERR_INVALID_ARG, "table launches take no epilogue field operands");
+    bool x16 = true;
+    for (int b = 0; b < plan->m.B; b++) {
+        if (!host_table[b].in0 || !host_table[b].out)
+            return fail(COGCONV_ERR_INVALID_ARG, "null buffer in table entry %d", b);
+        x16 = x16 && (reinterpret_cast<uintptr_t>(host_table[b].in0) & 15) == 0;
     }
-    return check_hip(e, "convolution launch");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (int st = check_hip(hipMemcpyAsync(device_table, host_table, sizeof(cogconv_image_io) * plan->m.B,
+                                          hipMemcpyHostToDevice, s),
+                           "table copy"))
+        return st;
+    return launch_table(plan, device_table, x16, in1, workspace, packed != 0, s);
+}
+
+extern "C" int cogconv_launch_epilogue(cogconv_plan plan, const float* in0, const float* in1, float* out,
+                                       void* workspace, const float* const* operands, cogconv_stream stream) {
+    clear_error();
+    if (!plan || !in0 || !in1 || !out) return fail(COGCONV_ERR_INVALID_ARG, "null plan or buffer");
+    if (plan->workspace && !workspace)
+        return fail(COGCONV_ERR_INVALID_ARG, "plan needs %zu bytes of workspace", plan->workspace);
+    Epi ep;
+    if (int st = bind_epilogue(plan, operands, &ep)) return st;
+    return launch_all(plan, in0, in1, out, workspace, reinterpret_cast<hipStream_t>(stream), ep);
+}
+
+namespace {
+
+bool epi_op_known(int op) {
+    return (op >= COGCONV_EPI_ADD_CONST && op <= COGCONV_EPI_MIN_CONST) ||
+           (op >= COGCONV_EPI_NEG && op <= COGCONV_EPI_SIGNUM) ||
+           (op >= COGCONV_EPI_ADD_FIELD && op <= COGCONV_EPI_MIN_FIELD);
+}
+
+// Checks `in` against a field of type `t` and builds the device form.  An
+// operand must combine with `t` to give `t` under ResultType.scala:29-58: the
+// same type, a 0-D field of its tensor shape, a tensor-order-0 field of its
+// field shape, or a 0-D scalar.
+}  // namespace
+
+int cogconv::epi_build(const cogconv_field_type& t, const cogconv_epilogue* in, Epi* out, int* operands) {
+    std::memset(out, 0, sizeof *out);
+    *operands = 0;
+    if (!in || in->n_ops == 0) return COGCONV_OK;
+    if (in->n_ops < 0 || in->n_ops > COGCONV_EPI_MAX_OPS)
+        return fail(COGCONV_ERR_INVALID_ARG, "epilogue has %d ops (at most %d)", in->n_ops, COGCONV_EPI_MAX_OPS);
+    if (in->n_operands < 0 || in->n_operands > COGCONV_EPI_MAX_OPERANDS)
+        return fail(COGCONV_ERR_INVALID_ARG, "epilogue has %d operands (at most %d)", in->n_operands,
+                    COGCONV_EPI_MAX_OPERANDS);
+    const int points = t.rows * t.cols;
+    for (int s = 0; s < in->n_operands; s++) {
+        const cogconv_field_type& f = in->operands[s];
+        const bool zeroD = f.rows * f.cols == 1 && f.dims == 0;
+        const bool sameField = f.dims == t.dims && f.rows == t.rows && f.cols == t.cols;
+        const bool sameTensor = f.planes == t.planes && f.tensor_order == t.tensor_order;
+        const bool scalar = f.tensor_order == 0 && f.planes == 1;
+        if (!((sameField && sameTensor) || (zeroD && sameTensor) || (sameField && scalar) || (zeroD && scalar)))
+            return fail(COGCONV_ERR_ILLEGAL_OP,
+                        "epilogue operand %d (%dD %dx%d, %d planes) does not combine with the %dD %dx%d x %d-plane "
+                        "output into the output's type",
+                        s, f.dims, f.rows, f.cols, f.planes, t.dims, t.rows, t.cols, t.planes);
+        out->srcPlanes[s] = scalar ? 1 : t.planes;
+        out->srcPoints[s] = zeroD ? 1 : points;
+    }
+    for (int i = 0; i < in->n_ops; i++) {
+        const cogconv_epilogue_op& o = in->ops[i];
+        if (!epi_op_known(o.op)) return fail(COGCONV_ERR_INVALID_ARG, "unknown epilogue op %d", o.op);
+        if (o.op >= COGCONV_EPI_ADD_FIELD && (o.operand < 0 || o.operand >= in->n_operands))
+            return fail(COGCONV_ERR_INVALID_ARG, "epilogue op %d names operand %d of %d", i, o.operand,
+                        in->n_operands);
+        out->op[i] = o.op;
+        out->val[i] = o.value;
+        out->slot[i] = o.op >= COGCONV_EPI_ADD_FIELD ? o.operand : 0;
+    }
+    out->n = in->n_ops;
+    *operands = in->n_operands;
+    return COGCONV_OK;
+}
+
+int cogconv::epi_bind(Epi* e, int n, const float* const* operands) {
+    if (n && !operands) return fail(COGCONV_ERR_INVALID_ARG, "the epilogue reads %d field operand(s)", n);
+    for (int i = 0; i < n; i++) {
+        if (!operands[i]) return fail(COGCONV_ERR_INVALID_ARG, "null epilogue operand %d", i);
+        e->src[i] = operands[i];
+    }
+    return COGCONV_OK;
+}
+
+namespace {
+}  // namespace
+
+extern "C" int cogconv_plan_set_epilogue(cogconv_plan plan, const cogconv_epilogue* epi) {
+    clear_error();
+    if (!plan) return fail(COGCONV_ERR_INVALID_ARG, "null plan");
+    Epi e;
+    int n = 0;
+    if (int st = epi_build(plan->out, epi, &e, &n)) return st;
+    plan->epi = e;
+    plan->epi_operands = n;
+    return COGCONV_OK;
+}
+
+extern "C" int cogconv_elementwise(const float* in, float* out, const cogconv_field_type* type,
+                                   const cogconv_epilogue* epi, const float* const* operands,
+                                   cogconv_stream stream) {
+    clear_error();
+    if (!in || !out || !type) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    if (type->rows < 1 || type->cols < 1 || type->planes < 1) return fail(COGCONV_ERR_INVALID_ARG, "empty field");
+    Epi e;
+    int n = 0;
+    if (int st = epi_build(*type, epi, &e, &n)) return st;
+    if (int st = epi_bind(&e, n, operands)) return st;
+    return check_hip(launch_elementwise(in, out, (int64_t)type->rows * type->cols, type->planes, e,
+                                        reinterpret_cast<hipStream_t>(stream)),
+                     "elementwise launch");
 }

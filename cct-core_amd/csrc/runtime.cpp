@@ -253,3 +253,26 @@ extern "C" int cogconv_block_reduce_sum(const float* in, float* out, int64_t poi
                                              reinterpret_cast<hipStream_t>(stream)),
                      "blockReduceSum launch");
 }
+
+extern "C" int cogconv_block_reduce_sum_epilogue(const float* in, float* out, const cogconv_field_type* in_type,
+                                                 int32_t factor, const cogconv_epilogue* epi,
+                                                 const float* const* operands, cogconv_stream stream) {
+    clear_error();
+    if (!in || !out || !in_type) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
+    const int64_t points = (int64_t)in_type->rows * in_type->cols;
+    if (points < 1 || in_type->planes < 1 || factor < 1)
+        return fail(COGCONV_ERR_INVALID_ARG, "bad blockReduceSum sizes");
+    if (in_type->planes % factor)
+        return fail(COGCONV_ERR_ILLEGAL_OP, "tensor length %d is not a multiple of the reduction factor %d",
+                    in_type->planes, factor);
+    cogconv_field_type t = *in_type;
+    t.planes = in_type->planes / factor;
+    t.tensor_order = t.planes == 1 ? 0 : 1;
+    Epi e;
+    int n = 0;
+    if (int st = epi_build(t, epi, &e, &n)) return st;
+    if (int st = epi_bind(&e, n, operands)) return st;
+    return check_hip(launch_block_reduce_sum(in, out, points, t.planes, factor, reinterpret_cast<hipStream_t>(stream),
+                                             e),
+                     "blockReduceSum launch");
+}

@@ -14,7 +14,14 @@
 //   - stacks queued requests of one shape bucket (same op, same bank, same field
 //     shape) into a single launch when the vector mode keeps images independent
 //     (ProjectFrame / BackProjectFrame, with or without block reduction: batch is
-//     folded into planes, so stacking is concatenation of planes);
+//     folded into planes, so stacking is concatenation of planes).  Requests in
+//     host memory are copied into consecutive planes; requests whose fields are
+//     resident in HBM at unrelated addresses run as one request-table launch
+//     (each image's blocks read their buffers from a table the host builds in
+//     pinned memory and copies ahead of the kernel on the launch's stream);
+//   - captures the launch sequence of a hot (plan, bank, stream) once into a
+//     hipGraph and replays it (the table copy stays outside the graph, so a
+//     replay runs on that launch's own table);
 //   - spreads launches over several streams: copies and kernels of different
 //     launches overlap on the device;
 //   - records submit -> result-ready latency per request (result in host memory,
@@ -127,6 +134,15 @@ struct Launch {
     int stream;
     hipEvent_t ev;
     std::vector<std::pair<size_t, size_t>> blocks;  // arena (offset, bytes)
+    int slot = -1;                                  // pinned request-table slot
+};
+
+// A captured launch sequence of one (plan, bank, stream).
+struct GraphEntry {
+    int uses = 0;
+    hipGraphExec_t exec = nullptr;
+    bool capturing = false;  // a worker is capturing it (others launch directly meanwhile)
+    bool failed = false;     // capture did not work: launch directly
 };
 
 bool stackable(int mode) {
@@ -164,6 +180,15 @@ struct cogconv_server_s {
     // operand 2 (the low-precision frame kernels): arena offset, kept for the
     // server's life; plans that do not pack map to SIZE_MAX.
     std::map<std::pair<cogconv_plan, int>, size_t> packed;
+    // Request tables: per stream one device table of max_stack entries (a
+    // stream runs its launches in order, so the table copy of a launch lands
+    // after the previous launch on that stream has read its own), and pinned
+    // host slots, one per launch in flight.
+    std::vector<cogconv_image_io*> dev_tables;
+    cogconv_image_io* host_tables = nullptr;
+    std::vector<int> free_slots;
+    std::map<std::tuple<cogconv_plan, int, int>, GraphEntry> graphs;
+    uint64_t graph_launches = 0;
     size_t reserved = 0;  // arena bytes held by packed banks (taken from the top of the arena)
     // Worker threads (two when there are two streams or more): each admits
     // requests, issues launches and retires completions.  All shared state is
@@ -205,6 +230,15 @@ struct cogconv_server_s {
             return hipSuccess;
         }
         return hipEventCreateWithFlags(ev, hipEventDisableTiming);
+    }
+
+    // Requests in host memory stack by copying into consecutive planes;
+    // resident ones need a request-table launch, which the MFMA frame plans take.
+    bool stack_ok(const Req* h) {
+        if (!(h->r.flags & COGCONV_REQUEST_RESIDENT)) return true;
+        int err = 0;
+        cogconv_plan p = plan_for(h->r.desc, &err);
+        return p && plan_table_capable(p);
     }
 
     void fail_reqs(const std::vector<Req*>& g, int status, const std::string& msg) {
@@ -280,28 +314,97 @@ struct cogconv_server_s {
                 if (!arena.alloc(ws, &o_ws)) return false;
                 blocks.push_back({o_ws, ws});
             }
+            void* wsp = pk != SIZE_MAX ? arena.base + pk : (ws ? arena.base + o_ws : nullptr);
+            // Requests of a table-capable plan on a resident bank run as a
+            // request-table launch (one image per request), which a graph can replay.
+            const bool table = h->r.bank >= 0 && plan_table_capable(plan) && !free_slots.empty();
+            if (g.size() > 1 && !table) {  // a stack needs a table: the caller issues the head alone
+                for (auto& bk : blocks) arena.release(bk.first, bk.second);
+                return false;
+            }
+            int slot = -1;
+            bool x16 = true;
+            GraphEntry* ge = nullptr;
+            bool capture = false;
+            if (table) {
+                slot = free_slots.back();
+                free_slots.pop_back();
+                cogconv_image_io* ht = host_tables + (size_t)slot * cfg.max_stack;
+                for (size_t i = 0; i < g.size(); i++) {
+                    ht[i].in0 = g[i]->r.host_in0;
+                    ht[i].out = g[i]->r.host_out;
+                    x16 = x16 && (reinterpret_cast<uintptr_t>(g[i]->r.host_in0) & 15) == 0;
+                }
+                if (cfg.graph_after > 0 && x16) {
+                    ge = &graphs[std::make_tuple(plan, h->r.bank, si)];
+                    ge->uses++;
+                    if (!ge->exec && !ge->failed && !ge->capturing && ge->uses >= cfg.graph_after) {
+                        capture = true;
+                        ge->capturing = true;
+                    }
+                }
+            }
+            hipGraphExec_t exec = ge ? ge->exec : nullptr;
             hipEvent_t ev = nullptr;
             hipError_t e = take_event(&ev);
+            bool captured = false, capture_failed = false;
             const int st = e != hipSuccess ? COGCONV_ERR_RUNTIME : unlocked(lk, [&] {
-                int r;
-                if (pk != SIZE_MAX)
-                    r = cogconv_launch_packed(plan, h->r.host_in0, h->r.host_out, arena.base + pk,
+                int r = COGCONV_OK;
+                if (table) {
+                    r = check_hip(hipMemcpyAsync(dev_tables[si], host_tables + (size_t)slot * cfg.max_stack,
+                                                 sizeof(cogconv_image_io) * g.size(), hipMemcpyHostToDevice, s),
+                                  "request table copy");
+                    if (r == COGCONV_OK && capture) {
+                        hipGraph_t graph = nullptr;
+                        hipGraphExec_t ex = nullptr;
+                        bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+                        if (ok) {
+                            const int lr = launch_table(plan, dev_tables[si], x16, in1, wsp, pk != SIZE_MAX, s);
+                            ok = hipStreamEndCapture(s, &graph) == hipSuccess && lr == COGCONV_OK && graph;
+                        }
+                        ok = ok && hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0) == hipSuccess;
+                        if (graph) (void)hipGraphDestroy(graph);
+                        clear_error();
+                        if (ok) {
+                            exec = ex;
+                            captured = true;
+                        } else {
+                            if (ex) (void)hipGraphExecDestroy(ex);
+                            (void)hipGetLastError();
+                            capture_failed = true;
+                        }
+                    }
+                    if (r == COGCONV_OK) {
+                        if (exec) r = check_hip(hipGraphLaunch(exec, s), "graph launch");
+                        else r = launch_table(plan, dev_tables[si], x16, in1, wsp, pk != SIZE_MAX, s);
+                    }
+                } else if (pk != SIZE_MAX) {
+                    r = cogconv_launch_packed(plan, h->r.host_in0, h->r.host_out, wsp,
                                               reinterpret_cast<cogconv_stream>(s));
-                else
-                    r = cogconv_launch(plan, h->r.host_in0, in1, h->r.host_out, ws ? arena.base + o_ws : nullptr,
+                } else {
+                    r = cogconv_launch(plan, h->r.host_in0, in1, h->r.host_out, wsp,
                                        reinterpret_cast<cogconv_stream>(s));
+                }
                 if (r == COGCONV_OK) e = hipEventRecord(ev, s);
                 return r;
             });
+            if (capture) {
+                ge->capturing = false;
+                if (captured) ge->exec = exec;
+                if (capture_failed) ge->failed = true;
+            }
             if (st != COGCONV_OK || e != hipSuccess) {
                 (void)hipStreamSynchronize(s);
-                for (auto& b : blocks) arena.release(b.first, b.second);
+                for (auto& bk : blocks) arena.release(bk.first, bk.second);
+                if (slot >= 0) free_slots.push_back(slot);
                 if (ev) event_pool.push_back(ev);
                 fail_reqs(g, st ? st : COGCONV_ERR_RUNTIME, st ? cogconv_last_error() : hipGetErrorString(e));
                 return true;
             }
-            inflight.push_back(new Launch{g, si, ev, blocks});
+            if (exec) graph_launches++;
+            inflight.push_back(new Launch{g, si, ev, blocks, slot});
             launches++;
+            if (g.size() > 1) stacked += g.size();
             return true;
         }
         const bool sendIn1 = h->r.bank < 0;
@@ -396,6 +499,7 @@ struct cogconv_server_s {
                     }
                 }
                 for (auto& b : L->blocks) arena.release(b.first, b.second);
+                if (L->slot >= 0) free_slots.push_back(L->slot);
                 event_pool.push_back(L->ev);
                 delete L;
                 inflight.erase(inflight.begin() + i);
@@ -411,8 +515,7 @@ struct cogconv_server_s {
                 Req* h = pending.front();
                 std::vector<Req*> g{h};
                 std::vector<size_t> idx{0};
-                if (stackable(h->r.desc.mode) && cfg.max_stack > 1 && h->r.bank >= 0 &&
-                    !(h->r.flags & COGCONV_REQUEST_RESIDENT)) {
+                if (stackable(h->r.desc.mode) && cfg.max_stack > 1 && h->r.bank >= 0 && stack_ok(h)) {
                     const std::string key = bucket_key(h);
                     for (size_t j = 1; j < pending.size() && (int)g.size() < cfg.max_stack; j++) {
                         if (bucket_key(pending[j]) == key) {
@@ -471,7 +574,8 @@ extern "C" int cogconv_server_create(const cogconv_server_config* config, cogcon
     clear_error();
     if (!config || !server) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
     *server = nullptr;
-    if (config->hbm_budget_bytes < 4096 || config->max_inflight < 1 || config->streams < 1 || config->max_stack < 1)
+    if (config->hbm_budget_bytes < 4096 || config->max_inflight < 1 || config->streams < 1 || config->max_stack < 1 ||
+        config->graph_after < 0)
         return fail(COGCONV_ERR_INVALID_ARG, "bad server configuration");
     int e = cogconv_init(config->device);
     if (e) return e;
@@ -496,6 +600,27 @@ extern "C" int cogconv_server_create(const cogconv_server_config* config, cogcon
         }
         s->streams.push_back(st);
     }
+    for (int i = 0; i < config->streams; i++) {
+        void* t = nullptr;
+        e = check_hip(hipMalloc(&t, sizeof(cogconv_image_io) * config->max_stack), "request table");
+        if (e) {
+            (void)cogconv_server_destroy(s);
+            return e;
+        }
+        s->dev_tables.push_back(static_cast<cogconv_image_io*>(t));
+    }
+    {
+        void* h = nullptr;
+        e = check_hip(hipHostMalloc(&h, sizeof(cogconv_image_io) * config->max_stack * config->max_inflight,
+                                    hipHostMallocDefault),
+                      "pinned request tables");
+        if (e) {
+            (void)cogconv_server_destroy(s);
+            return e;
+        }
+        s->host_tables = static_cast<cogconv_image_io*>(h);
+        for (int i = config->max_inflight - 1; i >= 0; i--) s->free_slots.push_back(i);
+    }
     const int nw = std::min(config->streams, 3);
     for (int i = 0; i < nw; i++) s->workers.emplace_back([s] { s->run(); });
     *server = s;
@@ -512,6 +637,10 @@ extern "C" int cogconv_server_destroy(cogconv_server s) {
     s->cv_work.notify_all();
     for (auto& w : s->workers)
         if (w.joinable()) w.join();
+    for (auto& kv : s->graphs)
+        if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+    for (auto t : s->dev_tables) (void)hipFree(t);
+    if (s->host_tables) (void)hipHostFree(s->host_tables);
     for (auto st : s->streams) (void)hipStreamDestroy(st);
     for (auto ev : s->event_pool) (void)hipEventDestroy(ev);
     for (auto& b : s->banks) (void)hipFree(b.dptr);
@@ -674,6 +803,7 @@ extern "C" int cogconv_server_stats_get(cogconv_server s, cogconv_server_stats* 
     out->rejected = s->rejected;
     out->launches = s->launches;
     out->stacked = s->stacked;
+    out->graph_launches = s->graph_launches;
     out->first_submit_s = s->first_submit;
     out->last_complete_s = s->last_complete;
     out->gflop = s->gflop;
@@ -704,7 +834,7 @@ extern "C" int cogconv_server_stats_reset(cogconv_server s) {
     if (!s) return fail(COGCONV_ERR_INVALID_ARG, "null server");
     std::lock_guard<std::mutex> lk(s->mu);
     s->lat_ms.clear();
-    s->submitted = s->completed = s->rejected = s->launches = s->stacked = 0;
+    s->submitted = s->completed = s->rejected = s->launches = s->stacked = s->graph_launches = 0;
     s->first_submit = s->last_complete = s->gflop = 0;
     s->arena.peak = s->arena.in_use;
     for (auto it = s->reqs.begin(); it != s->reqs.end();) {

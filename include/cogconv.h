@@ -127,6 +127,12 @@ typedef struct {
 } cogconv_desc;
 
 typedef struct cogconv_plan_s* cogconv_plan;
+
+/* One image of a table launch: its operand-1 planes and its output planes. */
+typedef struct {
+    const float* in0;
+    float* out;
+} cogconv_image_io;
 typedef struct cogconv_stream_s* cogconv_stream; /* NULL = the device's default stream */
 typedef struct cogconv_event_s* cogconv_event;
 
@@ -218,6 +224,85 @@ int cogconv_plan_describe(cogconv_plan plan, char* buf, size_t len);
 int cogconv_launch(cogconv_plan plan, const float* in0, const float* in1, float* out, void* workspace,
                    cogconv_stream stream);
 
+/* ---- elementwise epilogue (merging of elementwise sinks into the convolution) ----
+ * Replaces the kernel merger's forward merge of a convolution into its elementwise
+ * sink (HyperKernel.canMergeWithSink, $FR/HyperKernel.scala:478-530, applied by
+ * src/main/scala/cogx/compiler/optimizer/HyperKernelMerger.scala under
+ * Cog.localMemoryMerging, $M/parameters/Cog.scala:72).  The epilogue's ops run in
+ * order on each output element before its store, so a conv -> bias -> nonlinearity
+ * chain writes the output once.  Ops follow the reference's elementwise operators:
+ * Field +,-,*,/ with a Float or a Field (Field.scala:180-390), max/min with a Float
+ * or a Field, abs, exp, log, sq, sqrt, tanh, signum, unary minus
+ * (CogFunctionAPI.scala:109-303, 1574-1590).  R* ops take the constant or field
+ * as the left operand (c - x, c / x).  Field operands follow the reference's
+ * broadcast rule (src/main/scala/cogx/compiler/parser/semantics/ResultType.scala:29-58):
+ * the output's own type, a ScalarField of its field shape (broadcast over planes),
+ * a 0-D field with its planes (one value per plane), or a 0-D scalar. */
+enum {
+    COGCONV_EPI_ADD_CONST = 1,
+    COGCONV_EPI_SUB_CONST = 2,
+    COGCONV_EPI_RSUB_CONST = 3,
+    COGCONV_EPI_MUL_CONST = 4,
+    COGCONV_EPI_DIV_CONST = 5,
+    COGCONV_EPI_RDIV_CONST = 6,
+    COGCONV_EPI_MAX_CONST = 7,
+    COGCONV_EPI_MIN_CONST = 8,
+    COGCONV_EPI_NEG = 16,
+    COGCONV_EPI_ABS = 17,
+    COGCONV_EPI_SQ = 18,
+    COGCONV_EPI_SQRT = 19,
+    COGCONV_EPI_EXP = 20,
+    COGCONV_EPI_LOG = 21,
+    COGCONV_EPI_TANH = 22,
+    COGCONV_EPI_SIGNUM = 23,
+    COGCONV_EPI_ADD_FIELD = 32,
+    COGCONV_EPI_SUB_FIELD = 33,
+    COGCONV_EPI_RSUB_FIELD = 34,
+    COGCONV_EPI_MUL_FIELD = 35,
+    COGCONV_EPI_DIV_FIELD = 36,
+    COGCONV_EPI_RDIV_FIELD = 37,
+    COGCONV_EPI_MAX_FIELD = 38,
+    COGCONV_EPI_MIN_FIELD = 39
+};
+#define COGCONV_EPI_MAX_OPS 8
+#define COGCONV_EPI_MAX_OPERANDS 4
+
+typedef struct {
+    int32_t op;      /* COGCONV_EPI_* */
+    float value;     /* constant of a *_CONST op */
+    int32_t operand; /* operand slot of a *_FIELD op */
+} cogconv_epilogue_op;
+
+typedef struct {
+    int32_t n_ops;
+    cogconv_epilogue_op ops[COGCONV_EPI_MAX_OPS];
+    int32_t n_operands;
+    cogconv_field_type operands[COGCONV_EPI_MAX_OPERANDS];
+} cogconv_epilogue;
+
+/* Attach an epilogue to a plan (NULL or n_ops == 0 removes it).  Checks every op
+ * and operand type against the plan's output type (COGCONV_ERR_ILLEGAL_OP when
+ * the reference's broadcast rule does not give the output type). */
+int cogconv_plan_set_epilogue(cogconv_plan plan, const cogconv_epilogue* epi);
+/* cogconv_launch for a plan with an epilogue: operands[i] is the device buffer of
+ * operand slot i.  cogconv_launch refuses a plan whose epilogue reads fields. */
+int cogconv_launch_epilogue(cogconv_plan plan, const float* in0, const float* in1, float* out, void* workspace,
+                            const float* const* operands, cogconv_stream stream);
+/* The same ops as their own pass: out = epilogue(in) over a field of `type`
+ * (in == out allowed).  The unmerged form of an elementwise sink. */
+int cogconv_elementwise(const float* in, float* out, const cogconv_field_type* type, const cogconv_epilogue* epi,
+                        const float* const* operands, cogconv_stream stream);
+
+/* Separable convolution (CogFunctionAPI.crossCorrelateSeparable / convolveSeparable,
+ * src/main/scala/cogx/api/CogFunctionAPI.scala:1277-1356; Field.scala:1500-1531):
+ * a 2-D ScalarField, an odd 1-D row filter, then an odd 1-D column filter, with
+ * BorderClamp, BorderZero or BorderCyclic, in one kernel (the reference runs
+ * ConvolveRows2DHyperKernel then ConvolveColumns2DHyperKernel).  Filters of up to
+ * 63 taps each; `epi` (may be NULL) is applied at the stores. */
+int cogconv_separable(const float* in, const cogconv_field_type* type, const float* row_filter, int32_t row_taps,
+                      const float* column_filter, int32_t column_taps, int32_t border, int32_t orientation, float* out,
+                      const cogconv_epilogue* epi, const float* const* operands, cogconv_stream stream);
+
 /* Resident operand 2 (no counterpart in the reference: for callers that apply one
  * filter bank to many inputs, such as the serving loop).  The BF16 / FP16 /
  * split-BF16 frame plans convert operand 2 into a packed low-precision form in
@@ -229,6 +314,21 @@ int cogconv_launch(cogconv_plan plan, const float* in0, const float* in1, float*
 int cogconv_plan_pack_operand(cogconv_plan plan, const float* in1, void* workspace, cogconv_stream stream,
                               int32_t* packed);
 int cogconv_launch_packed(cogconv_plan plan, const float* in0, float* out, void* workspace, cogconv_stream stream);
+
+/* Request-table launch (no counterpart in the reference: for the serving loop,
+ * which stacks requests of one shape whose fields live at unrelated device
+ * addresses).  The plan's batch B images read their operand-1 planes from
+ * host_table[b].in0 and write their output planes to host_table[b].out instead of
+ * consecutive planes of one buffer.  The table is copied to `device_table` (B
+ * entries of device memory) on `stream` before the launch; with pinned host memory
+ * that copy is asynchronous and host_table must stay unchanged until the stream
+ * reaches it.  packed = 1 runs on an operand 2 packed earlier into `workspace`
+ * (cogconv_plan_pack_operand; in1 unused).  Offered for the MFMA frame plans
+ * without up-sampling (cogconv_plan_table_supported), otherwise
+ * COGCONV_ERR_UNSUPPORTED. */
+int cogconv_plan_table_supported(cogconv_plan plan, int32_t* supported);
+int cogconv_launch_table(cogconv_plan plan, const cogconv_image_io* host_table, cogconv_image_io* device_table,
+                         const float* in1, void* workspace, int32_t packed, cogconv_stream stream);
 
 /* ---- autotuning ----
  * Counterpart of Profiler.bestVariant ($M/runtime/execution/Profiler.scala:131-190) and
@@ -273,6 +373,46 @@ int cogconv_tune_cache_load(const char* path);
  * `points` = rows*cols of one plane. */
 int cogconv_block_reduce_sum(const float* in, float* out, int64_t points, int32_t in_planes, int32_t factor,
                              cogconv_stream stream);
+/* blockReduceSum with an elementwise epilogue on its stores (the reduce kernel
+ * merged with its elementwise sinks, as cogconv_plan_set_epilogue does for a
+ * convolution); `in_type` is the input field, the result has planes/factor planes. */
+int cogconv_block_reduce_sum_epilogue(const float* in, float* out, const cogconv_field_type* in_type, int32_t factor,
+                                      const cogconv_epilogue* epi, const float* const* operands,
+                                      cogconv_stream stream);
+
+/* ---- compiled graphs: checkpoint / restore and the evaluate() Function Interface ----
+ * A compiled graph file holds a convolution graph after its optimizers ran: the
+ * fields (types, constant data, which are inputs), and per launch step its
+ * descriptor (ConvolveOp, blockReduceSum factor, separable filters, elementwise
+ * epilogue), in dependence order.  It replaces the reference's checkpoint of the
+ * kernel circuit with OpenCL source (ComputeGraph.write / readFromFile,
+ * $M/runtime/ComputeGraph.scala:523-666, OpenCLRestoredDeviceKernel.scala:28-60):
+ * restoring re-plans every step for the device (cogconv_plan_create, autotuned
+ * choices included) instead of re-running saved kernel code.  The Python
+ * ComputeGraph.write() produces the file.
+ *
+ * The function calls follow the reference's CogFunction interface
+ * (src/main/java/cogjni/CogJNIWrapper.java:85-161): cogconv_function_create
+ * (cogFunctionCreateComputeFunction), input / output counts, names and
+ * FieldTypes, cogconv_function_evaluate (cogFunctionEvaluate2D: one host buffer
+ * per input and per output, planar float32), optimal thread count, name, and
+ * destroy (cogFunctionShutdown).  evaluate() is reentrant: any number of host
+ * threads may call it at once on one function; each call runs on an execution
+ * context of its own (stream and device buffers, at most 4 made, later calls
+ * wait for a free one), and returns when its outputs are in host memory.  Results
+ * are the same whichever threads run beside it. */
+typedef struct cogconv_function_s* cogconv_function;
+
+int cogconv_function_create(const char* path, int32_t device, cogconv_function* function);
+int cogconv_function_destroy(cogconv_function function);
+int cogconv_function_name(cogconv_function function, char* buf, size_t len);
+int cogconv_function_optimal_thread_count(cogconv_function function, int32_t* count);
+int cogconv_function_input_count(cogconv_function function, int32_t* count);
+int cogconv_function_output_count(cogconv_function function, int32_t* count);
+/* Name and FieldType of input (output = 0) or output (output = 1) `index`. */
+int cogconv_function_io(cogconv_function function, int32_t output, int32_t index, char* name, size_t name_len,
+                        cogconv_field_type* type);
+int cogconv_function_evaluate(cogconv_function function, const float* const* inputs, float* const* outputs);
 
 /* ---- serving loop ----
  * No counterpart inside the reference's convolution path: the reference specifies a
@@ -292,6 +432,8 @@ typedef struct {
     int32_t max_inflight;      /* launches on the device at once */
     int32_t streams;           /* HIP streams work is spread over */
     int32_t max_stack;         /* requests stacked into one launch (1 = no stacking) */
+    int32_t graph_after;       /* launches of one (plan, bank, stream) before its launch sequence is captured
+                                  into a hipGraph and replayed (0 = never capture) */
 } cogconv_server_config;
 
 /* Request flags. */
@@ -312,6 +454,7 @@ typedef struct {
     uint64_t submitted, completed, rejected;
     uint64_t launches;         /* kernel launches issued */
     uint64_t stacked;          /* requests that shared a launch with another */
+    uint64_t graph_launches;   /* launches replayed from a captured hipGraph */
     double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> result in host memory */
     double first_submit_s, last_complete_s; /* monotonic clock */
     double requests_per_s;     /* completed / (last_complete - first_submit) */

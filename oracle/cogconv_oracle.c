@@ -405,6 +405,66 @@ void oracle_upsample(const float* p, int rows, int cols, int ur, int uc, float* 
     free(q);
 }
 
+/*
+ * Separable convolution of one 2-D plane (CogFunctionAPI.scala:1277-1316):
+ * ConvolveRows2DHyperKernel.scala:73-117 (row pass, border on the column index)
+ * then ConvolveColumns2DHyperKernel (column pass over the row-pass result,
+ * border on the row index).  The tap i of a filter of length F reads offset
+ * i - (F-1)/2; the convolution orientation reverses the filter.  Each pass is
+ * materialised in full, in double (prec 0) or in float in tap order (prec 1).
+ * border: OB_ZERO, OB_CLAMP or OB_CYCLIC.
+ */
+static double sep_read(const double* a, int rows, int cols, int y, int x, int border) {
+    if (border == OB_CLAMP) {
+        y = y < 0 ? 0 : (y >= rows ? rows - 1 : y);
+        x = x < 0 ? 0 : (x >= cols ? cols - 1 : x);
+    } else if (border == OB_CYCLIC) {
+        y %= rows; if (y < 0) y += rows;
+        x %= cols; if (x < 0) x += cols;
+    } else if (y < 0 || y >= rows || x < 0 || x >= cols) {
+        return 0.0;
+    }
+    return a[(size_t)y * cols + x];
+}
+
+int oracle_separable(const float* in, int rows, int cols, const float* rf, int fr, const float* cf, int fc, int border,
+                     int orient, int prec, double* out) {
+    if (fr < 1 || fc < 1 || fr % 2 == 0 || fc % 2 == 0) return OE_EVEN_FILTER;
+    if (border != OB_ZERO && border != OB_CLAMP && border != OB_CYCLIC) return OE_ARG;
+    size_t n = (size_t)rows * cols;
+    double* a = (double*)malloc(n * sizeof(double));
+    double* mid = (double*)malloc(n * sizeof(double));
+    for (size_t q = 0; q < n; q++) a[q] = in[q];
+    int hr = (fr - 1) / 2, hc = (fc - 1) / 2;
+    for (int r = 0; r < rows; r++)
+        for (int c = 0; c < cols; c++) {
+            double sd = 0.0;
+            float sf = 0.0f;
+            for (int i = 0; i < fr; i++) {
+                float t = rf[orient == OO_CONV ? fr - 1 - i : i];
+                double v = sep_read(a, rows, cols, r, c - hr + i, border);
+                sd += (double)t * v;
+                sf += (float)v * t;
+            }
+            mid[(size_t)r * cols + c] = prec == 0 ? sd : (double)sf;
+        }
+    for (int r = 0; r < rows; r++)
+        for (int c = 0; c < cols; c++) {
+            double sd = 0.0;
+            float sf = 0.0f;
+            for (int j = 0; j < fc; j++) {
+                float t = cf[orient == OO_CONV ? fc - 1 - j : j];
+                double v = sep_read(mid, rows, cols, r - hc + j, c, border);
+                sd += (double)t * v;
+                sf += (float)v * t;
+            }
+            out[(size_t)r * cols + c] = prec == 0 ? sd : (double)sf;
+        }
+    free(a);
+    free(mid);
+    return OE_OK;
+}
+
 int oracle_max_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -69,6 +69,9 @@ def lib():
         L.oracle_plane_pair.argtypes = [ctypes.POINTER(Desc), ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.oracle_max_threads.restype = ctypes.c_int
+        L.oracle_separable.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p]
         _LIB = L
     return _LIB
 
@@ -144,6 +147,20 @@ def upsample(p, ur, uc):
     p = np.ascontiguousarray(p, dtype=np.float32)
     out = np.empty((p.shape[0] * ur, p.shape[1] * uc), dtype=np.float32)
     lib().oracle_upsample(p.ctypes.data, p.shape[0], p.shape[1], ur, uc, out.ctypes.data)
+    return out
+
+
+def separable(x, row_filter, column_filter, border, orient, prec="f64"):
+    """Separable convolution of a 2-D plane: the row pass, then the column pass
+    (float64 result; with prec="f32" each pass is summed in float in tap order)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    rf = np.ascontiguousarray(row_filter, dtype=np.float32).ravel()
+    cf = np.ascontiguousarray(column_filter, dtype=np.float32).ravel()
+    out = np.empty(x.shape, dtype=np.float64)
+    e = lib().oracle_separable(x.ctypes.data, x.shape[0], x.shape[1], rf.ctypes.data, rf.size, cf.ctypes.data,
+                               cf.size, border, orient, 0 if prec == "f64" else 1, out.ctypes.data)
+    if e:
+        raise OracleError(e)
     return out
 
 

@@ -6,6 +6,7 @@
 - plans build on the host and report the algorithmic FLOPs of the configs;
 - the Field / ComputeGraph API raises at graph construction on illegal ops.
 """
+import ctypes
 import itertools
 import os
 import re
@@ -151,3 +152,60 @@ def test_illegal_ops_raise_at_graph_construction():
         # Up/Down(1) behave as NoSampling (CogFunctionAPI.scala:1140-1144)
         z = cogconv.projectFrame(xf, odd, cogconv.BorderZero, cogconv.DownsampleOutputConvolution(1))
         assert z.shape == (6, 8, 8)
+
+
+def test_epilogue_checks_are_host_side():
+    """cogconv_plan_set_epilogue checks ops and operand types against the output
+    type (ResultType.scala:29-58) without a device."""
+    import cogconv._lib as L
+    desc = L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CROSS_CORRELATION, L.NO_SAMPLING, 1, 2,
+                       (2, 16, 16, 8), (2, 3, 3, 16))
+    plan = L.Plan(desc)  # output: 2 images x 4 planes of 16x16 = 8 planes
+    ok = [(2, 16, 16, 8, 1), (0, 1, 1, 8, 1), (2, 16, 16, 1, 0), (0, 1, 1, 1, 0)]
+    for t in ok:
+        plan.set_epilogue(L.make_epilogue([(L.EPI_ADD_FIELD, 0.0, 0)], [t]))
+    for t in [(2, 16, 16, 3, 1), (2, 8, 16, 8, 1), (0, 1, 1, 3, 1)]:
+        with pytest.raises(L.CogconvError) as e:
+            plan.set_epilogue(L.make_epilogue([(L.EPI_ADD_FIELD, 0.0, 0)], [t]))
+        assert e.value.status == L.ERR_ILLEGAL_OP
+    with pytest.raises(L.CogconvError) as e:
+        plan.set_epilogue(L.make_epilogue([(99, 0.0, 0)]))
+    assert e.value.status == L.ERR_INVALID_ARG
+    with pytest.raises(L.CogconvError) as e:
+        plan.set_epilogue(L.make_epilogue([(L.EPI_MUL_FIELD, 0.0, 1)], [(2, 16, 16, 8, 1)]))
+    assert e.value.status == L.ERR_INVALID_ARG
+    plan.set_epilogue(None)
+    plan.close()
+
+
+def test_graph_file_written_and_parse_errors(tmp_path):
+    """ComputeGraph.write produces the documented text header + data layout, and
+    cogconv_function_create rejects malformed files before touching a device."""
+    rng = np.random.default_rng(9)
+    cg = cogconv.ComputeGraph()
+    with cg:
+        x = cogconv.VectorField(rng.uniform(-1, 1, (4, 10, 10)).astype(np.float32))
+        w = cogconv.VectorField(rng.uniform(-1, 1, (8, 3, 3)).astype(np.float32))
+        y = cogconv.max(cogconv.blockReduceSum(cogconv.projectFrame(x, w, cogconv.BorderZero, batchSize=2), 2), 0.0)
+        cogconv.probe(y)
+    path = cg.write(str(tmp_path / "g.cgraph"), inputs=[x], name="tiny")
+    raw = open(path, "rb").read()
+    head, data = raw.split(b"end\n", 1)
+    lines = head.decode().splitlines()
+    assert lines[0] == "COGCONV-GRAPH 1 0" and lines[1] == "name tiny"
+    assert [ln.split()[7] for ln in lines if ln.startswith("field")] == ["input", "const", "computed"]
+    step = [ln for ln in lines if ln.startswith("step")]
+    assert len(step) == 1 and step[0].split()[1] == "conv"
+    assert int(step[0].split()[5]) == L.PROJECT_FRAME_BLOCK_REDUCE_SUM
+    assert [ln.split()[1] for ln in lines if ln.startswith("epi")] == [str(L.EPI_MAX_CONST)]
+    assert np.array_equal(np.frombuffer(data, "<f4"), cg._fields[1].data.ravel())
+    for bad in (b"NOT-A-GRAPH\n", b"COGCONV-GRAPH 2 0\nend\n",
+                b"COGCONV-GRAPH 1 0\nfield 0 2 4 4 1 0 input a\nstep reduce 1 0 2\nend\n",
+                b"COGCONV-GRAPH 1 0\nfield 0 2 4 4 2 1 input a\nfield 1 2 4 4 1 0 computed b\n"
+                b"step elementwise 1 0\nend\n",
+                b"COGCONV-GRAPH 1 0\nfield 0 2 4 4 1 0 const a\nend\n"):
+        p = tmp_path / "bad.cgraph"
+        p.write_bytes(bad)
+        h = ctypes.c_void_p()
+        st = L.lib().cogconv_function_create(str(p).encode(), 0, ctypes.byref(h))
+        assert st == L.ERR_INVALID_ARG, (bad, st, L.last_error())

@@ -359,3 +359,68 @@ def test_brute_force_tiny_all_borders():
                                 s += float(t) * float(a[yy // up, xx // up])
                         want[R, C] = s
                 assert np.abs(got - want).max() <= 1e-12, (border, sampling, orient)
+
+
+# ---- separable convolution (CogFunctionAPI.scala:1277-1356) ----------------------------------
+
+SEP_ROW = np.array([0.1, 0.3, 0.4, 0.2, 0.0], np.float32)   # DynamicConvolutionSpec.scala:75-77
+SEP_COL = np.array([0.0, 0.5, -0.4, -0.2, -0.1], np.float32)
+
+
+@pytest.mark.parametrize("border", [O.ZERO, O.CLAMP, O.CYCLIC])
+@pytest.mark.parametrize("orient", [O.XCORR, O.CONV])
+def test_separable_equals_outer_product_convolution(border, orient):
+    """The reference's own check (DynamicConvolutionSpec.scala:74-110, there with
+    BorderZero): separable(a, row, col) == convolve(a, col outer row); the identity
+    holds for every border whose padding is per coordinate."""
+    rng = np.random.default_rng(40 + border * 2 + orient)
+    for size, rf, cf in ((10, SEP_ROW, SEP_COL), (17, rng.uniform(-1, 1, 7), rng.uniform(-1, 1, 3)),
+                         (23, rng.uniform(-1, 1, 3), rng.uniform(-1, 1, 9))):
+        a = rng.uniform(-1, 1, (size, size + 3)).astype(np.float32)
+        rf = np.asarray(rf, np.float32)
+        cf = np.asarray(cf, np.float32)
+        square = np.outer(cf, rf)  # float32 taps, as the reference's outer product (Matrix of Float)
+        got = O.separable(a, rf, cf, border, orient)
+        # 2-D reference in float64 from the definition (outer-product taps, per-coordinate border)
+        d = O.make_desc(O.PBP, border, orient, O.NONE, 1, 1, (1,) + a.shape, (1,) + square.shape)
+        ref2d = O.conv(d, a, square.astype(np.float32))[0]
+        # the float32 outer product rounds the taps: compare in the rounding of the taps
+        assert np.abs(got - ref2d).max() <= 1e-6 * max(1.0, np.abs(ref2d).max())
+
+
+@pytest.mark.parametrize("border,pad", [(O.ZERO, "constant"), (O.CLAMP, "replicate"), (O.CYCLIC, "circular")])
+def test_separable_matches_torch(border, pad):
+    """torch conv2d on the padded plane with the outer-product weight (xcorr),
+    and the two passes as separate 1-D convolutions."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(7)
+    a = rng.uniform(-1, 1, (19, 26)).astype(np.float32)
+    rf = rng.uniform(-1, 1, 5).astype(np.float32)
+    cf = rng.uniform(-1, 1, 7).astype(np.float32)
+    x = torch.from_numpy(a.astype(np.float64))[None, None]
+    xp = torch.nn.functional.pad(x, (2, 2, 3, 3), mode=pad)
+    w = torch.from_numpy(np.outer(cf.astype(np.float64), rf.astype(np.float64)))[None, None]
+    ref = torch.nn.functional.conv2d(xp, w)[0, 0].numpy()
+    got = O.separable(a, rf, cf, border, O.XCORR)
+    assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    # convolution orientation = cross-correlation with both filters reversed
+    got_c = O.separable(a, rf[::-1], cf[::-1], border, O.CONV)
+    assert np.abs(got_c - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def test_separable_brute_force_and_errors():
+    """Tiny case written out tap by tap; even filters are rejected."""
+    a = np.arange(12, dtype=np.float32).reshape(3, 4)
+    rf = np.array([1.0, 2.0, 3.0], np.float32)
+    cf = np.array([0.0, 1.0, -1.0], np.float32)
+    mid = np.zeros((3, 4))
+    for r in range(3):
+        for c in range(4):
+            mid[r, c] = sum(rf[i] * (a[r, c - 1 + i] if 0 <= c - 1 + i < 4 else 0.0) for i in range(3))
+    out = np.zeros((3, 4))
+    for r in range(3):
+        for c in range(4):
+            out[r, c] = sum(cf[j] * (mid[r - 1 + j, c] if 0 <= r - 1 + j < 3 else 0.0) for j in range(3))
+    assert np.array_equal(O.separable(a, rf, cf, O.ZERO, O.XCORR), out)
+    with pytest.raises(O.OracleError):
+        O.separable(a, np.ones(4, np.float32), cf, O.ZERO, O.XCORR)
