@@ -113,8 +113,8 @@ int parse(std::istream& in, cogconv_function_s* fn) {
         ls >> magic >> major >> minor;
         if (magic != "COGCONV-GRAPH") return fail(COGCONV_ERR_INVALID_ARG, "not a cogconv graph file");
         if (major != kMajor)
-            return fail(COGCONV_ERR_INVALID_ARG, "graph file spec %d.%d, this library reads %d.x", major, minor,
-                        kMajor);
+            return fail(COGCONV_ERR_INVALID_ARG, "graph file spec %d.%d, this library reads %d.%d", major, minor,
+                        kMajor, kMinor);
     }
     bool ended = false;
     while (!ended && std::getline(in, line)) {

@@ -34,11 +34,13 @@ constexpr int LP_RC = 32;   // planes per staged chunk (one MFMA k slice)
 // Lane byte offset with bit 31 set: past every staged buffer (each is under
 // 2^31 bytes), so the buffer unit's range check answers the load with 0.
 constexpr int LP_OOB = (int)0x80000000u;
-constexpr int LP_RCP = 40;  // LDS pitch of a staged pixel / filter row, in 16-bit elements
 
-// Filter pre-pack: Wp[term][t][m][r] in the MFMA input type, m padded to
-// Mpad, r padded to Rpad (zeros), taps flipped for the convolution
-// orientation.  term 0 = rounded value, term 1 (split only) = rounded residual.
+// Filter pre-pack: Wp[term][chunk][t][q][m][e] in the MFMA input type, for
+// input plane r = 32 chunk + 8 q + e (zeros past the planes), output m
+// padded to Mpad (zeros), taps flipped for the convolution orientation:
+// the order in which a block stages a tap group into LDS, so that staging is
+// a contiguous copy.  term 0 = rounded value, term 1 (split only) = rounded
+// residual.
 template <int PREC>
 __global__ void k_pack_filters_lp(Geom g, PlaneMap m, int Mout, int Mpad, int Rpad, const float* __restrict__ Wt,
                                   typename LpType<PREC>::T* __restrict__ Wp) {
@@ -48,10 +50,14 @@ __global__ void k_pack_filters_lp(Geom g, PlaneMap m, int Mout, int Mpad, int Rp
     const bool bpf = m.mode == COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM;
     const int L = m.T / m.N;
     for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-        const int r = (int)(idx % Rpad);
-        const long rest = idx / Rpad;
+        const int e8 = (int)(idx & 7);
+        long rest = idx >> 3;
         const int mm = (int)(rest % Mpad);
-        const int t = (int)(rest / Mpad);
+        rest /= Mpad;
+        const int q = (int)(rest & 3);
+        rest >>= 2;
+        const int t = (int)(rest % taps);
+        const int r = (int)(rest / taps) * LP_RC + 8 * q + e8;
         float w = 0.f;
         if (r < m.R && mm < Mout) {
             const int pf = bpf ? r * L + mm : mm * m.N + r;
@@ -77,11 +83,18 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     extern __shared__ __attribute__((aligned(16))) unsigned char lp_smem[];
     const int TRp = c.tileRows, TCp = c.tileCols;
     const int npix = TRp * TCp;
-    T* sB = reinterpret_cast<T*>(lp_smem);                    // [TERMS][npix][RCP]
-    T* sA = sB + (size_t)TERMS * npix * LP_RCP;               // [TERMS][TAPS][MT][RCP]
+    // LDS holds both operands quarter-major: 8 planes (16 bytes) of one pixel
+    // or output plane per slot, the four 8-plane quarters of a 32-plane chunk
+    // in separate planes of slots, so that the 16 lanes of an MFMA row group
+    // read 16 consecutive slots (no padding, no bank conflicts).  The filter
+    // slice holds TRG tap rows at a time (all F when they fit).
+    const int TRG = c.trg;
+    const int TAPG = TRG * F;
+    T* sB = reinterpret_cast<T*>(lp_smem);                    // [TERMS][4][npix][8]
+    T* sA = sB + (size_t)TERMS * 4 * npix * 8;               // [TERMS][TAPG][4][MT][8]
     // Scratch slots (offset from sB) for the stores of lanes beyond the tile
     // width, after the filter slice: TERMS x 64 lanes x 8 elements.
-    const size_t sJunk = (size_t)TERMS * npix * LP_RCP + (size_t)TERMS * TAPS * MT * LP_RCP;
+    const size_t sJunk = (size_t)TERMS * 4 * npix * 8 + (size_t)TERMS * TAPG * 4 * MT * 8;
 
     int bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tc = bid % c.tilesC;
@@ -110,6 +123,7 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     const int Mpad = c.mTiles * MT;
     const int Rpad = c.RC;  // planes padded to a multiple of 32
     const long wTerm = (long)TAPS * Mpad * Rpad;
+    const int wChunkBytes = 2 * TAPS * 4 * Mpad * 8;  // one 32-plane chunk of one term
     const __amdgpu_buffer_rsrc_t wr = buffer_rsrc(Wp, (uint32_t)(TERMS * wTerm * 2));  // the plan bounds this
 
     // Per-lane column offset of the staged tile (TCp <= 64) with the border applied.
@@ -143,29 +157,36 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     const bool gIn = si < IP && (unsigned)colA < (unsigned)g.W;
     const float invTRp = 1.0f / TRp;
 
-    // Filter-slice units (term, tap, output plane, 8-plane quarter) of this
-    // thread: byte offsets in the packed bank (the chunk adds 2*r0 as the
-    // uniform offset) and LDS destinations (elements from sB; units past the
-    // end go to the scratch slot), computed once when there are few per thread.
+    // Filter-slice units (term, tap, 8-plane quarter, output plane) of this
+    // thread, output plane fastest (consecutive threads copy consecutive 16
+    // bytes of the packed bank into consecutive LDS slots): byte offsets in
+    // the packed bank (the chunk adds its offset as the uniform offset) and
+    // LDS destinations (elements from sB; units past the end go to the scratch
+    // slot).  With the whole filter in one group they are computed once and
+    // their loads issued with the input tile's.
     constexpr int UNITS = TERMS * TAPS * MT * 4;
     constexpr int WPT = (UNITS + 255) / 256;
     constexpr bool WPRE = WPT <= 12;
+    const bool onePass = TRG == F;  // block-uniform
     int wOff[WPRE ? WPT : 1], wDst[WPRE ? WPT : 1];
-    if (WPRE) {
+    auto unit_of = [&](int u, int tap0, int& off, int& dst) {
+        const int uc = min(u, TERMS * TAPG * MT * 4 - 1);
+        const int mm = uc % MT;
+        const int rest = uc / MT;
+        const int qq = rest & 3;
+        const int rest2 = rest >> 2;
+        const int tg = rest2 % TAPG;
+        const int term = rest2 / TAPG;
+        // Taps past the filter (the last group of a multi-group slice) read 0 through
+        // an out-of-range offset; no MFMA uses them.
+        off = tap0 + tg < TAPS ? 2 * (int)(term * wTerm + ((long)((tap0 + tg) * 4 + qq) * Mpad + m0 + mm) * 8)
+                               : LP_OOB;
+        dst = u < TERMS * TAPG * MT * 4 ? (int)((size_t)TERMS * 4 * npix * 8 + (size_t)uc * 8)
+                                        : (int)(sJunk + 8 * lane);
+    };
+    if (WPRE && onePass) {
 #pragma unroll
-        for (int i = 0; i < (WPRE ? WPT : 1); i++) {
-            const int q = tid + 256 * i;
-            const int qc = min(q, UNITS - 1);
-            const int qq = qc & 3;
-            const int rest = qc >> 2;
-            const int mm = rest % MT;
-            const int rest2 = rest / MT;
-            const int t = rest2 % TAPS;
-            const int term = rest2 / TAPS;
-            wOff[i] = 2 * (int)(term * wTerm + ((long)t * Mpad + m0 + mm) * Rpad + 8 * qq);
-            wDst[i] = q < UNITS ? (int)((size_t)TERMS * npix * LP_RCP + ((size_t)(term * TAPS + t) * MT + mm) * LP_RCP + 8 * qq)
-                                : (int)(sJunk + 8 * lane);
-        }
+        for (int i = 0; i < (WPRE ? WPT : 1); i++) unit_of(tid + 256 * i, 0, wOff[i], wDst[i]);
     }
 
     // Per-lane table of the staged tile's row offsets (lane = tile row, at
@@ -198,7 +219,7 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     for (int pb = 0; pb < 4; pb++) {
         const int row = 2 * wave + (pb >> 1);
         const int col = (pb & 1) * 16 + l16;
-        boff[pb] = ((row * g.dnR) * TCp + col * g.dnC) * LP_RCP + 8 * g4;
+        boff[pb] = (g4 * npix + (row * g.dnR) * TCp + col * g.dnC) * 8;
     }
 
     // Input tile: items (8-plane group, row); each lane loads the eight planes
@@ -249,21 +270,22 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
                     hv[e] = h;
                     if (SPLIT) lv[e] = (T)(v[u][e] - (float)h);
                 }
-                const size_t o = (size_t)(ty * TCp + lane) * LP_RCP + 8 * grp;
+                const size_t o = ((size_t)grp * npix + ty * TCp + lane) * 8;
                 *reinterpret_cast<V*>(sB + (inTile ? o : sJunk + 8 * lane)) = hv;
-                if (SPLIT) *reinterpret_cast<V*>(sB + (inTile ? (size_t)npix * LP_RCP + o : sJunk + 512 + 8 * lane)) = lv;
+                if (SPLIT) *reinterpret_cast<V*>(sB + (inTile ? (size_t)4 * npix * 8 + o : sJunk + 512 + 8 * lane)) = lv;
             }
             ty += 4;
             if (ty >= TRp) { ty -= TRp; grp++; }
         }
     };
     for (int r0 = 0; r0 < m.R; r0 += LP_RC) {
+        const int wChunk = (r0 / LP_RC) * wChunkBytes;  // uniform byte offset of the chunk's packed filters
         __syncthreads();
         V wv[WPRE ? WPT : 1];
-        if (WPRE) {
+        if (WPRE && onePass) {
 #pragma unroll
             for (int i = 0; i < (WPRE ? WPT : 1); i++)
-                wv[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, wOff[i], 2 * r0, 0));
+                wv[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, wOff[i], wChunk, 0));
         }
         if (c.vec4) {  // block-uniform
             constexpr int VP = (F + 12) / 6;  // passes: a wave owns TRp = F + 7 items, IP >= 6 a pass
@@ -283,7 +305,7 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
                     q[pp][e] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, vo, 0, 0));
                     vo += planeBytes;
                 }
-                dBase[pp] = itemOk ? ty * TCp * LP_RCP + 8 * grp : -1;
+                dBase[pp] = itemOk ? (grp * npix + ty * TCp) * 8 : -1;
             }
             __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll
@@ -300,9 +322,9 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
                         if (SPLIT) lv[e] = (T)(x - (float)h);
                     }
                     const bool ok = dBase[pp] >= 0 && (unsigned)col < (unsigned)TCp;
-                    const size_t o = (size_t)dBase[pp] + (size_t)col * LP_RCP;
+                    const size_t o = (size_t)dBase[pp] + (size_t)col * 8;
                     *reinterpret_cast<V*>(sB + (ok ? o : sJunk + 8 * lane)) = hv;
-                    if (SPLIT) *reinterpret_cast<V*>(sB + (ok ? (size_t)npix * LP_RCP + o : sJunk + 512 + 8 * lane)) = lv;
+                    if (SPLIT) *reinterpret_cast<V*>(sB + (ok ? (size_t)4 * npix * 8 + o : sJunk + 512 + 8 * lane)) = lv;
                 }
             }
         } else {
@@ -313,75 +335,69 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
                 commit(base, v);
             }
         }
-        // Filter slice from the pre-packed bank: 16-byte units (term, t, m, q).
-        if (WPRE) {
+        for (int tr0 = 0; tr0 < F; tr0 += TRG) {
+            // Filter slice of tap rows tr0 .. tr0 + TRG - 1 from the pre-packed bank.
+            if (WPRE && onePass) {
 #pragma unroll
-            for (int i = 0; i < (WPRE ? WPT : 1); i++) *reinterpret_cast<V*>(sB + wDst[i]) = wv[i];
-        } else {
-            constexpr int WU = 8;
-            for (int q0 = tid; q0 < UNITS; q0 += 256 * WU) {
-                V wb[WU];
+                for (int i = 0; i < (WPRE ? WPT : 1); i++) *reinterpret_cast<V*>(sB + wDst[i]) = wv[i];
+            } else {
+                if (tr0 > 0) __syncthreads();  // the previous group's MFMAs are done with the slice
+                const int units = TERMS * TAPG * MT * 4;
+                constexpr int WU = 8;
+                for (int u0 = tid; u0 < units; u0 += 256 * WU) {
+                    V wb[WU];
+                    int dst[WU];
 #pragma unroll
-                for (int i = 0; i < WU; i++) {
-                    const int q = min(q0 + 256 * i, UNITS - 1);
-                    const int qq = q & 3;
-                    const int rest = q >> 2;
-                    const int mm = rest % MT;
-                    const int rest2 = rest / MT;
-                    const int t = rest2 % TAPS;
-                    const int term = rest2 / TAPS;
-                    const int eoff = (int)(term * wTerm + ((long)t * Mpad + m0 + mm) * Rpad + 8 * qq);
-                    wb[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, 2 * eoff, 2 * r0, 0));
-                }
-#pragma unroll
-                for (int i = 0; i < WU; i++) {
-                    const int q = q0 + 256 * i;
-                    if (q < UNITS) {
-                        const int qq = q & 3;
-                        const int rest = q >> 2;
-                        const int mm = rest % MT;
-                        const int rest2 = rest / MT;
-                        const int t = rest2 % TAPS;
-                        const int term = rest2 / TAPS;
-                        *reinterpret_cast<V*>(sA + ((size_t)(term * TAPS + t) * MT + mm) * LP_RCP + 8 * qq) = wb[i];
+                    for (int i = 0; i < WU; i++) {
+                        int off;
+                        unit_of(u0 + 256 * i, tr0 * F, off, dst[i]);
+                        if (u0 + 256 * i >= units) dst[i] = (int)(sJunk + 8 * lane);
+                        wb[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, off, wChunk, 0));
                     }
+#pragma unroll
+                    for (int i = 0; i < WU; i++) *reinterpret_cast<V*>(sB + dst[i]) = wb[i];
                 }
             }
-        }
-        __syncthreads();
+            __syncthreads();
+            // Taps of this group: sAg addresses tap t = ti * F + tj of the whole
+            // filter, so the per-tap offsets stay compile-time constants.
+            const T* sAg = sA - (size_t)tr0 * F * 4 * MT * 8;
+            const T* sAg2 = sAg + (size_t)TAPG * 4 * MT * 8;  // split: the residual term
+            const int tEnd = min(F, tr0 + TRG);
 #pragma unroll
-        for (int ti = 0; ti < F; ti++) {
+            for (int ti = 0; ti < F; ti++) {
+                if (ti < tr0 || ti >= tEnd) continue;  // block-uniform
 #pragma unroll
-            for (int tj = 0; tj < F; tj++) {
-                const int t = ti * F + tj;
-                const int toff = (ti * TCp + tj) * LP_RCP;
-                V av[MB], bv[4];
-#pragma unroll
-                for (int mb = 0; mb < MB; mb++)
-                    av[mb] = *reinterpret_cast<const V*>(sA + ((size_t)t * MT + mb * 16 + l16) * LP_RCP + 8 * g4);
-#pragma unroll
-                for (int pb = 0; pb < 4; pb++) bv[pb] = *reinterpret_cast<const V*>(sB + boff[pb] + toff);
-                if (SPLIT) {
-                    V al[MB], bl[4];
+                for (int tj = 0; tj < F; tj++) {
+                    const int t = ti * F + tj;
+                    const int toff = (ti * TCp + tj) * 8;
+                    V av[MB], bv[4];
 #pragma unroll
                     for (int mb = 0; mb < MB; mb++)
-                        al[mb] = *reinterpret_cast<const V*>(sA + ((size_t)(TAPS + t) * MT + mb * 16 + l16) * LP_RCP +
-                                                             8 * g4);
+                        av[mb] = *reinterpret_cast<const V*>(sAg + ((size_t)(t * 4 + g4) * MT + mb * 16 + l16) * 8);
 #pragma unroll
-                    for (int pb = 0; pb < 4; pb++)
-                        bl[pb] = *reinterpret_cast<const V*>(sB + (size_t)npix * LP_RCP + boff[pb] + toff);
+                    for (int pb = 0; pb < 4; pb++) bv[pb] = *reinterpret_cast<const V*>(sB + boff[pb] + toff);
+                    if (SPLIT) {
+                        V al[MB], bl[4];
+#pragma unroll
+                        for (int mb = 0; mb < MB; mb++)
+                            al[mb] = *reinterpret_cast<const V*>(sAg2 + ((size_t)(t * 4 + g4) * MT + mb * 16 + l16) * 8);
+#pragma unroll
+                        for (int pb = 0; pb < 4; pb++)
+                            bl[pb] = *reinterpret_cast<const V*>(sB + (size_t)4 * npix * 8 + boff[pb] + toff);
+#pragma unroll
+                        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                            for (int pb = 0; pb < 4; pb++) {
+                                acc[mb][pb] = lp_mfma<PREC>(bv[pb], al[mb], acc[mb][pb]);
+                                acc[mb][pb] = lp_mfma<PREC>(bl[pb], av[mb], acc[mb][pb]);
+                            }
+                    }
 #pragma unroll
                     for (int mb = 0; mb < MB; mb++)
 #pragma unroll
-                        for (int pb = 0; pb < 4; pb++) {
-                            acc[mb][pb] = lp_mfma<PREC>(bv[pb], al[mb], acc[mb][pb]);
-                            acc[mb][pb] = lp_mfma<PREC>(bl[pb], av[mb], acc[mb][pb]);
-                        }
+                        for (int pb = 0; pb < 4; pb++) acc[mb][pb] = lp_mfma<PREC>(bv[pb], av[mb], acc[mb][pb]);
                 }
-#pragma unroll
-                for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-                    for (int pb = 0; pb < 4; pb++) acc[mb][pb] = lp_mfma<PREC>(bv[pb], av[mb], acc[mb][pb]);
             }
         }
     }
@@ -432,8 +448,10 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
 
 size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision) {
     const int terms = precision == COGCONV_SPLIT_BF16 ? 2 : 1;
-    return (size_t)terms * ((size_t)c.tileRows * c.tileCols + (size_t)F * F * 16 * c.MB) * LP_RCP * 2 +
-           (size_t)terms * 64 * 8 * 2;  // scratch slots of the lanes beyond the tile width
+    // tile [4][npix][8] + filter group [trg * F][4][16 MB][8], 16-bit elements,
+    // + scratch slots of the lanes beyond the tile width
+    return (size_t)terms * ((size_t)c.tileRows * c.tileCols + (size_t)c.trg * F * 16 * c.MB) * 64 +
+           (size_t)terms * 64 * 8 * 2;
 }
 
 size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision) {

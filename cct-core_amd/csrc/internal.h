@@ -177,6 +177,7 @@ struct IgemmCfg {
     int aStride;      // LDS floats per staged filter-slice plane (padded)
     int splits;       // split-K count (FilterAdjoint)
     int vec4;         // k_igemm_lp: stage 4-column groups with 16-byte loads (see choose_igemm_lp)
+    int trg;          // k_igemm_lp: filter tap rows staged in LDS at a time (F: the whole filter)
     size_t lds_bytes;
 };
 

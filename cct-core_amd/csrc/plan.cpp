@@ -475,12 +475,25 @@ bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ig
     // columns contiguous.  (The launch also checks the input's alignment.)
     c.vec4 = (g.border == COGCONV_BORDER_ZERO || g.border == COGCONV_BORDER_VALID ||
               g.border == COGCONV_BORDER_FULL) && g.dnR == 1 && g.dnC == 1 && g.W % 4 == 0 && g.FH <= 9;
+    // The largest output block (MB) whose filter slice fits in LDS in at most two
+    // groups of tap rows (beside the input tile); with mbOnly, that block with
+    // as many tap rows a group as fit.  A larger MB reads each staged pixel for
+    // more MFMAs and stages the input tile once for more outputs.
     for (int MB = c.Mout <= 16 ? 1 : (c.Mout <= 32 ? 2 : 4); MB >= 1; MB = MB == 4 ? 2 : MB - 1) {
         c.MB = MB;
         c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
-        c.lds_bytes = igemm_lp_lds_bytes(c, g.FH, d->precision);
-        if (c.lds_bytes <= 160 * 1024 && (!mbOnly || MB == mbOnly) &&
-            igemm_lp_workspace_bytes(c, g.FH, d->precision) < (1UL << 31)) {
+        int trg = 0;
+        for (int t = g.FH; t >= 1; t--) {
+            c.trg = t;
+            if (igemm_lp_lds_bytes(c, g.FH, d->precision) <= 160 * 1024) {
+                trg = t;
+                break;
+            }
+        }
+        const bool fits = trg > 0 && (mbOnly || MB == 1 || 2 * trg >= g.FH);
+        if (fits && (!mbOnly || MB == mbOnly) && igemm_lp_workspace_bytes(c, g.FH, d->precision) < (1UL << 31)) {
+            c.trg = trg;
+            c.lds_bytes = igemm_lp_lds_bytes(c, g.FH, d->precision);
             *out = c;
             return true;
         }
@@ -944,9 +957,9 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
         break;
     case KK_IGEMM_LP: {
         static const char* pn[] = {"fp32", "bf16", "fp16", "split_bf16"};
-        snprintf(buf, len, "igemm_%s %s out=%dx%dx%d MB=%d tile=8x32 threads=256%s lds=%zu", pn[plan->desc.precision],
-                 mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->icfg.MB, plan->icfg.vec4 ? " ld16" : "",
-                 plan->icfg.lds_bytes);
+        snprintf(buf, len, "igemm_%s %s out=%dx%dx%d MB=%d tile=8x32 threads=256%s taprows=%d lds=%zu",
+                 pn[plan->desc.precision], mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->icfg.MB,
+                 plan->icfg.vec4 ? " ld16" : "", plan->icfg.trg, plan->icfg.lds_bytes);
         break;
     }
     case KK_IGEMM_UP: {

@@ -185,6 +185,10 @@ struct cogconv_server_s {
     // after the previous launch on that stream has read its own), and pinned
     // host slots, one per launch in flight.
     std::vector<cogconv_image_io*> dev_tables;
+    // Graphs are captured on a stream of their own per serving stream (no event
+    // is ever recorded there): events recorded on a serving stream stay
+    // queryable while a launch sequence is being captured.
+    std::vector<hipStream_t> capture_streams;
     cogconv_image_io* host_tables = nullptr;
     std::vector<int> free_slots;
     std::map<std::tuple<cogconv_plan, int, int>, GraphEntry> graphs;
@@ -207,6 +211,32 @@ struct cogconv_server_s {
     bool stop = false;
     uint64_t next_ticket = 1;
     int rr = 0;
+    // Per stream: a worker is enqueueing on it with mu dropped.  A stream takes
+    // one worker's enqueues at a time: its request table is overwritten by the
+    // next launch's copy only after the previous launch read it (stream order),
+    // which holds only if launches reach the stream one whole launch at a time,
+    // and a stream being captured into a graph must receive no other work.
+    std::vector<char> stream_issuing;
+    // Round robin over the streams no other worker is enqueueing on; -1 if none.
+    int take_stream() {
+        const int n = (int)streams.size();
+        for (int k = 0; k < n; k++) {
+            const int si = (rr + k) % n;
+            if (!stream_issuing[si]) {
+                rr = si + 1;
+                stream_issuing[si] = 1;
+                return si;
+            }
+        }
+        return -1;
+    }
+    struct StreamHold {  // gives the stream back when issue() returns (mu held)
+        cogconv_server_s* s;
+        int si;
+        ~StreamHold() {
+            if (si >= 0) s->stream_issuing[si] = 0;
+        }
+    };
     // statistics
     std::vector<double> lat_ms;
     uint64_t submitted = 0, completed = 0, rejected = 0, launches = 0, stacked = 0;
@@ -277,7 +307,9 @@ struct cogconv_server_s {
         const bool resident = (h->r.flags & COGCONV_REQUEST_RESIDENT) != 0;
         if (resident) {
             // State is already in HBM: only the workspace comes from the budget.
-            const int si = rr++ % (int)streams.size();
+            const int si = take_stream();
+            if (si < 0) return false;  // every stream is taken by another worker: retry
+            StreamHold hold{this, si};
             hipStream_t s = streams[si];
             const float* in1 = h->r.bank < 0 ? h->r.host_in1 : banks[h->r.bank].dptr;
             // A resident bank a plan packs is packed once (synchronously, the
@@ -357,10 +389,11 @@ struct cogconv_server_s {
                     if (r == COGCONV_OK && capture) {
                         hipGraph_t graph = nullptr;
                         hipGraphExec_t ex = nullptr;
-                        bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+                        hipStream_t cs = capture_streams[si];
+                        bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
                         if (ok) {
-                            const int lr = launch_table(plan, dev_tables[si], x16, in1, wsp, pk != SIZE_MAX, s);
-                            ok = hipStreamEndCapture(s, &graph) == hipSuccess && lr == COGCONV_OK && graph;
+                            const int lr = launch_table(plan, dev_tables[si], x16, in1, wsp, pk != SIZE_MAX, cs);
+                            ok = hipStreamEndCapture(cs, &graph) == hipSuccess && lr == COGCONV_OK && graph;
                         }
                         ok = ok && hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0) == hipSuccess;
                         if (graph) (void)hipGraphDestroy(graph);
@@ -420,7 +453,12 @@ struct cogconv_server_s {
             for (auto& b : blocks) arena.release(b.first, b.second);
             return false;
         }
-        const int si = rr++ % (int)streams.size();
+        const int si = take_stream();
+        if (si < 0) {
+            for (auto& b : blocks) arena.release(b.first, b.second);
+            return false;
+        }
+        StreamHold hold{this, si};
         hipStream_t s = streams[si];
         char* base = arena.base;
         hipEvent_t ev = nullptr;
@@ -599,7 +637,14 @@ extern "C" int cogconv_server_create(const cogconv_server_config* config, cogcon
             return e;
         }
         s->streams.push_back(st);
+        e = check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream create");
+        if (e) {
+            (void)cogconv_server_destroy(s);
+            return e;
+        }
+        s->capture_streams.push_back(st);
     }
+    s->stream_issuing.assign(s->streams.size(), 0);
     for (int i = 0; i < config->streams; i++) {
         void* t = nullptr;
         e = check_hip(hipMalloc(&t, sizeof(cogconv_image_io) * config->max_stack), "request table");
@@ -642,6 +687,7 @@ extern "C" int cogconv_server_destroy(cogconv_server s) {
     for (auto t : s->dev_tables) (void)hipFree(t);
     if (s->host_tables) (void)hipHostFree(s->host_tables);
     for (auto st : s->streams) (void)hipStreamDestroy(st);
+    for (auto st : s->capture_streams) (void)hipStreamDestroy(st);
     for (auto ev : s->event_pool) (void)hipEventDestroy(ev);
     for (auto& b : s->banks) (void)hipFree(b.dptr);
     for (auto& kv : s->plans) cogconv_plan_destroy(kv.second);
