@@ -168,6 +168,7 @@ SIGNATURES = {
     "cogconv_launch": [_P, _P, _P, _P, _P, _P],
     "cogconv_plan_pack_operand": [_P, _P, _P, _P, _P],
     "cogconv_launch_packed": [_P, _P, _P, _P, _P],
+    "cogconv_noalloc_scratch_bytes": [ctypes.POINTER(_SZ)],
     "cogconv_plan_table_supported": [_P, ctypes.POINTER(ctypes.c_int32)],
     "cogconv_launch_table": [_P, _P, _P, _P, _P, ctypes.c_int32, _P],
     "cogconv_plan_set_epilogue": [_P, ctypes.POINTER(EpilogueC)],
@@ -185,6 +186,7 @@ SIGNATURES = {
     "cogconv_function_evaluate": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)],
     "cogconv_separable": [_P, ctypes.POINTER(FieldTypeC), _P, ctypes.c_int32, _P, ctypes.c_int32, ctypes.c_int32,
                           ctypes.c_int32, _P, ctypes.POINTER(EpilogueC), ctypes.POINTER(_P), _P],
+    "cogconv_launch_noalloc": [_P, _P, _P, _P, _P],
     "cogconv_block_reduce_sum": [_P, _P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _P],
     "cogconv_plan_variant_count": [ctypes.POINTER(DescC), ctypes.POINTER(ctypes.c_int32)],
     "cogconv_plan_create_variant": [ctypes.POINTER(DescC), ctypes.c_int32, ctypes.POINTER(_P)],
@@ -465,6 +467,13 @@ class Plan:
                                          1 if packed else 0, s))
         if stream is None:
             synchronize()  # the pageable host table must outlive the copy
+
+    def launch_noalloc(self, in0, in1, out, stream=None):
+        """cogconv_launch_noalloc: the workspace comes from the library's per-device scratch array."""
+        def ptr(x):
+            return x.ptr if isinstance(x, DeviceBuffer) else x
+        s = stream.handle if isinstance(stream, Stream) else stream
+        check(lib().cogconv_launch_noalloc(self.handle, ptr(in0), ptr(in1), ptr(out), s))
 
     def close(self):
         if getattr(self, "handle", None):

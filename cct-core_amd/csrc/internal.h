@@ -289,6 +289,10 @@ bool plan_table_capable(const cogconv_plan_s* plan);
 int launch_table(cogconv_plan plan, const cogconv_image_io* device_table, bool x16, const float* in1, void* workspace,
                  bool packed, hipStream_t s);
 
+// Device address of the no-allocation entry point's scratch array on `device`
+// (current), taken once per device (noalloc.hip).
+int noalloc_setup(int device);
+
 // Candidate configurations of a plan (plan.cpp) and the autotuning cache
 // (tune.cpp): the tuned variant index for `desc` on the current device and the
 // description it had when it was timed.

@@ -55,7 +55,9 @@ extern "C" int cogconv_init(int32_t device) {
     if (device < 0 || device >= n) return fail(COGCONV_ERR_INVALID_ARG, "device %d out of range (%d devices)", device, n);
     e = check_hip(hipSetDevice(device), "hipSetDevice");
     if (e) return e;
-    return check_hip(hipFree(nullptr), "context init");
+    e = check_hip(hipFree(nullptr), "context init");
+    if (e) return e;
+    return noalloc_setup(device);
 }
 
 extern "C" int cogconv_shutdown(void) {

@@ -315,6 +315,18 @@ int cogconv_plan_pack_operand(cogconv_plan plan, const float* in1, void* workspa
                               int32_t* packed);
 int cogconv_launch_packed(cogconv_plan plan, const float* in0, float* out, void* workspace, cogconv_stream stream);
 
+/* No-allocation launch.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Runs cogconv_launch with the plan's workspace
+ * taken from a device array the library holds on each device until the process
+ * ends (cogconv_noalloc_scratch_bytes gives its size, 64 MiB), so no call
+ * allocates or frees memory.  A plan needing more workspace than that fails with
+ * COGCONV_ERR_UNSUPPORTED.  All calls on one device share the array: calls that
+ * overlap on a device (several threads, or several streams with work in flight)
+ * race, so the caller serialises them. */
+int cogconv_noalloc_scratch_bytes(size_t* bytes);
+int cogconv_launch_noalloc(cogconv_plan plan, const float* in0, const float* in1, float* out, cogconv_stream stream);
+
 /* Request-table launch (no counterpart in the reference: for the serving loop,
  * which stacks requests of one shape whose fields live at unrelated device
  * addresses).  The plan's batch B images read their operand-1 planes from
