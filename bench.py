@@ -332,6 +332,30 @@ def pmc_traffic(fetch_csv, write_csv, calib_kernel="__amd_rocclr_copyBuffer", ca
         out[k] = {"fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
                   "hbm_bytes_per_dispatch": int(round(1024.0 * (fk * ff + wk * wf)))}
     return out
+def trace_summary(trace_csv):
+    """Per-kernel dispatch durations from a rocprofv3 --kernel-trace CSV, split
+    by grid size (a full-batch launch and a one-image check launch of one
+    kernel are different work): calls, mean, median, min and max in us."""
+    import csv
+    import re
+    groups = {}
+    with open(trace_csv) as fh:
+        for row in csv.DictReader(fh):
+            name = row["Kernel_Name"]
+            m = re.search(r"(\w+(?:<[^()]*>)?)\(", name)
+            short = m.group(1) if m else name.split("(")[0]
+            short = short.replace("cogconv::", "")
+            grid = "x".join(row.get("Grid_Size_%s" % a, "?") for a in "XYZ")
+            us = (int(row["End_Timestamp"]) - int(row["Start_Timestamp"])) / 1e3
+            groups.setdefault((short, grid), []).append(us)
+    out = []
+    for (short, grid), v in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+        out.append({"kernel": short, "grid": grid, "calls": len(v), "mean_us": round(statistics.mean(v), 2),
+                    "median_us": round(statistics.median(v), 2), "min_us": round(min(v), 2),
+                    "max_us": round(max(v), 2), "total_us": round(sum(v), 1)})
+    return out
+
+
 def precision_workload(args):
     """B32 128x128x64 planes, 64 filters 3x3x64: ProjectFrame+BRS and
     BackProjectFrame+BRS (BorderZero) and FilterAdjoint+BRS (BorderValid, dy
@@ -843,7 +867,13 @@ def main():
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="no GPU work: print HBM bytes per dispatch per kernel from two rocprofv3 counter CSVs "
                          "(FETCH_SIZE, WRITE_SIZE) of a run made with --copy-calibration")
+    ap.add_argument("--trace-summary", metavar="KERNEL_TRACE_CSV",
+                    help="no GPU work: print per-kernel, per-grid dispatch durations of a rocprofv3 --kernel-trace CSV")
     args = ap.parse_args()
+    if args.trace_summary:
+        for r in trace_summary(args.trace_summary):
+            print(json.dumps(r))
+        return
     if args.pmc_traffic:
         print(json.dumps(pmc_traffic(*args.pmc_traffic), indent=1))
         return

@@ -69,11 +69,12 @@ __global__ void k_pack_filters_lp(Geom g, PlaneMap m, int Mout, int Mpad, int Rp
     }
 }
 
-template <int F, int MB, int PREC>
-__global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c, const float* __restrict__ X,
-                                                  const typename LpType<PREC>::T* __restrict__ Wp,
-                                                  float* __restrict__ O, const cogconv_image_io* __restrict__ tab,
-                                                  Epi e) {
+template <int F, int MB, int PREC, bool RING>
+__device__ __forceinline__ void igemm_lp_body(const Geom& g, const PlaneMap& m, const IgemmCfg& c,
+                                              const float* __restrict__ X,
+                                              const typename LpType<PREC>::T* __restrict__ Wp,
+                                              float* __restrict__ O, const cogconv_image_io* __restrict__ tab,
+                                              const Epi& e) {
     typedef typename LpType<PREC>::T T;
     typedef typename LpType<PREC>::V V;
     constexpr int TAPS = F * F;
@@ -87,14 +88,17 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     // or output plane per slot, the four 8-plane quarters of a 32-plane chunk
     // in separate planes of slots, so that the 16 lanes of an MFMA row group
     // read 16 consecutive slots (no padding, no bank conflicts).  The filter
-    // slice holds TRG tap rows at a time (all F when they fit).
-    const int TRG = c.trg;
+    // slice holds TRG tap rows at a time (all F when they fit); RING: one tap
+    // row at a time in two buffers, the next row's loads in flight while the
+    // current row's MFMAs run.
+    const int TRG = RING ? 1 : c.trg;
     const int TAPG = TRG * F;
+    const int gElems = TERMS * TAPG * 4 * MT * 8;             // one filter group (buffer)
     T* sB = reinterpret_cast<T*>(lp_smem);                    // [TERMS][4][npix][8]
-    T* sA = sB + (size_t)TERMS * 4 * npix * 8;               // [TERMS][TAPG][4][MT][8]
+    T* sA = sB + (size_t)TERMS * 4 * npix * 8;               // [RING ? 2 : 1][TERMS][TAPG][4][MT][8]
     // Scratch slots (offset from sB) for the stores of lanes beyond the tile
-    // width, after the filter slice: TERMS x 64 lanes x 8 elements.
-    const size_t sJunk = (size_t)TERMS * 4 * npix * 8 + (size_t)TERMS * TAPG * 4 * MT * 8;
+    // width, after the filter slice(s): TERMS x 64 lanes x 8 elements.
+    const size_t sJunk = (size_t)TERMS * 4 * npix * 8 + (size_t)(RING ? 2 : 1) * gElems;
 
     int bid = xcd_swizzle(blockIdx.x, gridDim.x);
     const int tc = bid % c.tilesC;
@@ -125,6 +129,7 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     const long wTerm = (long)TAPS * Mpad * Rpad;
     const int wChunkBytes = 2 * TAPS * 4 * Mpad * 8;  // one 32-plane chunk of one term
     const __amdgpu_buffer_rsrc_t wr = buffer_rsrc(Wp, (uint32_t)(TERMS * wTerm * 2));  // the plan bounds this
+    const bool bordered = g.border == COGCONV_BORDER_CLAMP || g.border == COGCONV_BORDER_CYCLIC;
 
     // Per-lane column offset of the staged tile (TCp <= 64) with the border applied.
     int colOff;
@@ -148,7 +153,10 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     // 16-byte staging (c.vec4): the staged columns re-based to xa = x0 rounded
     // down to a multiple of 4; lane (si, gc) loads the 4-column group gc of
     // item slot si (IP items of NG groups per wave pass), eight planes as
-    // eight 16-byte loads, and writes the group's (up to) four pixels.
+    // eight 16-byte loads, and writes the group's (up to) four pixels.  With
+    // W % 4 == 0 a group lies wholly inside or wholly outside the field; for
+    // Clamp and Cyclic the groups outside are left to a second pass that
+    // loads each of their columns from its bordered position.
     const int s4 = ((x0 % 4) + 4) % 4;
     const int NG = (s4 + TCp + 3) >> 2;
     const int IP = 64 / NG;
@@ -157,36 +165,50 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     const bool gIn = si < IP && (unsigned)colA < (unsigned)g.W;
     const float invTRp = 1.0f / TRp;
 
-    // Filter-slice units (term, tap, 8-plane quarter, output plane) of this
-    // thread, output plane fastest (consecutive threads copy consecutive 16
-    // bytes of the packed bank into consecutive LDS slots): byte offsets in
-    // the packed bank (the chunk adds its offset as the uniform offset) and
-    // LDS destinations (elements from sB; units past the end go to the scratch
-    // slot).  With the whole filter in one group they are computed once and
-    // their loads issued with the input tile's.
+    // Filter-slice units (term, tap, 8-plane quarter, output plane), output
+    // plane fastest: consecutive threads copy consecutive 16 bytes of the
+    // packed bank into consecutive LDS slots.  256 is a multiple of the units
+    // of one tap (4 MT), so a thread's quarter and output plane are fixed and
+    // its i-th unit lies TPI taps after its first: offsets are affine in i.
+    constexpr int UPT = 4 * MT;
+    constexpr int TPI = 256 / UPT;
+    const int uLow = tid % UPT;                     // quarter * MT + output plane
+    const int tg0 = tid / UPT;
+    const int unitCol = 2 * (((uLow / MT) * Mpad + m0 + uLow % MT) * 8);  // bytes within a tap
+    // Byte offset in the packed bank (the chunk adds its offset as the uniform
+    // offset; taps past the filter read 0 through an out-of-range offset) and
+    // LDS destination (elements from sB; units past the group go to the
+    // scratch slot) of the thread's i-th unit of a group of `tapg` taps
+    // starting at tap `tap0`, written into the buffer at `buf` (elements from sB).
+    auto group_unit = [&](int i, int tap0, int tapg, int buf, int& off, int& dst) {
+        const int L = tg0 + TPI * i;                 // linear tap over [TERMS][tapg]
+        const int term = SPLIT && L >= tapg ? 1 : 0;
+        const int tg = L - term * tapg;
+        const bool live = L < TERMS * tapg;
+        off = live && tap0 + tg < TAPS ? (int)(2 * term * wTerm) + (tap0 + tg) * (2 * 4 * Mpad * 8) + unitCol
+                                       : LP_OOB;
+        dst = live ? buf + (L * UPT + uLow) * 8 : (int)(sJunk + 8 * lane);
+    };
     constexpr int UNITS = TERMS * TAPS * MT * 4;
     constexpr int WPT = (UNITS + 255) / 256;
-    constexpr bool WPRE = WPT <= 12;
+    constexpr bool WPRE = !RING && WPT <= 12;
     const bool onePass = TRG == F;  // block-uniform
+    const int sAoff = (int)((size_t)TERMS * 4 * npix * 8);  // elements from sB to the filter buffer(s)
     int wOff[WPRE ? WPT : 1], wDst[WPRE ? WPT : 1];
-    auto unit_of = [&](int u, int tap0, int& off, int& dst) {
-        const int uc = min(u, TERMS * TAPG * MT * 4 - 1);
-        const int mm = uc % MT;
-        const int rest = uc / MT;
-        const int qq = rest & 3;
-        const int rest2 = rest >> 2;
-        const int tg = rest2 % TAPG;
-        const int term = rest2 / TAPG;
-        // Taps past the filter (the last group of a multi-group slice) read 0 through
-        // an out-of-range offset; no MFMA uses them.
-        off = tap0 + tg < TAPS ? 2 * (int)(term * wTerm + ((long)((tap0 + tg) * 4 + qq) * Mpad + m0 + mm) * 8)
-                               : LP_OOB;
-        dst = u < TERMS * TAPG * MT * 4 ? (int)((size_t)TERMS * 4 * npix * 8 + (size_t)uc * 8)
-                                        : (int)(sJunk + 8 * lane);
-    };
     if (WPRE && onePass) {
 #pragma unroll
-        for (int i = 0; i < (WPRE ? WPT : 1); i++) unit_of(tid + 256 * i, 0, wOff[i], wDst[i]);
+        for (int i = 0; i < (WPRE ? WPT : 1); i++) group_unit(i, 0, TAPS, sAoff, wOff[i], wDst[i]);
+    }
+    // RING: the units of one tap row (per thread), prefetched into registers.
+    constexpr int WPG = (TERMS * F + TPI - 1) / TPI;
+    int rOff[RING ? WPG : 1], rDst[2][RING ? WPG : 1];
+    if (RING) {
+#pragma unroll
+        for (int i = 0; i < (RING ? WPG : 1); i++) {
+            group_unit(i, 0, F, sAoff, rOff[i], rDst[0][i]);
+            int o2;
+            group_unit(i, 0, F, sAoff + gElems, o2, rDst[1][i]);
+        }
     }
 
     // Per-lane table of the staged tile's row offsets (lane = tile row, at
@@ -236,7 +258,8 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     // chunk (7 dn + F); IU covers all of them when dn = 1 and F <= 3, so the
     // chunk's staging costs one memory round trip.
     const int nitems = 4 * TRp;
-    constexpr int IU = F + 7 < 10 ? F + 7 : 10;
+    // Ring kernels keep fewer items in flight (registers for two blocks a CU).
+    constexpr int IU = RING ? (F + 7 < 4 ? F + 7 : 4) : (F + 7 < 10 ? F + 7 : 10);
     const int planeBytes = 4 * HW;
     const bool inTile = lane < TCp;
     auto issue = [&](int r0, int base, float (&v)[IU][8]) {
@@ -278,6 +301,12 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
             if (ty >= TRp) { ty -= TRp; grp++; }
         }
     };
+    // Halo columns of a Clamp / Cyclic tile staged with 16-byte loads: the
+    // staged columns left of 0 and right of W - 1 (block-uniform counts).
+    const int nhl = c.vec4 && bordered ? min(max(-x0, 0), TCp) : 0;
+    const int nhr = c.vec4 && bordered ? min(max(x0 + TCp - g.W, 0), TCp - nhl) : 0;
+    const int nh = nhl + nhr;
+    const int rowBytesW = 2 * F * 4 * Mpad * 8;  // one tap row of one term in the packed bank
     for (int r0 = 0; r0 < m.R; r0 += LP_RC) {
         const int wChunk = (r0 / LP_RC) * wChunkBytes;  // uniform byte offset of the chunk's packed filters
         __syncthreads();
@@ -287,29 +316,48 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
             for (int i = 0; i < (WPRE ? WPT : 1); i++)
                 wv[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, wOff[i], wChunk, 0));
         }
+        V rv[RING ? WPG : 1];
+        if (RING) {
+#pragma unroll
+            for (int i = 0; i < (RING ? WPG : 1); i++)
+                rv[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, rOff[i], wChunk, 0));
+        }
         if (c.vec4) {  // block-uniform
             constexpr int VP = (F + 12) / 6;  // passes: a wave owns TRp = F + 7 items, IP >= 6 a pass
-            float4 q[VP][8];
-            int dBase[VP];
+            constexpr int VG = RING ? 1 : VP;   // passes whose loads are in flight at once
 #pragma unroll
-            for (int pp = 0; pp < VP; pp++) {
-                const int j = pp * IP + si;  // the wave's item slot
+            for (int p0 = 0; p0 < VP; p0 += VG) {
+            float4 q[VG][8];
+            int dBase[VG];
+#pragma unroll
+            for (int pp = 0; pp < VG; pp++) {
+                const int j = (p0 + pp) * IP + si;  // the wave's item slot
                 const int it = wave + 4 * j;
                 const int grp = fast_div(it, TRp, invTRp), ty = it - grp * TRp;
                 const bool itemOk = si < IP && j < TRp;
-                const int y = y0 + ty;
-                const bool on = itemOk && gIn && (unsigned)y < (unsigned)g.H;
+                int y = y0 + ty;
+                bool yIn = (unsigned)y < (unsigned)g.H;
+                if (g.border == COGCONV_BORDER_CLAMP) {
+                    y = min(max(y, 0), g.H - 1);
+                    yIn = true;
+                } else if (g.border == COGCONV_BORDER_CYCLIC) {
+                    y %= g.H;
+                    y += y < 0 ? g.H : 0;
+                    yIn = true;
+                }
+                const bool on = itemOk && gIn && yIn;
                 int vo = on ? 4 * ((r0 + 8 * grp) * HW + y * g.W + colA) : LP_OOB;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
                     q[pp][e] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, vo, 0, 0));
                     vo += planeBytes;
                 }
-                dBase[pp] = itemOk ? (grp * npix + ty * TCp) * 8 : -1;
+                // Groups outside a Clamp / Cyclic field are written by the halo pass.
+                dBase[pp] = itemOk && (gIn || !bordered) ? (grp * npix + ty * TCp) * 8 : -1;
             }
             __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll
-            for (int pp = 0; pp < VP; pp++) {
+            for (int pp = 0; pp < VG; pp++) {
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const int col = 4 * gc + k - s4;
@@ -327,6 +375,41 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
                     if (SPLIT) *reinterpret_cast<V*>(sB + (ok ? (size_t)4 * npix * 8 + o : sJunk + 512 + 8 * lane)) = lv;
                 }
             }
+            }
+            // Halo pass: each slot (quarter, row, halo column) loads its eight
+            // planes from the bordered position.
+            for (int hq = tid; hq < 4 * TRp * nh; hq += 256) {
+                const int k = hq % nh, rest = hq / nh;
+                const int ty = rest % TRp, grp = rest / TRp;
+                const int cx = k < nhl ? k : TCp - nhr + (k - nhl);
+                int x = x0 + cx, y = y0 + ty;
+                if (g.border == COGCONV_BORDER_CYCLIC) {
+                    x %= g.W;
+                    x += x < 0 ? g.W : 0;
+                    y %= g.H;
+                    y += y < 0 ? g.H : 0;
+                } else {
+                    x = min(max(x, 0), g.W - 1);
+                    y = min(max(y, 0), g.H - 1);
+                }
+                int vo = 4 * ((r0 + 8 * grp) * HW + y * g.W + x);
+                float hvals[8];
+#pragma unroll
+                for (int e8 = 0; e8 < 8; e8++) {
+                    hvals[e8] = buffer_load_f32(xr, vo, 0);
+                    vo += planeBytes;
+                }
+                V hv, lv;
+#pragma unroll
+                for (int e8 = 0; e8 < 8; e8++) {
+                    const T h = (T)hvals[e8];
+                    hv[e8] = h;
+                    if (SPLIT) lv[e8] = (T)(hvals[e8] - (float)h);
+                }
+                const size_t o = ((size_t)grp * npix + ty * TCp + cx) * 8;
+                *reinterpret_cast<V*>(sB + o) = hv;
+                if (SPLIT) *reinterpret_cast<V*>(sB + (size_t)4 * npix * 8 + o) = lv;
+            }
         } else {
             for (int base = wave; base < nitems; base += 4 * IU) {
                 float v[IU][8];
@@ -336,32 +419,44 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
             }
         }
         for (int tr0 = 0; tr0 < F; tr0 += TRG) {
-            // Filter slice of tap rows tr0 .. tr0 + TRG - 1 from the pre-packed bank.
-            if (WPRE && onePass) {
+            if (RING) {
+                // Tap row tr0 (prefetched) into buffer tr0 & 1; after the barrier,
+                // every wave is past the MFMAs of row tr0 - 2, the buffer's last user.
+#pragma unroll
+                for (int i = 0; i < (RING ? WPG : 1); i++) *reinterpret_cast<V*>(sB + rDst[tr0 & 1][i]) = rv[i];
+                __syncthreads();
+                if (tr0 + 1 < F) {
+#pragma unroll
+                    for (int i = 0; i < (RING ? WPG : 1); i++)
+                        rv[i] = __builtin_bit_cast(
+                            V, __builtin_amdgcn_raw_buffer_load_b128(wr, rOff[i] + (tr0 + 1) * rowBytesW, wChunk, 0));
+                }
+            } else if (WPRE && onePass) {
 #pragma unroll
                 for (int i = 0; i < (WPRE ? WPT : 1); i++) *reinterpret_cast<V*>(sB + wDst[i]) = wv[i];
+                __syncthreads();
             } else {
+                // Filter slice of tap rows tr0 .. tr0 + TRG - 1 from the pre-packed bank.
                 if (tr0 > 0) __syncthreads();  // the previous group's MFMAs are done with the slice
-                const int units = TERMS * TAPG * MT * 4;
+                const int nUnits = TERMS * TAPG;  // taps of the group over both terms; TPI a step
                 constexpr int WU = 8;
-                for (int u0 = tid; u0 < units; u0 += 256 * WU) {
+                for (int i0 = 0; i0 * TPI < nUnits; i0 += WU) {
                     V wb[WU];
                     int dst[WU];
 #pragma unroll
                     for (int i = 0; i < WU; i++) {
                         int off;
-                        unit_of(u0 + 256 * i, tr0 * F, off, dst[i]);
-                        if (u0 + 256 * i >= units) dst[i] = (int)(sJunk + 8 * lane);
+                        group_unit(i0 + i, tr0 * F, TAPG, sAoff, off, dst[i]);
                         wb[i] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b128(wr, off, wChunk, 0));
                     }
 #pragma unroll
                     for (int i = 0; i < WU; i++) *reinterpret_cast<V*>(sB + dst[i]) = wb[i];
                 }
+                __syncthreads();
             }
-            __syncthreads();
             // Taps of this group: sAg addresses tap t = ti * F + tj of the whole
             // filter, so the per-tap offsets stay compile-time constants.
-            const T* sAg = sA - (size_t)tr0 * F * 4 * MT * 8;
+            const T* sAg = sA + (RING ? (size_t)(tr0 & 1) * gElems : 0) - (size_t)tr0 * F * 4 * MT * 8;
             const T* sAg2 = sAg + (size_t)TAPG * 4 * MT * 8;  // split: the residual term
             const int tEnd = min(F, tr0 + TRG);
 #pragma unroll
@@ -446,11 +541,33 @@ __global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c
     }
 }
 
+template <int F, int MB, int PREC, bool RING>
+__global__ __launch_bounds__(256) void k_igemm_lp(Geom g, PlaneMap m, IgemmCfg c, const float* __restrict__ X,
+                                                  const typename LpType<PREC>::T* __restrict__ Wp,
+                                                  float* __restrict__ O, const cogconv_image_io* __restrict__ tab,
+                                                  Epi e) {
+    igemm_lp_body<F, MB, PREC, RING>(g, m, c, X, Wp, O, tab, e);
+}
+
+// Ring mode with two blocks a CU (LDS within half the CU's): registers held to
+// half the SIMD's file so that both blocks' waves are resident.
+template <int F, int MB, int PREC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_igemm_lp_ring2(
+    Geom g, PlaneMap m, IgemmCfg c, const float* __restrict__ X, const typename LpType<PREC>::T* __restrict__ Wp,
+    float* __restrict__ O, const cogconv_image_io* __restrict__ tab, Epi e) {
+    igemm_lp_body<F, MB, PREC, true>(g, m, c, X, Wp, O, tab, e);
+}
+
+// A ring configuration whose LDS leaves room for a second block on the CU.
+bool igemm_lp_two_blocks(const IgemmCfg& c) { return c.ring && c.lds_bytes <= 80 * 1024; }
+
 size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision) {
     const int terms = precision == COGCONV_SPLIT_BF16 ? 2 : 1;
-    // tile [4][npix][8] + filter group [trg * F][4][16 MB][8], 16-bit elements,
-    // + scratch slots of the lanes beyond the tile width
-    return (size_t)terms * ((size_t)c.tileRows * c.tileCols + (size_t)c.trg * F * 16 * c.MB) * 64 +
+    // tile [4][npix][8] + filter group [trg * F][4][16 MB][8] (two one-row
+    // groups in ring mode), 16-bit elements, + scratch slots of the lanes
+    // beyond the tile width
+    const size_t group = (size_t)(c.ring ? 1 : c.trg) * F * 16 * c.MB;
+    return (size_t)terms * ((size_t)c.tileRows * c.tileCols + (c.ring ? 2 : 1) * group) * 64 +
            (size_t)terms * 64 * 8 * 2;
 }
 
@@ -476,8 +593,27 @@ static hipError_t launch_lp3(const Geom& g, const PlaneMap& m, const IgemmCfg& c
         if (e != hipSuccess || stage == LP_PACK_ONLY) return e;
     }
     const long blocks = (long)c.tilesC * c.tilesR * c.mTiles * m.B;
-    hipLaunchKernelGGL((k_igemm_lp<F, MB, PREC>), dim3((unsigned)blocks), dim3(256), c.lds_bytes, s, g, m, c, x, wp,
-                       out, tab, ep);
+    // Two blocks a CU fit the LDS only for MB <= 2, or MB = 4 with F <= 5 (the
+    // planner's sizes); split-BF16 tiles never do.
+    constexpr bool TWO = PREC != COGCONV_SPLIT_BF16 && (MB <= 2 || F <= 5);
+    if constexpr (TWO) {
+        if (igemm_lp_two_blocks(c)) {
+            hipLaunchKernelGGL((k_igemm_lp_ring2<F, MB, PREC>), dim3((unsigned)blocks), dim3(256), c.lds_bytes, s, g,
+                               m, c, x, wp, out, tab, ep);
+            return hipGetLastError();
+        }
+    }
+    if constexpr (PREC != COGCONV_SPLIT_BF16) {  // the planner offers ring mode for BF16 / FP16 only
+        if (c.ring) {
+            hipLaunchKernelGGL((k_igemm_lp<F, MB, PREC, true>), dim3((unsigned)blocks), dim3(256), c.lds_bytes, s, g,
+                               m, c, x, wp, out, tab, ep);
+            return hipGetLastError();
+        }
+    } else if (c.ring) {
+        return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL((k_igemm_lp<F, MB, PREC, false>), dim3((unsigned)blocks), dim3(256), c.lds_bytes, s, g, m, c,
+                       x, wp, out, tab, ep);
     return hipGetLastError();
 }
 

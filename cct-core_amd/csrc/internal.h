@@ -178,6 +178,7 @@ struct IgemmCfg {
     int splits;       // split-K count (FilterAdjoint)
     int vec4;         // k_igemm_lp: stage 4-column groups with 16-byte loads (see choose_igemm_lp)
     int trg;          // k_igemm_lp: filter tap rows staged in LDS at a time (F: the whole filter)
+    int ring;         // k_igemm_lp: one tap row a group, two LDS buffers, the next row prefetched (trg = 1)
     size_t lds_bytes;
 };
 
@@ -264,6 +265,7 @@ hipError_t launch_fa_igemm_lp(int precision, const Geom& g, const PlaneMap& m, c
                               const float* dy, float* out, float* ws, hipStream_t s, const Epi& e = kNoEpi);
 bool igemm_lp_supported(int F);
 size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision);
+bool igemm_lp_two_blocks(const IgemmCfg& c);
 size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision);
 enum LpStage { LP_PACK_AND_RUN = 0, LP_PACK_ONLY = 1, LP_RUN_ONLY = 2 };
 hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x,

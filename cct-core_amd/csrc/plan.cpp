@@ -448,7 +448,8 @@ bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m,
 }
 
 // BF16 / FP16 / split-BF16 MFMA implicit GEMM for the frame block-reduce modes.
-bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, IgemmCfg* out, int mbOnly = 0) {
+bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, IgemmCfg* out, int mbOnly = 0,
+                     bool ring = false) {
     if (d->mode != COGCONV_PROJECT_FRAME_BLOCK_REDUCE_SUM && d->mode != COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM)
         return false;
     if (d->in0.dims != 2 || g.FH != g.FW || !igemm_lp_supported(g.FH) || g.upAr != 1 || g.upAc != 1 ||
@@ -473,8 +474,15 @@ bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ig
     // wholly inside or outside a row when W % 4 == 0, so borders that read 0
     // outside the field need no per-column remap; unit steps keep the staged
     // columns contiguous.  (The launch also checks the input's alignment.)
-    c.vec4 = (g.border == COGCONV_BORDER_ZERO || g.border == COGCONV_BORDER_VALID ||
-              g.border == COGCONV_BORDER_FULL) && g.dnR == 1 && g.dnC == 1 && g.W % 4 == 0 && g.FH <= 9;
+    // Clamp and Cyclic tiles stage their columns outside the field in a second
+    // pass.  A wave's passes (VP of IP item slots) must cover its TRp items.
+    c.vec4 = g.dnR == 1 && g.dnC == 1 && g.W % 4 == 0 && g.FH <= 9;
+    if (c.vec4) {
+        const int s4 = ((-g.hC) % 4 + 4) % 4;  // every tile starts at 32 tc - hC
+        const int ng = (s4 + c.tileCols + 3) / 4;
+        const int ip = 64 / ng, vp = (g.FH + 12) / 6;
+        if (vp * ip < c.tileRows) c.vec4 = 0;
+    }
     // The largest output block (MB) whose filter slice fits in LDS in at most two
     // groups of tap rows (beside the input tile); with mbOnly, that block with
     // as many tap rows a group as fit.  A larger MB reads each staged pixel for
@@ -483,14 +491,15 @@ bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ig
         c.MB = MB;
         c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
         int trg = 0;
-        for (int t = g.FH; t >= 1; t--) {
+        c.ring = ring ? 1 : 0;
+        for (int t = ring ? 1 : g.FH; t >= 1; t--) {
             c.trg = t;
             if (igemm_lp_lds_bytes(c, g.FH, d->precision) <= 160 * 1024) {
                 trg = t;
                 break;
             }
         }
-        const bool fits = trg > 0 && (mbOnly || MB == 1 || 2 * trg >= g.FH);
+        const bool fits = trg > 0 && (mbOnly || ring || MB == 1 || 2 * trg >= g.FH);
         if (fits && (!mbOnly || MB == mbOnly) && igemm_lp_workspace_bytes(c, g.FH, d->precision) < (1UL << 31)) {
             c.trg = trg;
             c.lds_bytes = igemm_lp_lds_bytes(c, g.FH, d->precision);
@@ -683,8 +692,17 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         IgemmCfg lp;
         if (!choose_igemm_lp(desc, g, m, &lp)) return;
         add(KK_IGEMM_LP, nullptr, nullptr, &lp, nullptr);
+        const int mbMax = lp.MB;
         for (int mb : {4, 2, 1}) {
-            if (mb < lp.MB && choose_igemm_lp(desc, g, m, &lp, mb)) add(KK_IGEMM_LP, nullptr, nullptr, &lp, nullptr);
+            if (mb < mbMax && choose_igemm_lp(desc, g, m, &lp, mb)) add(KK_IGEMM_LP, nullptr, nullptr, &lp, nullptr);
+        }
+        // One tap row of the filter at a time, double-buffered: a smaller LDS
+        // footprint (two blocks a CU for 7x7 / 9x9 at MB <= 2) and the next
+        // row's loads behind the current row's MFMAs.
+        for (int mb : {4, 2, 1}) {
+            if (mb <= mbMax && g.FH > 1 && desc->precision != COGCONV_SPLIT_BF16 &&
+                choose_igemm_lp(desc, g, m, &lp, mb, true))
+                add(KK_IGEMM_LP, nullptr, nullptr, &lp, nullptr);
         }
         return;
     }
@@ -957,9 +975,9 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
         break;
     case KK_IGEMM_LP: {
         static const char* pn[] = {"fp32", "bf16", "fp16", "split_bf16"};
-        snprintf(buf, len, "igemm_%s %s out=%dx%dx%d MB=%d tile=8x32 threads=256%s taprows=%d lds=%zu",
+        snprintf(buf, len, "igemm_%s %s out=%dx%dx%d MB=%d tile=8x32 threads=256%s taprows=%d%s lds=%zu",
                  pn[plan->desc.precision], mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->icfg.MB,
-                 plan->icfg.vec4 ? " ld16" : "", plan->icfg.trg, plan->icfg.lds_bytes);
+                 plan->icfg.vec4 ? " ld16" : "", plan->icfg.trg, plan->icfg.ring ? " ring" : "", plan->icfg.lds_bytes);
         break;
     }
     case KK_IGEMM_UP: {

@@ -227,3 +227,48 @@ def test_precision_large_filters(precision, mode, border, F):
     err = normwise(got, ref)
     print("  normwise error %.3g" % err)
     assert err <= TOL[precision]
+
+
+EVERY_CASES = []
+for _mode in (O.PF_BRS, O.BPF_BRS):
+    for _border in (O.ZERO, O.CLAMP, O.CYCLIC, O.VALID, O.FULL):
+        for _F in (3, 5, 7, 9):
+            EVERY_CASES.append((_mode, _border, _F))
+
+
+@pytest.mark.parametrize("precision", (L.BF16, L.FP16))
+@pytest.mark.parametrize("mode,border,F", EVERY_CASES)
+def test_precision_frame_every_candidate(precision, mode, border, F):
+    """Every candidate configuration of the BF16 / FP16 frame kernel (output
+    block sizes, whole-filter or tap-row groups, the double-buffered one-row
+    ring, 16-byte staging with the Clamp / Cyclic halo pass) adds the same
+    products in the same order, so all agree bit for bit; the first meets the
+    oracle bound.  Width 68 (a multiple of 4: 16-byte staging) and 37 (per-column
+    staging); 36 planes (a partial 8-plane group), ragged row and column tiles."""
+    rng = np.random.default_rng(900 + 100 * precision + 10 * border + F + 3 * mode)
+    B, N, K = 2, 36, 20
+    orient = O.CONV if mode == O.BPF_BRS else O.XCORR
+    for H, W in ((23, 68), (21, 37)):
+        a = inputs(rng, (B * N, H, W), False)
+        k = inputs(rng, (K * N, F, F), False)
+        od, cd = desc_pair(mode, border, orient, O.NONE, 1, B, a.shape, k.shape, precision=precision)
+        ref = O.conv(od, a, k)
+        first = None
+        names = []
+        for v in range(L.variant_count(cd)):
+            p = L.Plan(cd, v)
+            name = p.description
+            p.close()
+            names.append(name)
+            print("CASE every", NAMES[precision], mode, border, F, W, v, name, flush=True)
+            got = run_gpu(cd, a, k, variant=v)
+            if first is None:
+                first = got
+                err = normwise(got, ref)
+                assert err <= TOL[precision], (name, err)
+            else:
+                assert np.array_equal(got, first), "%s differs from %s" % (name, names[0])
+        if W % 4 == 0:
+            assert any("ld16" in n for n in names), names
+        if F > 1:
+            assert any("ring" in n for n in names), names
