@@ -359,7 +359,8 @@ def trace_summary(trace_csv):
 def precision_workload(args):
     """B32 128x128x64 planes, 64 filters 3x3x64: ProjectFrame+BRS and
     BackProjectFrame+BRS (BorderZero) and FilterAdjoint+BRS (BorderValid, dy
-    126x126x64) in FP32, BF16, FP16 and split-BF16; GFLOP/s from HIP events,
+    126x126x64) in FP32, BF16, FP16 and split-BF16; per-op medians of
+    --op-reps launches, each after a 512 MiB cache-flush write, HIP events;
     errors against the FP64 oracle on a 2-image slice (uniform[-1,1) and
     wide-range 10^U(-3,3) with random signs)."""
     import cogconv._lib as L
@@ -375,6 +376,7 @@ def precision_workload(args):
     bx, bw, bdy, bdyv = (L.DeviceBuffer.from_array(a) for a in (x, w, dy, dyv))
     f = lambda planes, side=H: (2, side, side, planes)  # noqa: E731
     stream = L.Stream()
+    flush = L.DeviceBuffer(FLUSH_BYTES)
     names = {L.FP32: "fp32", L.BF16: "bf16", L.FP16: "fp16", L.SPLIT_BF16: "split_bf16"}
     res = {}
     for prec in (L.FP32, L.BF16, L.FP16, L.SPLIT_BF16):
@@ -387,24 +389,17 @@ def precision_workload(args):
                  f(B * K, H - 2), bdyv)):
             plan = tuned_plan(L, L.make_desc(mode, border, orient, L.NO_SAMPLING, 1, B, f(a.shape[0]), t1, prec),
                               args)
-            out = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
-            ws = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
-            for _ in range(3):
-                plan.launch(ab, b1, out, ws, stream)
-            e0, e1 = L.Event(), L.Event()
-            reps = 20
-            e0.record(stream)
-            for _ in range(reps):
-                plan.launch(ab, b1, out, ws, stream)
-            e1.record(stream)
-            e1.synchronize()
-            ms = e0.elapsed_ms(e1) / reps
+            op = Op(L, opname, plan, ab, b1)
+            ms, lo, hi = flushed_median_ms(L, op, stream, flush, reps=args.op_reps)
+            warm = timed_steps(L, [op], stream, 20, 3)  # back to back, operands left in the caches
             res.setdefault(names[prec], {})[opname] = {
-                "ms": round(ms, 4), "gflops": round(plan.flops / (ms * 1e-3) / 1e9, 1), "kernel": plan.description}
-            log(names[prec], opname, plan.description, "%.4f ms" % ms)
-            out.close()
-            if ws:
-                ws.close()
+                "ms": round(ms, 4), "min_ms": round(lo, 4), "max_ms": round(hi, 4),
+                "gflops": round(plan.flops / (ms * 1e-3) / 1e9, 1),
+                "warm_ms": round(warm, 4), "warm_gflops": round(plan.flops / (warm * 1e-3) / 1e9, 1),
+                "kernel": plan.description}
+            log(names[prec], opname, plan.description, "%.4f ms flushed, %.4f ms warm" % (ms, warm))
+            op.close()
+            plan.close()
     # Errors on a 2-image slice, both input sets.
     errs = {}
     for label, gen in (("uniform", lambda sh: rng.uniform(-1, 1, sh)),
@@ -443,6 +438,12 @@ def precision_workload(args):
             "bf16_over_fp32_projectFrame": round(res["bf16"]["projectFrame+BRS"]["gflops"] / fp32, 2),
             "bf16_over_fp32_filterAdjoint": round(res["bf16"]["filterAdjoint+BRS"]["gflops"] /
                                                   res["fp32"]["filterAdjoint+BRS"]["gflops"], 2),
+            "warm_bf16_over_fp32_projectFrame": round(res["bf16"]["projectFrame+BRS"]["warm_gflops"] /
+                                                      res["fp32"]["projectFrame+BRS"]["warm_gflops"], 2),
+            "warm_bf16_over_fp32_filterAdjoint": round(res["bf16"]["filterAdjoint+BRS"]["warm_gflops"] /
+                                                       res["fp32"]["filterAdjoint+BRS"]["warm_gflops"], 2),
+            "timing": "ms: median of --op-reps launches, each after a 512 MiB cache-flush write; warm_ms: mean "
+                      "of 20 back-to-back launches after 3 (operands resident in the last-level cache)",
             "peaks": {"fp32_tflops": PEAK_FP32_TFLOPS, "bf16_tflops": PEAK_BF16_TFLOPS, "hbm_gbs": PEAK_HBM_GBS}}
     print(json.dumps(line), flush=True)
 

@@ -1,0 +1,309 @@
+// Row-streaming BF16 / FP16 FilterAdjointBlockReduceSum for BorderValid,
+// unit steps, cross-correlation orientation (the weight gradient of a
+// stride-1 layer):
+//
+//   dW[k*N + n](i, j) = sum_b sum_{y, x} dy_{b*K + k}(y, x) * X_{b*N + n}(y + i, x + j)
+//
+// (ConvolveHyperKernel.scala:1310-1336 for the contraction and output shape;
+// ConvolveToSmallFieldPipelinedTiledHyperKernel.scala:45-127 for the
+// reference's own split reduction of this case.)
+//
+// GEMM view: rows = the K dy planes, columns = (n, i, j), reduction over the
+// pixels (b, y, x).  v_mfma_f32_16x16x32_{bf16,f16}: the 32-deep slice is 32
+// consecutive pixels x0 .. x0 + 31 of one dy row y.  A = dy (lane l: plane
+// l%16, pixels 8(l/16) .. +7), B = the x samples under tap (i, j) (lane l:
+// plane l%16, pixels 8(l/16) + j .. +7 of x row y + i); D[k][n] leaves lane
+// l with planes k = 4(l/16) .. +3 of column n = l%16.
+//
+// A block owns a 32-pixel column chunk of one image, a run of dy rows and a
+// tile of (k, n) planes, and walks down the rows one a step: each step stages
+// the next dy row and the x row entering the window (a ring of OD + 1 x rows)
+// while the current row's MFMAs run, so each sample is loaded once per block.
+// The tap-column shift j is applied in registers: a lane reads its 8 pixels
+// and the next 8 (two 16-byte LDS reads per x row) and forms the window at
+// offset j with byte-align instructions (odd j) or a choice of registers
+// (even j); no shifted copies are staged.
+//
+// Wave w owns n block w % NB and k blocks (w / NB) * KBW .. +KBW-1 of the
+// tile and writes its accumulators straight into the split's partial,
+// ws[s][k][n][i][j]; the partials are summed afterwards in split order
+// (launch_reduce_partials), so the result does not depend on timing.
+#include "lp.h"
+#include "stage.h"
+
+namespace cogconv {
+
+namespace {
+
+constexpr int FR_OOB = (int)0x80000000u;  // lane offset past every buffer: the load reads 0
+constexpr int FR_XS = 5;                   // 8-pixel slots of a staged x row (32 + OD - 1 <= 40 pixels)
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Eight consecutive floats at lane byte offset `vo` + uniform offset `so`:
+// two 16-byte loads (vec 4, rows 16-byte aligned) or eight 4-byte loads (a
+// plan-time, block-uniform choice).  (No 8-byte form: this toolchain lowers
+// the 8-byte raw buffer load builtin to a single 4-byte load.)
+__device__ __forceinline__ void load8(__amdgpu_buffer_rsrc_t r, int vo, int so, int vec, float (&v)[8]) {
+    if (vec == 4) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16 * h, so, 0));
+            v[4 * h] = q.x;
+            v[4 * h + 1] = q.y;
+            v[4 * h + 2] = q.z;
+            v[4 * h + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = buffer_load_f32(r, vo + 4 * e, so);
+    }
+}
+
+template <int PREC>
+__device__ __forceinline__ typename LpType<PREC>::V round8(const float (&v)[8]) {
+    typename LpType<PREC>::V h;
+#pragma unroll
+    for (int e = 0; e < 8; e++) h[e] = (typename LpType<PREC>::T)v[e];
+    return h;
+}
+
+// Elements j .. j + 7 of the 16-element run cur ++ nxt (16-bit elements).
+template <int J, typename V>
+__device__ __forceinline__ V window(V cur, V nxt) {
+    if constexpr (J == 0) {
+        return cur;
+    } else {
+        const u32x4 c = __builtin_bit_cast(u32x4, cur), n = __builtin_bit_cast(u32x4, nxt);
+        const unsigned w[8] = {c.x, c.y, c.z, c.w, n.x, n.y, n.z, n.w};
+        u32x4 o;
+        constexpr int q = J / 2;
+        if constexpr (J % 2 == 0) {
+            o = u32x4{w[q], w[q + 1], w[q + 2], w[q + 3]};
+        } else {
+            o = u32x4{__builtin_amdgcn_alignbyte(w[q + 1], w[q], 2), __builtin_amdgcn_alignbyte(w[q + 2], w[q + 1], 2),
+                      __builtin_amdgcn_alignbyte(w[q + 3], w[q + 2], 2),
+                      __builtin_amdgcn_alignbyte(w[q + 4], w[q + 3], 2)};
+        }
+        return __builtin_bit_cast(V, o);
+    }
+}
+
+}  // namespace
+
+template <int PREC, int OD, int KBW, int NB>
+__global__ __launch_bounds__(256) void k_fa_rows_lp(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
+                                                    const float* __restrict__ DY, float* __restrict__ ws) {
+    typedef typename LpType<PREC>::T T;
+    typedef typename LpType<PREC>::V V;
+    constexpr int KH = 4 / NB;         // k groups of waves
+    constexpr int KT = KH * KBW * 16;  // dy planes of the tile
+    constexpr int NT = NB * 16;        // x planes of the tile
+    constexpr int RB = OD + 1;         // x rows in the ring
+    constexpr int P = OD * OD;
+    extern __shared__ __attribute__((aligned(16))) unsigned char fr_smem[];
+    T* sD = reinterpret_cast<T*>(fr_smem);  // [2][4][KT][8]: dy rows, double-buffered
+    T* sX = sD + 2 * 4 * KT * 8;            // [RB][FR_XS][NT][8]: x row ring
+
+    const int tiles = c.nTiles * c.kTiles;
+    const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int tile = bid % tiles, s = bid / tiles;  // s: the partial this block writes
+    const int nt = tile % c.nTiles, kt = tile / c.nTiles;
+    const int rs = s % c.rowSplits, bc = s / c.rowSplits;
+    const int ch = bc % c.chunks, b = bc / c.chunks;
+    const int K = c.K, N = m.N;
+    const int dyH = g.FH, dyW = g.FW, H = g.H, W = g.W;
+    const int x0 = ch * 32;
+    const int y0 = rs * c.rowsPer, y1 = min(dyH, y0 + c.rowsPer);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g4 = lane >> 4, l16 = lane & 15;
+    const int nb = wave % NB, kh = wave / NB;
+
+    const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * dyH * dyW, (uint32_t)K * dyH * dyW * 4u);
+    const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + (size_t)b * N * H * W, (uint32_t)N * H * W * 4u);
+
+    // Staging roles.  dy: thread (slot gs, plane k), plane fastest, 8 pixels
+    // x0 + 8 gs .. +7 of the row; x: thread (slot gs, plane n), 8 pixels of the
+    // entering row.  The whole offset is the lane offset, so the buffer's range
+    // check covers it (a load past the image reads 0); pixels past the row are
+    // zeroed after the load (the loads run on into the next row).
+    // dy: 4 KT units (KT <= 128), DU a thread: unit u = tid + 256 d.
+    constexpr int DU = (4 * KT + 255) / 256;
+    bool dUnit[DU];
+    int dPix[DU], dVo[DU], dDst[DU];
+#pragma unroll
+    for (int d = 0; d < DU; d++) {
+        const int u = tid + 256 * d;
+        const int dk = u % KT, dgs = u / KT;
+        const int dPlane = kt * KT + dk;
+        dUnit[d] = u < 4 * KT;
+        dPix[d] = x0 + 8 * dgs;
+        dVo[d] = dUnit[d] && dPlane < K ? 4 * (dPlane * dyH * dyW + dPix[d]) : FR_OOB;
+        dDst[d] = (dgs * KT + dk) * 8;  // + buffer * 4 * KT * 8
+    }
+    const bool xUnit = tid < FR_XS * NT;
+    const int xn = tid % NT, xgs = tid / NT;
+    const int xPlane = nt * NT + xn;
+    const int xPix = x0 + 8 * xgs;
+    const int xVo = xUnit && xPlane < N ? 4 * (xPlane * H * W + xPix) : FR_OOB;
+    const int xDst = (xgs * NT + xn) * 8;  // + ring row * FR_XS * NT * 8
+
+    auto load_dy = [&](int y, float (&v)[DU][8]) {
+#pragma unroll
+        for (int d = 0; d < DU; d++) {
+            load8(dyr, dVo[d] + 4 * y * dyW, 0, c.vecDy, v[d]);
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[d][e] = dPix[d] + e < dyW ? v[d][e] : 0.f;
+        }
+    };
+    auto store_dy = [&](int buf, const float (&v)[DU][8]) {
+#pragma unroll
+        for (int d = 0; d < DU; d++)
+            if (dUnit[d]) *reinterpret_cast<V*>(sD + buf * 4 * KT * 8 + dDst[d]) = round8<PREC>(v[d]);
+    };
+    auto load_x = [&](int y, float (&v)[8]) {
+        load8(xr, xVo + 4 * y * W, 0, c.vecX, v);
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = xPix + e < W ? v[e] : 0.f;
+    };
+
+    floatx4 acc[KBW][OD][OD];
+#pragma unroll
+    for (int kb = 0; kb < KBW; kb++)
+#pragma unroll
+        for (int i = 0; i < OD; i++)
+#pragma unroll
+            for (int j = 0; j < OD; j++) acc[kb][i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    if (y0 < y1) {  // block-uniform
+        // Prologue: x rows y0 .. y0 + OD - 1 into ring rows (y0 + i) % RB, dy row y0 into buffer 0.
+        for (int i = 0; i < OD; i++) {
+            float v[8];
+            load_x(y0 + i, v);
+            if (xUnit) *reinterpret_cast<V*>(sX + ((y0 + i) % RB) * FR_XS * NT * 8 + xDst) = round8<PREC>(v);
+        }
+        {
+            float v[DU][8];
+            load_dy(y0, v);
+            store_dy(0, v);
+        }
+        __syncthreads();
+        int ring0 = y0 % RB;  // ring row of x row y
+        for (int y = y0; y < y1; y++) {
+            const int t = y - y0;
+            const bool more = y + 1 < y1;  // block-uniform
+            float pd[DU][8], px[8];
+            if (more) {  // the next step's rows, in flight behind this step's MFMAs
+                load_dy(y + 1, pd);
+                load_x(y + OD, px);
+            }
+            const T* dBuf = sD + (t & 1) * 4 * KT * 8;
+            V a[KBW];
+#pragma unroll
+            for (int kb = 0; kb < KBW; kb++)
+                a[kb] = *reinterpret_cast<const V*>(dBuf + (g4 * KT + (kh * KBW + kb) * 16 + l16) * 8);
+#pragma unroll
+            for (int i = 0; i < OD; i++) {
+                int rr = ring0 + i;
+                rr -= rr >= RB ? RB : 0;
+                const T* xRow = sX + (rr * FR_XS * NT + nb * 16 + l16) * 8;
+                const V cur = *reinterpret_cast<const V*>(xRow + g4 * NT * 8);
+                const V nxt = *reinterpret_cast<const V*>(xRow + (g4 + 1) * NT * 8);
+                auto tap = [&](auto jc) {
+                    constexpr int J = decltype(jc)::value;
+                    const V bj = window<J>(cur, nxt);
+#pragma unroll
+                    for (int kb = 0; kb < KBW; kb++) acc[kb][i][J] = lp_mfma<PREC>(a[kb], bj, acc[kb][i][J]);
+                };
+                tap(std::integral_constant<int, 0>{});
+                if constexpr (OD > 1) {
+                    tap(std::integral_constant<int, 1>{});
+                    tap(std::integral_constant<int, 2>{});
+                }
+                if constexpr (OD > 3) {
+                    tap(std::integral_constant<int, 3>{});
+                    tap(std::integral_constant<int, 4>{});
+                }
+            }
+            if (more) {
+                // dy buffer (t + 1) & 1 and the ring row of x row y - 1 were last
+                // read by the previous step, which every wave finished before
+                // that step's barrier.
+                store_dy((t + 1) & 1, pd);
+                int rw = ring0 + OD;
+                rw -= rw >= RB ? RB : 0;
+                if (xUnit) *reinterpret_cast<V*>(sX + rw * FR_XS * NT * 8 + xDst) = round8<PREC>(px);
+            }
+            __syncthreads();
+            ring0 = ring0 + 1 == RB ? 0 : ring0 + 1;
+        }
+    }
+    // The split's partial: ws[s][k][n][i][j] (every (k, n, i, j) of the tile,
+    // zeros for a block without rows).
+    const size_t total = (size_t)K * N * P;
+    float* out = ws + (size_t)s * total;
+    const int n = nt * NT + nb * 16 + l16;
+#pragma unroll
+    for (int kb = 0; kb < KBW; kb++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int k = kt * KT + (kh * KBW + kb) * 16 + 4 * g4 + v;
+            if (k < K && n < N) {
+                float* o = out + ((size_t)k * N + n) * P;
+#pragma unroll
+                for (int i = 0; i < OD; i++)
+#pragma unroll
+                    for (int j = 0; j < OD; j++) o[i * OD + j] = acc[kb][i][j][v];
+            }
+        }
+}
+
+size_t fa_rows_lp_lds_bytes(const FaCfg& c, int OD) {
+    const int KT = (4 / c.nb) * c.kbw * 16, NT = c.nb * 16;
+    return (size_t)2 * 4 * KT * 16 + (size_t)(OD + 1) * FR_XS * NT * 16;
+}
+
+template <int PREC, int OD>
+static void fa_rows_launch2(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
+                            float* ws, hipStream_t s, dim3 grid) {
+    const int sel = c.nb * 10 + c.kbw;
+    switch (sel) {
+    case 21: hipLaunchKernelGGL((k_fa_rows_lp<PREC, OD, 1, 2>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    case 11: hipLaunchKernelGGL((k_fa_rows_lp<PREC, OD, 1, 1>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    case 12: hipLaunchKernelGGL((k_fa_rows_lp<PREC, OD, 2, 1>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    default: hipLaunchKernelGGL((k_fa_rows_lp<PREC, OD, 2, 2>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+    }
+}
+
+template <int PREC>
+static hipError_t fa_rows_launch1(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
+                                  float* ws, hipStream_t s, dim3 grid) {
+    switch (g.OH) {
+    case 1: fa_rows_launch2<PREC, 1>(g, m, c, x, dy, ws, s, grid); break;
+    case 3: fa_rows_launch2<PREC, 3>(g, m, c, x, dy, ws, s, grid); break;
+    case 5: fa_rows_launch2<PREC, 5>(g, m, c, x, dy, ws, s, grid); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fa_rows_lp(int precision, const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x,
+                             const float* dy, float* out, float* ws, hipStream_t s, const Epi& ep) {
+    const long blocks = (long)c.nTiles * c.kTiles * c.splits;
+    if (blocks <= 0 || blocks > 0x7fffffffL) return hipErrorInvalidConfiguration;
+    const dim3 grid((unsigned)blocks);
+    hipError_t e;
+    switch (precision) {
+    case COGCONV_BF16: e = fa_rows_launch1<COGCONV_BF16>(g, m, c, x, dy, ws, s, grid); break;
+    case COGCONV_FP16: e = fa_rows_launch1<COGCONV_FP16>(g, m, c, x, dy, ws, s, grid); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    const long total = (long)c.K * m.N * g.OH * g.OW;
+    return launch_reduce_partials(ws, out, c.splits, total, (long)g.OH * g.OW, s, ep);
+}
+
+}  // namespace cogconv

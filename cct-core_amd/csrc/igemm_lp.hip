@@ -83,7 +83,11 @@ __device__ __forceinline__ void igemm_lp_body(const Geom& g, const PlaneMap& m, 
     constexpr int TERMS = SPLIT ? 2 : 1;
     extern __shared__ __attribute__((aligned(16))) unsigned char lp_smem[];
     const int TRp = c.tileRows, TCp = c.tileCols;
-    const int npix = TRp * TCp;
+    // Slots of one 8-plane quarter of the tile, padded to a multiple of 16
+    // (quarters start at multiples of 256 bytes): the four quarter groups of
+    // lanes of one read then meet no bank conflicts, where strides of 532 or
+    // 340 slots measured about three a read.
+    const int npix = (TRp * TCp + 15) & ~15;
     // LDS holds both operands quarter-major: 8 planes (16 bytes) of one pixel
     // or output plane per slot, the four 8-plane quarters of a 32-plane chunk
     // in separate planes of slots, so that the 16 lanes of an MFMA row group
@@ -94,7 +98,7 @@ __device__ __forceinline__ void igemm_lp_body(const Geom& g, const PlaneMap& m, 
     const int TRG = RING ? 1 : c.trg;
     const int TAPG = TRG * F;
     const int gElems = TERMS * TAPG * 4 * MT * 8;             // one filter group (buffer)
-    T* sB = reinterpret_cast<T*>(lp_smem);                    // [TERMS][4][npix][8]
+    T* sB = reinterpret_cast<T*>(lp_smem);                    // [TERMS][4][npix][8], npix padded
     T* sA = sB + (size_t)TERMS * 4 * npix * 8;               // [RING ? 2 : 1][TERMS][TAPG][4][MT][8]
     // Scratch slots (offset from sB) for the stores of lanes beyond the tile
     // width, after the filter slice(s): TERMS x 64 lanes x 8 elements.
@@ -567,7 +571,8 @@ size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision) {
     // groups in ring mode), 16-bit elements, + scratch slots of the lanes
     // beyond the tile width
     const size_t group = (size_t)(c.ring ? 1 : c.trg) * F * 16 * c.MB;
-    return (size_t)terms * ((size_t)c.tileRows * c.tileCols + (c.ring ? 2 : 1) * group) * 64 +
+    const size_t quarter = ((size_t)c.tileRows * c.tileCols + 15) & ~(size_t)15;  // padded (see the kernel)
+    return (size_t)terms * (quarter + (c.ring ? 2 : 1) * group) * 64 +
            (size_t)terms * 64 * 8 * 2;
 }
 

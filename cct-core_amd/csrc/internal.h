@@ -199,6 +199,11 @@ struct FaCfg {
     int rowChunks;  // ceil(FHe / RR)
     int splits;     // partial results
     int NW;         // waves per block (FP32 kernel: 4, or 8 with the tap columns split between wave pairs)
+    // Row-streaming BF16 / FP16 kernel (k_fa_rows_lp, rows = 1): NB x-plane
+    // blocks and KBW dy-plane blocks a wave; tiles of (4/nb)*kbw*16 dy planes
+    // by nb*16 x planes; 32-pixel column chunks of the dy rows; row splits of
+    // rowsPer rows; load widths (floats) of the dy and x rows.
+    int rows, nb, kbw, nTiles, kTiles, chunks, rowSplits, rowsPer, vecDy, vecX;
     size_t lds_bytes;
 };
 
@@ -261,6 +266,9 @@ hipError_t launch_fa_igemm_f32(const Geom& g, const PlaneMap& m, const FaCfg& c,
 hipError_t launch_reduce_partials(const float* ws, float* out, int splits, long total, long points, hipStream_t s,
                                   const Epi& e);
 size_t fa_igemm_lp_lds_bytes(const FaCfg& c, int OW, int precision);
+size_t fa_rows_lp_lds_bytes(const FaCfg& c, int OD);
+hipError_t launch_fa_rows_lp(int precision, const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x,
+                             const float* dy, float* out, float* ws, hipStream_t s, const Epi& ep);
 hipError_t launch_fa_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x,
                               const float* dy, float* out, float* ws, hipStream_t s, const Epi& e = kNoEpi);
 bool igemm_lp_supported(int F);

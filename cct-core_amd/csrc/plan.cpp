@@ -447,6 +447,54 @@ bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m,
     return true;
 }
 
+// Row-streaming BF16 / FP16 FilterAdjointBlockReduceSum (k_fa_rows_lp):
+// BorderValid, unit steps, cross-correlation, a square 1x1, 3x3 or 5x5 dW.
+// nb / kbw pick the tile (0: from K and N); rowSplits (0: from the block
+// count) splits each image's dy rows so that the grid fills the CUs.
+bool choose_fa_rows_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out, int nb = 0, int kbw = 0,
+                       int rowSplits = 0) {
+    if (d->mode != COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM || d->border != COGCONV_BORDER_VALID || d->in0.dims != 2)
+        return false;
+    if (d->precision != COGCONV_BF16 && d->precision != COGCONV_FP16) return false;
+    if (g.flip || g.upFr != 1 || g.upFc != 1 || g.upAr != 1 || g.upAc != 1 || g.dnR != 1 || g.dnC != 1) return false;
+    if (g.OH != g.OW || (g.OH != 1 && g.OH != 3 && g.OH != 5)) return false;
+    FaCfg c;
+    std::memset(&c, 0, sizeof c);
+    c.rows = 1;
+    c.K = m.T / m.B;
+    const int N = m.N;
+    if (!nb) nb = N <= 16 ? 1 : 2;
+    if (!kbw) kbw = (g.OH == 5 || c.K <= 32) ? 1 : 2;
+    if ((nb != 1 && nb != 2) || (kbw != 1 && kbw != 2)) return false;
+    if (g.OH == 5 && kbw == 2) return false;  // 2 x 25 accumulators would not fit the registers
+    c.nb = nb;
+    c.kbw = kbw;
+    const int KT = (4 / nb) * kbw * 16, NT = nb * 16;
+    c.nTiles = (N + NT - 1) / NT;
+    c.kTiles = (c.K + KT - 1) / KT;
+    c.chunks = (g.FW + 31) / 32;
+    // Byte offsets of the staged samples (up to 40 pixels past a row start,
+    // and the tile's padding planes) stay below 2^31 inside one image.
+    if ((long)(c.K + KT) * g.FH * g.FW * 4 >= (1L << 31) || (long)(N + NT) * g.H * g.W * 4 >= (1L << 31)) return false;
+    const long perRows = (long)c.nTiles * c.kTiles * m.B * c.chunks;  // blocks without row splits
+    if (!rowSplits) {
+        rowSplits = 1;
+        while (perRows * rowSplits < 512 && rowSplits * 2 <= g.FH && rowSplits < 64) rowSplits *= 2;
+    }
+    if (rowSplits < 1 || rowSplits > g.FH) return false;
+    if (perRows * rowSplits > 0x7fffffffL || (long)m.B * c.chunks * rowSplits > 0x7fffffffL) return false;
+    c.rowSplits = rowSplits;
+    c.rowsPer = (g.FH + rowSplits - 1) / rowSplits;
+    c.splits = m.B * c.chunks * rowSplits;
+    // Load widths: 16-byte loads where rows start 16-byte aligned, else
+    // 4-byte (the launch checks the buffers' own alignment).
+    c.vecDy = g.FW % 4 == 0 ? 4 : 1;
+    c.vecX = g.W % 4 == 0 ? 4 : 1;
+    c.lds_bytes = fa_rows_lp_lds_bytes(c, g.OH);
+    *out = c;
+    return true;
+}
+
 // BF16 / FP16 / split-BF16 MFMA implicit GEMM for the frame block-reduce modes.
 bool choose_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, IgemmCfg* out, int mbOnly = 0,
                      bool ring = false) {
@@ -676,7 +724,19 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         out.push_back(v);
     };
     if (desc->precision != COGCONV_FP32) {
-        FaCfg flp;
+        FaCfg flp, frl;
+        const bool rows = choose_fa_rows_lp(desc, g, m, &frl);
+        if (rows) {
+            add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &frl);
+            for (int nb : {2, 1})
+                for (int kbw : {2, 1})
+                    for (int rsp : {1, 2, 4, 8}) {
+                        FaCfg c2;
+                        if (choose_fa_rows_lp(desc, g, m, &c2, nb, kbw, rsp) &&
+                            (c2.nb != frl.nb || c2.kbw != frl.kbw || c2.rowSplits != frl.rowSplits))
+                            add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &c2);
+                    }
+        }
         if (choose_fa_igemm_lp(desc, g, m, &flp)) {
             add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &flp);
             for (int cw : {0, 128, 64, 32})
@@ -689,6 +749,7 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
                         }
             return;
         }
+        if (rows) return;
         IgemmCfg lp;
         if (!choose_igemm_lp(desc, g, m, &lp)) return;
         add(KK_IGEMM_LP, nullptr, nullptr, &lp, nullptr);
@@ -990,6 +1051,14 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
     }
     case KK_FA_IGEMM_LP: {
         static const char* pn[] = {"fp32", "bf16", "fp16", "split_bf16"};
+        if (plan->fcfg.rows) {
+            const FaCfg& f = plan->fcfg;
+            snprintf(buf, len, "fa_rows_%s %s out=%dx%dx%d tile=%dx%d chunks=%d rowsplits=%d splits=%d ld=%d/%d lds=%zu",
+                     pn[plan->desc.precision], mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW,
+                     (4 / f.nb) * f.kbw * 16, f.nb * 16, f.chunks, f.rowSplits, f.splits, f.vecDy, f.vecX,
+                     f.lds_bytes);
+            break;
+        }
         snprintf(buf, len, "fa_igemm_%s %s out=%dx%dx%d MB=%d CBW=%d RR=%d CW=%d groups=%d splits=%d waves=%d lds=%zu",
                  pn[plan->desc.precision], mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, plan->fcfg.MB,
                  plan->fcfg.CBW, plan->fcfg.RR, plan->fcfg.CW, plan->fcfg.colGroups, plan->fcfg.splits,
@@ -1075,8 +1144,18 @@ int launch_all(cogconv_plan plan, const float* in0, const float* in1, float* out
             e = launch_igemm_f32_phase(plan->ps.g[i], plan->m, plan->ps.c[i], plan->ps.p[i], in0, in1, out, s, ep);
         break;
     case KK_FA_IGEMM_LP:
-        e = launch_fa_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->fcfg, in0, in1, out,
-                               static_cast<float*>(workspace), s, ep);
+        if (plan->fcfg.rows) {
+            // The row loads also need the buffers aligned to their width.
+            FaCfg f = plan->fcfg;
+            const uintptr_t ax = reinterpret_cast<uintptr_t>(in0), ad = reinterpret_cast<uintptr_t>(in1);
+            if (ax % (4 * f.vecX)) f.vecX = 1;
+            if (ad % (4 * f.vecDy)) f.vecDy = 1;
+            e = launch_fa_rows_lp(plan->desc.precision, plan->g, plan->m, f, in0, in1, out,
+                                  static_cast<float*>(workspace), s, ep);
+        } else {
+            e = launch_fa_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->fcfg, in0, in1, out,
+                                   static_cast<float*>(workspace), s, ep);
+        }
         break;
     case KK_FFT:
         return launch_fft(plan->g, plan->fft, &plan->fft_state, in0, in1, out, workspace, s, ep);

@@ -149,7 +149,7 @@ def test_filter_adjoint_precision_accuracy(precision, orient, sampling, step, xs
         plan = L.Plan(cd)
         print("CASE fa precision", NAMES[precision], xs, ys, "wide" if wide else "uniform", plan.description,
               flush=True)
-        assert plan.description.startswith("fa_igemm_" + NAMES[precision])
+        assert plan.description.startswith(("fa_igemm_" + NAMES[precision], "fa_rows_" + NAMES[precision]))
         plan.close()
         got = run_gpu(cd, x, dy)
         ref = O.conv(od, x, dy, threads=16)
@@ -272,3 +272,63 @@ def test_precision_frame_every_candidate(precision, mode, border, F):
             assert any("ld16" in n for n in names), names
         if F > 1:
             assert any("ring" in n for n in names), names
+
+
+def rounded(a, precision):
+    """`a` rounded to the MFMA input type (round to nearest even), as float32:
+    products of such values are exact in FP32, so a kernel that adds them in
+    FP32 meets an FP64 oracle run on the rounded inputs to FP32 summation error."""
+    a = np.ascontiguousarray(a, np.float32)
+    if precision == L.FP16:
+        return a.astype(np.float16).astype(np.float32)
+    bits = a.view(np.uint32).astype(np.uint64)
+    bits = (bits + 0x7FFF + ((bits >> 16) & 1)) & 0xFFFF0000
+    return bits.astype(np.uint32).view(np.float32)
+
+
+ROWS_CASES = [
+    # (B, N, H, W), K: dW side H - dh + 1 with dh = H - OD + 1
+    ((2, 64, 34, 36), 64, 3),   # c3 planes; dy rows 34 wide: a full and a 2-pixel chunk
+    ((3, 20, 23, 70), 40, 3),   # partial x- and dy-plane tiles; x rows 70 wide (4-byte loads), dy rows 68 (16-byte)
+    ((2, 16, 19, 37), 24, 1),   # 1x1 dW; odd widths (4-byte loads)
+    ((2, 12, 21, 45), 20, 5),   # 5x5 dW; dy rows 41 wide
+]
+
+
+@pytest.mark.parametrize("precision", (L.BF16, L.FP16))
+@pytest.mark.parametrize("xs,K,OD", ROWS_CASES)
+def test_filter_adjoint_rows_every_candidate(precision, xs, K, OD):
+    """Every configuration of the row-streaming FilterAdjoint kernel (tiles,
+    row splits, load widths): against the FP64 oracle on the inputs rounded to
+    the MFMA type (normwise <= 2e-6: only FP32 summation differs), against the
+    oracle on the FP32 inputs (the precision bound), and bit for bit between
+    tile shapes with the same row splits (same partial sums, same order)."""
+    B, N, H, W = xs
+    dh, dw = H - OD + 1, W - OD + 1
+    rng = np.random.default_rng(8800 + 10 * precision + OD + K)
+    x = inputs(rng, (B * N, H, W), False)
+    dy = inputs(rng, (B * K, dh, dw), False)
+    od, cd = desc_pair(O.FA_BRS, O.VALID, O.XCORR, O.NONE, 1, B, x.shape, dy.shape, precision=precision)
+    ref = O.conv(od, x, dy, threads=16)
+    ref_r = O.conv(od, rounded(x, precision), rounded(dy, precision), threads=16)
+    by_split = {}
+    seen = 0
+    for v in range(L.variant_count(cd)):
+        plan = L.Plan(cd, v)
+        desc = plan.description
+        plan.close()
+        if not desc.startswith("fa_rows_"):
+            continue
+        seen += 1
+        print("CASE fa rows", NAMES[precision], xs, K, OD, v, desc, flush=True)
+        got = run_gpu(cd, x, dy, variant=v)
+        assert got.shape == ref.shape
+        err_r = normwise(got, ref_r)
+        assert err_r <= 2e-6, (desc, err_r)
+        assert normwise(got, ref) <= TOL[precision], desc
+        key = desc.split("rowsplits=")[1].split()[0]
+        if key in by_split:
+            assert np.array_equal(got, by_split[key][1]), (desc, by_split[key][0])
+        else:
+            by_split[key] = (desc, got)
+    assert seen >= 2, "row-streaming kernel not offered"
