@@ -591,7 +591,7 @@ def large_workload(args):
         b.close()
     print(json.dumps(line), flush=True)
 RATE_SWEEP = (4000, 8000, 12000, 16000, 20000, 24000, 28000, 32000, 40000, 48000, 56000, 64000, 72000, 80000,
-              96000)  # offered req/s, resident transport
+              96000, 112000, 128000, 144000, 160000)  # offered req/s, resident transport
 P99_TARGETS_MS = (0.25, 0.5, 1.0, 2.0)
 
 

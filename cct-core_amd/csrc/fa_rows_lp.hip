@@ -24,8 +24,10 @@
 // offset j with byte-align instructions (odd j) or a choice of registers
 // (even j); no shifted copies are staged.
 //
-// Wave w owns n block w % NB and k blocks (w / NB) * KBW .. +KBW-1 of the
-// tile and writes its accumulators straight into the split's partial,
+// A block has NB x KH waves: wave w owns x-plane block w % NB and dy-plane
+// blocks (w / NB) * KBW .. +KBW-1 of the tile (a 64 x 64 tile with 8 waves
+// reads each dy and x row once) and writes its accumulators straight into the
+// split's partial,
 // ws[s][k][n][i][j]; the partials are summed afterwards in split order
 // (launch_reduce_partials), so the result does not depend on timing.
 #include "lp.h"
@@ -37,6 +39,7 @@ namespace {
 
 constexpr int FR_OOB = (int)0x80000000u;  // lane offset past every buffer: the load reads 0
 constexpr int FR_XS = 5;                   // 8-pixel slots of a staged x row (32 + OD - 1 <= 40 pixels)
+constexpr int FR_PD = 4;                   // steps of row loads in flight (register queue stages)
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
@@ -91,12 +94,12 @@ __device__ __forceinline__ V window(V cur, V nxt) {
 
 }  // namespace
 
-template <int PREC, int OD, int KBW, int NB>
-__global__ __launch_bounds__(256) void k_fa_rows_lp(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
-                                                    const float* __restrict__ DY, float* __restrict__ ws) {
+template <int PREC, int OD, int KBW, int NB, int KH>
+__global__ __launch_bounds__(64 * NB * KH) void k_fa_rows_lp(Geom g, PlaneMap m, FaCfg c, const float* __restrict__ X,
+                                                             const float* __restrict__ DY, float* __restrict__ ws) {
     typedef typename LpType<PREC>::T T;
     typedef typename LpType<PREC>::V V;
-    constexpr int KH = 4 / NB;         // k groups of waves
+    constexpr int NTH = 64 * NB * KH;  // threads: NB x-plane blocks by KH groups of KBW dy-plane blocks
     constexpr int KT = KH * KBW * 16;  // dy planes of the tile
     constexpr int NT = NB * 16;        // x planes of the tile
     constexpr int RB = OD + 1;         // x rows in the ring
@@ -125,18 +128,19 @@ __global__ __launch_bounds__(256) void k_fa_rows_lp(Geom g, PlaneMap m, FaCfg c,
     const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * dyH * dyW, (uint32_t)K * dyH * dyW * 4u);
     const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + (size_t)b * N * H * W, (uint32_t)N * H * W * 4u);
 
-    // Staging roles.  dy: thread (slot gs, plane k), plane fastest, 8 pixels
-    // x0 + 8 gs .. +7 of the row; x: thread (slot gs, plane n), 8 pixels of the
-    // entering row.  The whole offset is the lane offset, so the buffer's range
-    // check covers it (a load past the image reads 0); pixels past the row are
-    // zeroed after the load (the loads run on into the next row).
-    // dy: 4 KT units (KT <= 128), DU a thread: unit u = tid + 256 d.
-    constexpr int DU = (4 * KT + 255) / 256;
-    bool dUnit[DU];
-    int dPix[DU], dVo[DU], dDst[DU];
+    // Staging units.  dy: (slot gs, plane k), plane fastest, 8 pixels x0 + 8 gs
+    // .. +7 of the row, 4 KT units; x: (slot gs, plane n), 8 pixels of the
+    // entering row, FR_XS NT units; unit u = tid + NTH d.  The whole offset is
+    // the lane offset, so the buffer's range check covers it (a load past the
+    // image reads 0); pixels past the row are zeroed after the load (the loads
+    // run on into the next row).
+    constexpr int DU = (4 * KT + NTH - 1) / NTH;
+    constexpr int XU = (FR_XS * NT + NTH - 1) / NTH;
+    bool dUnit[DU], xUnit[XU];
+    int dPix[DU], dVo[DU], dDst[DU], xPix[XU], xVo[XU], xDst[XU];
 #pragma unroll
     for (int d = 0; d < DU; d++) {
-        const int u = tid + 256 * d;
+        const int u = tid + NTH * d;
         const int dk = u % KT, dgs = u / KT;
         const int dPlane = kt * KT + dk;
         dUnit[d] = u < 4 * KT;
@@ -144,12 +148,16 @@ __global__ __launch_bounds__(256) void k_fa_rows_lp(Geom g, PlaneMap m, FaCfg c,
         dVo[d] = dUnit[d] && dPlane < K ? 4 * (dPlane * dyH * dyW + dPix[d]) : FR_OOB;
         dDst[d] = (dgs * KT + dk) * 8;  // + buffer * 4 * KT * 8
     }
-    const bool xUnit = tid < FR_XS * NT;
-    const int xn = tid % NT, xgs = tid / NT;
-    const int xPlane = nt * NT + xn;
-    const int xPix = x0 + 8 * xgs;
-    const int xVo = xUnit && xPlane < N ? 4 * (xPlane * H * W + xPix) : FR_OOB;
-    const int xDst = (xgs * NT + xn) * 8;  // + ring row * FR_XS * NT * 8
+#pragma unroll
+    for (int d = 0; d < XU; d++) {
+        const int u = tid + NTH * d;
+        const int xn = u % NT, xgs = u / NT;
+        const int xPlane = nt * NT + xn;
+        xUnit[d] = u < FR_XS * NT;
+        xPix[d] = x0 + 8 * xgs;
+        xVo[d] = xUnit[d] && xPlane < N ? 4 * (xPlane * H * W + xPix[d]) : FR_OOB;
+        xDst[d] = (xgs * NT + xn) * 8;  // + ring row * FR_XS * NT * 8
+    }
 
     auto load_dy = [&](int y, float (&v)[DU][8]) {
 #pragma unroll
@@ -164,10 +172,18 @@ __global__ __launch_bounds__(256) void k_fa_rows_lp(Geom g, PlaneMap m, FaCfg c,
         for (int d = 0; d < DU; d++)
             if (dUnit[d]) *reinterpret_cast<V*>(sD + buf * 4 * KT * 8 + dDst[d]) = round8<PREC>(v[d]);
     };
-    auto load_x = [&](int y, float (&v)[8]) {
-        load8(xr, xVo + 4 * y * W, 0, c.vecX, v);
+    auto load_x = [&](int y, float (&v)[XU][8]) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) v[e] = xPix + e < W ? v[e] : 0.f;
+        for (int d = 0; d < XU; d++) {
+            load8(xr, xVo[d] + 4 * y * W, 0, c.vecX, v[d]);
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[d][e] = xPix[d] + e < W ? v[d][e] : 0.f;
+        }
+    };
+    auto store_x = [&](int row, const float (&v)[XU][8]) {
+#pragma unroll
+        for (int d = 0; d < XU; d++)
+            if (xUnit[d]) *reinterpret_cast<V*>(sX + row * FR_XS * NT * 8 + xDst[d]) = round8<PREC>(v[d]);
     };
 
     floatx4 acc[KBW][OD][OD];
@@ -181,64 +197,82 @@ __global__ __launch_bounds__(256) void k_fa_rows_lp(Geom g, PlaneMap m, FaCfg c,
     if (y0 < y1) {  // block-uniform
         // Prologue: x rows y0 .. y0 + OD - 1 into ring rows (y0 + i) % RB, dy row y0 into buffer 0.
         for (int i = 0; i < OD; i++) {
-            float v[8];
+            float v[XU][8];
             load_x(y0 + i, v);
-            if (xUnit) *reinterpret_cast<V*>(sX + ((y0 + i) % RB) * FR_XS * NT * 8 + xDst) = round8<PREC>(v);
+            store_x((y0 + i) % RB, v);
         }
         {
             float v[DU][8];
             load_dy(y0, v);
             store_dy(0, v);
         }
+        // Register queue of FR_PD stages: the rows stored at the end of step t
+        // (dy row y + 1, x row y + OD) were loaded FR_PD - 1 steps earlier into
+        // stage (t + 1) % FR_PD, so that many steps of MFMAs cover their memory
+        // latency.  Steps run in groups of FR_PD (stage indices are constants).
+        float qd[FR_PD][DU][8], qx[FR_PD][XU][8];
+#pragma unroll
+        for (int k = 1; k < FR_PD; k++) {
+            if (y0 + k < y1) {  // block-uniform
+                load_dy(y0 + k, qd[k]);
+                load_x(y0 + k + OD - 1, qx[k]);
+            }
+        }
         __syncthreads();
         int ring0 = y0 % RB;  // ring row of x row y
-        for (int y = y0; y < y1; y++) {
-            const int t = y - y0;
-            const bool more = y + 1 < y1;  // block-uniform
-            float pd[DU][8], px[8];
-            if (more) {  // the next step's rows, in flight behind this step's MFMAs
-                load_dy(y + 1, pd);
-                load_x(y + OD, px);
-            }
-            const T* dBuf = sD + (t & 1) * 4 * KT * 8;
-            V a[KBW];
+        for (int yb = y0; yb < y1; yb += FR_PD) {
 #pragma unroll
-            for (int kb = 0; kb < KBW; kb++)
-                a[kb] = *reinterpret_cast<const V*>(dBuf + (g4 * KT + (kh * KBW + kb) * 16 + l16) * 8);
-#pragma unroll
-            for (int i = 0; i < OD; i++) {
-                int rr = ring0 + i;
-                rr -= rr >= RB ? RB : 0;
-                const T* xRow = sX + (rr * FR_XS * NT + nb * 16 + l16) * 8;
-                const V cur = *reinterpret_cast<const V*>(xRow + g4 * NT * 8);
-                const V nxt = *reinterpret_cast<const V*>(xRow + (g4 + 1) * NT * 8);
-                auto tap = [&](auto jc) {
-                    constexpr int J = decltype(jc)::value;
-                    const V bj = window<J>(cur, nxt);
-#pragma unroll
-                    for (int kb = 0; kb < KBW; kb++) acc[kb][i][J] = lp_mfma<PREC>(a[kb], bj, acc[kb][i][J]);
-                };
-                tap(std::integral_constant<int, 0>{});
-                if constexpr (OD > 1) {
-                    tap(std::integral_constant<int, 1>{});
-                    tap(std::integral_constant<int, 2>{});
+            for (int u = 0; u < FR_PD; u++) {
+                const int y = yb + u;
+                if (y >= y1) break;  // block-uniform
+                const int t = y - y0;
+                // Stage u (last stored at the end of the previous step) takes the
+                // rows for the end of step t + FR_PD - 1.
+                if (y + FR_PD < y1) {
+                    load_dy(y + FR_PD, qd[u]);
+                    load_x(y + FR_PD + OD - 1, qx[u]);
                 }
-                if constexpr (OD > 3) {
-                    tap(std::integral_constant<int, 3>{});
-                    tap(std::integral_constant<int, 4>{});
+                const T* dBuf = sD + (t & 1) * 4 * KT * 8;
+                V a[KBW];
+#pragma unroll
+                for (int kb = 0; kb < KBW; kb++)
+                    a[kb] = *reinterpret_cast<const V*>(dBuf + (g4 * KT + (kh * KBW + kb) * 16 + l16) * 8);
+#pragma unroll
+                for (int i = 0; i < OD; i++) {
+                    int rr = ring0 + i;
+                    rr -= rr >= RB ? RB : 0;
+                    const T* xRow = sX + (rr * FR_XS * NT + nb * 16 + l16) * 8;
+                    const V cur = *reinterpret_cast<const V*>(xRow + g4 * NT * 8);
+                    const V nxt = *reinterpret_cast<const V*>(xRow + (g4 + 1) * NT * 8);
+                    auto tap = [&](auto jc) {
+                        constexpr int J = decltype(jc)::value;
+                        const V bj = window<J>(cur, nxt);
+#pragma unroll
+                        for (int kb = 0; kb < KBW; kb++) acc[kb][i][J] = lp_mfma<PREC>(a[kb], bj, acc[kb][i][J]);
+                    };
+                    tap(std::integral_constant<int, 0>{});
+                    if constexpr (OD > 1) {
+                        tap(std::integral_constant<int, 1>{});
+                        tap(std::integral_constant<int, 2>{});
+                    }
+                    if constexpr (OD > 3) {
+                        tap(std::integral_constant<int, 3>{});
+                        tap(std::integral_constant<int, 4>{});
+                    }
                 }
+                if (y + 1 < y1) {
+                    // dy buffer (t + 1) & 1 and the ring row of x row y - 1 were
+                    // last read by the previous step, which every wave finished
+                    // before that step's barrier.
+                    const int nx = (u + 1) % FR_PD;  // a constant once the group loop is unrolled
+                    store_dy((t + 1) & 1, qd[nx]);
+                    int rw = ring0 + OD;
+                    rw -= rw >= RB ? RB : 0;
+                    store_x(rw, qx[nx]);
+                }
+                __syncthreads();
+                ring0 = ring0 + 1 == RB ? 0 : ring0 + 1;
             }
-            if (more) {
-                // dy buffer (t + 1) & 1 and the ring row of x row y - 1 were last
-                // read by the previous step, which every wave finished before
-                // that step's barrier.
-                store_dy((t + 1) & 1, pd);
-                int rw = ring0 + OD;
-                rw -= rw >= RB ? RB : 0;
-                if (xUnit) *reinterpret_cast<V*>(sX + rw * FR_XS * NT * 8 + xDst) = round8<PREC>(px);
-            }
-            __syncthreads();
-            ring0 = ring0 + 1 == RB ? 0 : ring0 + 1;
         }
     }
     // The split's partial: ws[s][k][n][i][j] (every (k, n, i, j) of the tile,
@@ -262,20 +296,34 @@ __global__ __launch_bounds__(256) void k_fa_rows_lp(Geom g, PlaneMap m, FaCfg c,
 }
 
 size_t fa_rows_lp_lds_bytes(const FaCfg& c, int OD) {
-    const int KT = (4 / c.nb) * c.kbw * 16, NT = c.nb * 16;
+    const int KT = c.kh * c.kbw * 16, NT = c.nb * 16;
     return (size_t)2 * 4 * KT * 16 + (size_t)(OD + 1) * FR_XS * NT * 16;
 }
 
+// Tiles offered (x-plane blocks NB, dy-plane block groups KH, dy-plane blocks a wave KBW).
 template <int PREC, int OD>
 static void fa_rows_launch2(const Geom& g, const PlaneMap& m, const FaCfg& c, const float* x, const float* dy,
                             float* ws, hipStream_t s, dim3 grid) {
-    const int sel = c.nb * 10 + c.kbw;
+    const int sel = c.nb * 100 + c.kh * 10 + c.kbw;
+    const dim3 block(64 * c.nb * c.kh);
+    // (5x5 dW: one dy block a wave, as the planner requires.)
+#define FR_CASE(NB_, KH_, KBW_)                                                                                  \
+    case NB_ * 100 + KH_ * 10 + KBW_:                                                                           \
+        if constexpr (OD != 5 || KBW_ == 1)                                                                     \
+            hipLaunchKernelGGL((k_fa_rows_lp<PREC, OD, KBW_, NB_, KH_>), grid, block, c.lds_bytes, s, g, m, c, x, dy, \
+                               ws);                                                                              \
+        break;
     switch (sel) {
-    case 21: hipLaunchKernelGGL((k_fa_rows_lp<PREC, OD, 1, 2>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
-    case 11: hipLaunchKernelGGL((k_fa_rows_lp<PREC, OD, 1, 1>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
-    case 12: hipLaunchKernelGGL((k_fa_rows_lp<PREC, OD, 2, 1>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
-    default: hipLaunchKernelGGL((k_fa_rows_lp<PREC, OD, 2, 2>), grid, dim3(256), c.lds_bytes, s, g, m, c, x, dy, ws); break;
+        FR_CASE(2, 2, 2)
+        FR_CASE(2, 2, 1)
+        FR_CASE(1, 4, 2)
+        FR_CASE(1, 4, 1)
+        FR_CASE(4, 2, 2)
+        FR_CASE(4, 2, 1)
+        FR_CASE(4, 1, 4)
+    default: break;
     }
+#undef FR_CASE
 }
 
 template <int PREC>

@@ -463,6 +463,29 @@ __device__ __forceinline__ void igemm_lp_body(const Geom& g, const PlaneMap& m, 
             const T* sAg = sA + (RING ? (size_t)(tr0 & 1) * gElems : 0) - (size_t)tr0 * F * 4 * MT * 8;
             const T* sAg2 = sAg + (size_t)TAPG * 4 * MT * 8;  // split: the residual term
             const int tEnd = min(F, tr0 + TRG);
+            if constexpr (!SPLIT) {
+                // The group's rows one at a time (a runtime loop: the row is
+                // uniform and the tap offsets within a row are constants, so the
+                // LDS reads take immediate offsets from one base per row).
+                for (int ti = tr0; ti < tEnd; ti++) {
+                    const T* aRow = sAg + ((size_t)(ti * F * 4 + g4) * MT + l16) * 8;
+                    const T* bRow = sB + (size_t)ti * TCp * 8;
+#pragma unroll
+                    for (int tj = 0; tj < F; tj++) {
+                        V av[MB], bv[4];
+#pragma unroll
+                        for (int mb = 0; mb < MB; mb++)
+                            av[mb] = *reinterpret_cast<const V*>(aRow + ((size_t)tj * 4 * MT + mb * 16) * 8);
+#pragma unroll
+                        for (int pb = 0; pb < 4; pb++) bv[pb] = *reinterpret_cast<const V*>(bRow + boff[pb] + tj * 8);
+#pragma unroll
+                        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                            for (int pb = 0; pb < 4; pb++) acc[mb][pb] = lp_mfma<PREC>(bv[pb], av[mb], acc[mb][pb]);
+                    }
+                }
+                continue;
+            }
 #pragma unroll
             for (int ti = 0; ti < F; ti++) {
                 if (ti < tr0 || ti >= tEnd) continue;  // block-uniform

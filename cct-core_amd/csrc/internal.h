@@ -199,11 +199,11 @@ struct FaCfg {
     int rowChunks;  // ceil(FHe / RR)
     int splits;     // partial results
     int NW;         // waves per block (FP32 kernel: 4, or 8 with the tap columns split between wave pairs)
-    // Row-streaming BF16 / FP16 kernel (k_fa_rows_lp, rows = 1): NB x-plane
-    // blocks and KBW dy-plane blocks a wave; tiles of (4/nb)*kbw*16 dy planes
-    // by nb*16 x planes; 32-pixel column chunks of the dy rows; row splits of
-    // rowsPer rows; load widths (floats) of the dy and x rows.
-    int rows, nb, kbw, nTiles, kTiles, chunks, rowSplits, rowsPer, vecDy, vecX;
+    // Row-streaming BF16 / FP16 kernel (k_fa_rows_lp, rows = 1): nb x kh waves,
+    // kbw dy-plane blocks a wave; tiles of kh*kbw*16 dy planes by nb*16 x
+    // planes; 32-pixel column chunks of the dy rows; row splits of rowsPer
+    // rows; load widths (floats) of the dy and x rows.
+    int rows, nb, kh, kbw, nTiles, kTiles, chunks, rowSplits, rowsPer, vecDy, vecX;
     size_t lds_bytes;
 };
 

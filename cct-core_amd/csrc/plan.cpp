@@ -449,10 +449,10 @@ bool choose_fa_igemm_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m,
 
 // Row-streaming BF16 / FP16 FilterAdjointBlockReduceSum (k_fa_rows_lp):
 // BorderValid, unit steps, cross-correlation, a square 1x1, 3x3 or 5x5 dW.
-// nb / kbw pick the tile (0: from K and N); rowSplits (0: from the block
-// count) splits each image's dy rows so that the grid fills the CUs.
-bool choose_fa_rows_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out, int nb = 0, int kbw = 0,
-                       int rowSplits = 0) {
+// (nb, kh, kbw) pick the tile and waves (0: from K and N); rowSplits (0: from
+// the wave count) splits each image's dy rows so that the grid fills the CUs.
+bool choose_fa_rows_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, FaCfg* out, int nb = 0, int kh = 0,
+                       int kbw = 0, int rowSplits = 0) {
     if (d->mode != COGCONV_FILTER_ADJOINT_BLOCK_REDUCE_SUM || d->border != COGCONV_BORDER_VALID || d->in0.dims != 2)
         return false;
     if (d->precision != COGCONV_BF16 && d->precision != COGCONV_FP16) return false;
@@ -463,13 +463,22 @@ bool choose_fa_rows_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, 
     c.rows = 1;
     c.K = m.T / m.B;
     const int N = m.N;
-    if (!nb) nb = N <= 16 ? 1 : 2;
-    if (!kbw) kbw = (g.OH == 5 || c.K <= 32) ? 1 : 2;
-    if ((nb != 1 && nb != 2) || (kbw != 1 && kbw != 2)) return false;
-    if (g.OH == 5 && kbw == 2) return false;  // 2 x 25 accumulators would not fit the registers
+    if (!nb) {
+        // Whole 64 x 64 tiles (8 waves) read every dy and x row once; narrower
+        // tiles for fewer planes.
+        if (N > 32 && c.K > 32 && g.OH <= 3) { nb = 4; kh = 2; kbw = 2; }
+        else if (N > 16) { nb = 2; kh = 2; kbw = (g.OH == 5 || c.K <= 32) ? 1 : 2; }
+        else { nb = 1; kh = 4; kbw = 1; }
+    }
+    // The tiles the launcher instantiates; 5x5 dW keeps one dy block a wave
+    // (25 accumulators).
+    const int sel = nb * 100 + kh * 10 + kbw;
+    if (sel != 222 && sel != 221 && sel != 142 && sel != 141 && sel != 422 && sel != 421 && sel != 414) return false;
+    if (g.OH == 5 && kbw != 1) return false;
     c.nb = nb;
+    c.kh = kh;
     c.kbw = kbw;
-    const int KT = (4 / nb) * kbw * 16, NT = nb * 16;
+    const int KT = kh * kbw * 16, NT = nb * 16;
     c.nTiles = (N + NT - 1) / NT;
     c.kTiles = (c.K + KT - 1) / KT;
     c.chunks = (g.FW + 31) / 32;
@@ -478,8 +487,9 @@ bool choose_fa_rows_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, 
     if ((long)(c.K + KT) * g.FH * g.FW * 4 >= (1L << 31) || (long)(N + NT) * g.H * g.W * 4 >= (1L << 31)) return false;
     const long perRows = (long)c.nTiles * c.kTiles * m.B * c.chunks;  // blocks without row splits
     if (!rowSplits) {
+        // About two waves a SIMD over the 256 CUs.
         rowSplits = 1;
-        while (perRows * rowSplits < 512 && rowSplits * 2 <= g.FH && rowSplits < 64) rowSplits *= 2;
+        while (perRows * rowSplits * nb * kh < 2048 && rowSplits * 2 <= g.FH && rowSplits < 64) rowSplits *= 2;
     }
     if (rowSplits < 1 || rowSplits > g.FH) return false;
     if (perRows * rowSplits > 0x7fffffffL || (long)m.B * c.chunks * rowSplits > 0x7fffffffL) return false;
@@ -728,14 +738,14 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         const bool rows = choose_fa_rows_lp(desc, g, m, &frl);
         if (rows) {
             add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &frl);
-            for (int nb : {2, 1})
-                for (int kbw : {2, 1})
-                    for (int rsp : {1, 2, 4, 8}) {
-                        FaCfg c2;
-                        if (choose_fa_rows_lp(desc, g, m, &c2, nb, kbw, rsp) &&
-                            (c2.nb != frl.nb || c2.kbw != frl.kbw || c2.rowSplits != frl.rowSplits))
-                            add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &c2);
-                    }
+            static const int tiles[][3] = {{4, 2, 2}, {4, 1, 4}, {4, 2, 1}, {2, 2, 2}, {2, 2, 1}, {1, 4, 2}, {1, 4, 1}};
+            for (const auto& t : tiles)
+                for (int rsp : {1, 2, 4, 8}) {
+                    FaCfg c2;
+                    if (choose_fa_rows_lp(desc, g, m, &c2, t[0], t[1], t[2], rsp) &&
+                        (c2.nb != frl.nb || c2.kh != frl.kh || c2.kbw != frl.kbw || c2.rowSplits != frl.rowSplits))
+                        add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &c2);
+                }
         }
         if (choose_fa_igemm_lp(desc, g, m, &flp)) {
             add(KK_FA_IGEMM_LP, nullptr, nullptr, nullptr, &flp);
@@ -1053,9 +1063,9 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
         static const char* pn[] = {"fp32", "bf16", "fp16", "split_bf16"};
         if (plan->fcfg.rows) {
             const FaCfg& f = plan->fcfg;
-            snprintf(buf, len, "fa_rows_%s %s out=%dx%dx%d tile=%dx%d chunks=%d rowsplits=%d splits=%d ld=%d/%d lds=%zu",
+            snprintf(buf, len, "fa_rows_%s %s out=%dx%dx%d tile=%dx%d waves=%d chunks=%d rowsplits=%d splits=%d ld=%d/%d lds=%zu",
                      pn[plan->desc.precision], mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW,
-                     (4 / f.nb) * f.kbw * 16, f.nb * 16, f.chunks, f.rowSplits, f.splits, f.vecDy, f.vecX,
+                     f.kh * f.kbw * 16, f.nb * 16, f.nb * f.kh, f.chunks, f.rowSplits, f.splits, f.vecDy, f.vecX,
                      f.lds_bytes);
             break;
         }

@@ -19,9 +19,10 @@
 //     resident in HBM at unrelated addresses run as one request-table launch
 //     (each image's blocks read their buffers from a table the host builds in
 //     pinned memory and copies ahead of the kernel on the launch's stream);
-//   - captures the launch sequence of a hot (plan, bank, stream) once into a
-//     hipGraph and replays it (the table copy stays outside the graph, so a
-//     replay runs on that launch's own table);
+//   - captures the kernel of a hot (plan, bank, stream) request-table launch
+//     once into a hipGraph and replays it (the table copy stays outside the
+//     graph, so a replay runs on that launch's own table); a request alone runs
+//     its plain launch;
 //   - spreads launches over several streams: copies and kernels of different
 //     launches overlap on the device;
 //   - records submit -> result-ready latency per request (result in host memory,
@@ -347,9 +348,13 @@ struct cogconv_server_s {
                 blocks.push_back({o_ws, ws});
             }
             void* wsp = pk != SIZE_MAX ? arena.base + pk : (ws ? arena.base + o_ws : nullptr);
-            // Requests of a table-capable plan on a resident bank run as a
-            // request-table launch (one image per request), which a graph can replay.
-            const bool table = h->r.bank >= 0 && plan_table_capable(plan) && !free_slots.empty();
+            // A stack of requests of a table-capable plan on a resident bank runs
+            // as one request-table launch (one image per request), which a graph
+            // replays once the (plan, bank, stream) is hot.  A request alone runs
+            // the plain (packed) launch: the table's host-to-device copy would
+            // cost more than it saves (c5 capacity 54 k req/s with tables for
+            // single requests, in FP32 and BF16 alike).
+            const bool table = g.size() > 1 && h->r.bank >= 0 && plan_table_capable(plan) && !free_slots.empty();
             if (g.size() > 1 && !table) {  // a stack needs a table: the caller issues the head alone
                 for (auto& bk : blocks) arena.release(bk.first, bk.second);
                 return false;
