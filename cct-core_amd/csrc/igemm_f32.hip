@@ -28,6 +28,73 @@ namespace cogconv {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// Stages input planes r0 .. r0 + RC - 1 of a tile of TRp rows from stored row
+// y0 (border applied) into sB[plane][row][col] (plane stride planeStride, row
+// stride TCp): one (plane, row) item per wave pass.  The row of an item is the
+// same for all lanes of a wave (scalar work); the lanes' column byte offsets
+// vc0 / vc1 (STAGE_OOB: no column or off the field) were computed once per
+// block.  Per batch of IU items: offsets and masks first, then all loads
+// (unconditional, from in-bounds offsets, no branch between them), then the
+// masked LDS stores; items past the end and lanes past the tile width store
+// into a scratch slot instead of branching.
+template <int IU>
+__device__ __forceinline__ void stage_input_f32(const Geom& g, const PlaneMap& m, __amdgpu_buffer_rsrc_t xr, int r0,
+                                                int RC, int TRp, int TCp, int planeStride, int y0, int HW, int vc0,
+                                                int vc1, bool twoPass, float* sB, float* sJunk, int wave, int lane) {
+    const int nrows = RC * TRp;
+    for (int base = wave; base < nrows; base += 8 * IU) {
+        // Per-lane row table of the batch (lane u = the wave's u-th item):
+        // byte offset (border applied), LDS destination (-1: not live) and
+        // in-field mask, read back per row with readlane / a ballot bit.
+        int rSoff, rDst;
+        uint64_t rMask;
+        {
+            const int item = base + 8 * lane;
+            const int rc = item / TRp, ty = item - rc * TRp;
+            const int r = r0 + rc;
+            int y = y0 + ty;
+            bool in = (unsigned)y < (unsigned)g.H;
+            if (g.border == COGCONV_BORDER_CYCLIC) {
+                y %= g.H;
+                y += y < 0 ? g.H : 0;
+                in = true;
+            } else {
+                in = in || g.border == COGCONV_BORDER_CLAMP;
+                y = min(max(y, 0), g.H - 1);
+            }
+            const bool live = lane < IU && item < nrows;
+            rSoff = 4 * (min(r, m.R - 1) * HW + y * g.W);
+            rDst = live ? rc * planeStride + ty * TCp : -1;
+            rMask = __ballot(live && in && r < m.R);
+        }
+        auto xrow = [&](int u, int& soff, bool& on) {
+            soff = __builtin_amdgcn_readlane(rSoff, u);
+            on = (rMask >> u) & 1;
+        };
+        auto xdst = [&](int u) { return __builtin_amdgcn_readlane(rDst, u); };
+        if (twoPass)
+            stage_rows_fn<true, IU>(xr, xrow, xdst, sB, sJunk, lane, TCp, 1, vc0, vc1);
+        else
+            stage_rows_fn<false, IU>(xr, xrow, xdst, sB, sJunk, lane, TCp, 1, vc0, vc1);
+    }
+}
+
+// Per-lane column of a staged tile starting at stored column x0 + 64 p, with
+// the border applied: the byte offset of the column, or STAGE_OOB where the
+// lane's column is off the field (BorderZero and the like).
+__device__ __forceinline__ int stage_col_offset(const Geom& g, int x) {
+    bool in = (unsigned)x < (unsigned)g.W;
+    if (g.border == COGCONV_BORDER_CYCLIC) {
+        x %= g.W;
+        x += x < 0 ? g.W : 0;
+        in = true;
+    } else {
+        in = in || g.border == COGCONV_BORDER_CLAMP;
+        x = min(max(x, 0), g.W - 1);
+    }
+    return in ? 4 * x : STAGE_OOB;
+}
+
 template <int FR, int FC, int MB, bool PH>
 __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg c, Phase ph, const float* __restrict__ X,
                                                    const float* __restrict__ Wt, float* __restrict__ O,
@@ -74,29 +141,7 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
     // Per-lane column offsets of the staged tile (two 64-lane passes at most)
     // with the border applied: the same for every staged row and plane.
     const bool twoPass = TCp > 64;
-    int colOff0, colOff1;
-    bool colIn0, colIn1;
-    {
-        int xs[2];
-        bool ins[2];
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            int x = x0 + lane + 64 * p;
-            bool in = (unsigned)x < (unsigned)g.W;
-            if (g.border == COGCONV_BORDER_CYCLIC) {
-                x %= g.W;
-                x += x < 0 ? g.W : 0;
-                in = true;
-            } else {
-                in = in || g.border == COGCONV_BORDER_CLAMP;
-                x = min(max(x, 0), g.W - 1);
-            }
-            xs[p] = x;
-            ins[p] = in;
-        }
-        colOff0 = xs[0]; colOff1 = xs[1];
-        colIn0 = ins[0]; colIn1 = ins[1];
-    }
+    const int vc0 = stage_col_offset(g, x0 + lane), vc1 = stage_col_offset(g, x0 + lane + 64);
     const int half = c.RC >> 1;
 
     floatx4 acc[MB][4];
@@ -115,55 +160,13 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
 
     for (int r0 = 0; r0 < m.R; r0 += c.RC) {
         __syncthreads();
-        // Input tile: one (plane, row) item per wave pass.  The row of an item
-        // is the same for all lanes of a wave (scalar work); the lanes' column
-        // offsets and border masks were computed once per block.  Per batch of
-        // IU items: offsets and masks first, then all loads (unconditional,
-        // from in-bounds offsets, no branch between them), then the masked LDS
-        // stores; items past the end and lanes past the tile width store into
-        // a scratch slot instead of branching.
-        const int nrows = c.RC * TRp;
         // Rows a wave stages per batch: 20 (one batch for 16 planes of a
         // 3x3 tile's 10 rows) where the registers allow two blocks per CU;
         // 12 for four row blocks and for the up-sampled phase kernels (2x
         // 7x7 Zero: 700 us at 12, 727 us at 20).
         constexpr int IU = (MB == 4 || PH) ? 12 : 20;
-        const int vc0 = colIn0 ? 4 * colOff0 : STAGE_OOB, vc1 = colIn1 ? 4 * colOff1 : STAGE_OOB;
-        for (int base = wave; base < nrows; base += 8 * IU) {
-            // Per-lane row table of the batch (lane u = the wave's u-th item):
-            // byte offset (border applied), LDS destination (-1: not live) and
-            // in-field mask, read back per row with readlane / a ballot bit.
-            int rSoff, rDst;
-            uint64_t rMask;
-            {
-                const int item = base + 8 * lane;
-                const int rc = item / TRp, ty = item - rc * TRp;
-                const int r = r0 + rc;
-                int y = y0 + ty;
-                bool in = (unsigned)y < (unsigned)g.H;
-                if (g.border == COGCONV_BORDER_CYCLIC) {
-                    y %= g.H;
-                    y += y < 0 ? g.H : 0;
-                    in = true;
-                } else {
-                    in = in || g.border == COGCONV_BORDER_CLAMP;
-                    y = min(max(y, 0), g.H - 1);
-                }
-                const bool live = lane < IU && item < nrows;
-                rSoff = 4 * (min(r, m.R - 1) * HW + y * g.W);
-                rDst = live ? rc * c.planeStride + ty * TCp : -1;
-                rMask = __ballot(live && in && r < m.R);
-            }
-            auto xrow = [&](int u, int& soff, bool& on) {
-                soff = __builtin_amdgcn_readlane(rSoff, u);
-                on = (rMask >> u) & 1;
-            };
-            auto xdst = [&](int u) { return __builtin_amdgcn_readlane(rDst, u); };
-            if (twoPass)
-                stage_rows_fn<true, IU>(xr, xrow, xdst, sB, sJunk, lane, TCp, 1, vc0, vc1);
-            else
-                stage_rows_fn<false, IU>(xr, xrow, xdst, sB, sJunk, lane, TCp, 1, vc0, vc1);
-        }
+        stage_input_f32<IU>(g, m, xr, r0, c.RC, TRp, TCp, c.planeStride, y0, HW, vc0, vc1, twoPass, sB, sJunk, wave,
+                            lane);
         // Filter slice: thread -> (plane rc, filter mm); the F*F taps are
         // contiguous in W.  Loads are unconditional from a clamped pair.
         for (int q = tid; q < c.RC * MT; q += 512) {
@@ -397,6 +400,227 @@ hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c,
     ph.OHf = g.OH;
     ph.OWf = g.OW;
     return launch_igemm_f32_phase(g, m, c, ph, x, w, out, s, e, tab);
+}
+
+// ---------------------------------------------------------------------------
+// 2x up-sampled ProjectFrame / BackProjectFrame with block reduction, all four
+// output phases in one launch.
+//
+// The zero-inserted input makes output pixel (2R + pr, 2C + pc) a dense
+// convolution of the stored samples with the sub-filter of taps
+// (i0(pr) + 2a, j0(pc) + 2b), i0(p) = (h - p) mod 2 for a low halo h
+// (ConvolveHyperKernel.scala:235-317 reads the same zero-inserted input).  A
+// block stages one window of stored samples, TRp x TCp = (8 + UR - 1) x
+// (32 + UC - 1) for a tile of 8 x 32 phase pixels, once per plane chunk,
+// and every phase reads its taps from that window: no phase re-stages the
+// input, and the whole filter is staged once, its taps ordered phase by phase
+// (phase (pr, pc) at tap base T(pr, pc), row-major within).
+//
+// Waves: w % 4 owns phase-pixel rows 2(w%4), 2(w%4)+1 (as in k_igemm_f32); w / 4
+// is the row phase pr, and a wave sums every staged plane for both column
+// phases, so a lane holds output columns 2C and 2C + 1 of its pixels and its
+// stores are 8 adjacent output columns (two 16-byte stores per row) without
+// an exchange through LDS.
+template <int F, int HP, int PR, int MB>
+__device__ __forceinline__ void up2q_chunk(const float* sB, const float* sA, int RC, int planeStride, int aStride,
+                                           int TCp, int dr, int dc0, int dc1, const int (&boff)[4], int g4, int l16,
+                                           floatx4 (&acc)[2][MB][4]) {
+    constexpr int MT = 16 * MB;
+    constexpr int FRp = up2_taps(F, HP, PR), FC0 = up2_taps(F, HP, 0), FC1 = up2_taps(F, HP, 1);
+    constexpr int T0 = PR ? up2_taps(F, HP, 0) * F : 0;  // taps of row phase 0 (both column phases)
+    constexpr int T1 = T0 + FRp * FC0;
+    for (int p = 0; p < RC; p += 4) {
+        const float* bb = sB + (p + g4) * planeStride + dr * TCp;
+        const float* ab = sA + (size_t)(p + g4) * aStride + l16;
+#pragma unroll
+        for (int a = 0; a < FRp; a++) {
+#pragma unroll
+            for (int pc = 0; pc < 2; pc++) {
+                const int FCp = pc ? FC1 : FC0;
+                const int tb = (pc ? T1 : T0) + a * FCp;
+                const float* bq = bb + a * TCp + (pc ? dc1 : dc0);
+#pragma unroll
+                for (int bc = 0; bc < (pc ? FC1 : FC0); bc++) {
+                    float av[MB];
+#pragma unroll
+                    for (int mb = 0; mb < MB; mb++) av[mb] = ab[(tb + bc) * MT + mb * 16];
+                    float bv[4];
+#pragma unroll
+                    for (int pb = 0; pb < 4; pb++) bv[pb] = bq[boff[pb] + bc];
+#pragma unroll
+                    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                        for (int pb = 0; pb < 4; pb++)
+                            acc[pc][mb][pb] = __builtin_amdgcn_mfma_f32_16x16x4f32(bv[pb], av[mb], acc[pc][mb][pb], 0, 0, 0);
+                }
+            }
+        }
+    }
+}
+
+template <int F, int HP, int MB>
+__global__ __launch_bounds__(512) void k_igemm_f32_up2q(Geom g, PlaneMap m, UpFused u, const float* __restrict__ X,
+                                                        const float* __restrict__ Wt, float* __restrict__ O, Epi e) {
+    constexpr int TAPS = F * F;
+    constexpr int MT = 16 * MB;
+    const IgemmCfg& c = u.c;
+    extern __shared__ float smem[];
+    float* sB = smem;
+    float* sA = smem + (size_t)c.RC * c.planeStride;
+    const int aStride = c.aStride;
+    float* sJunk = sA + (size_t)c.RC * aStride;
+
+    int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int tc = bid % c.tilesC;
+    bid /= c.tilesC;
+    const int tr = bid % c.tilesR;
+    bid /= c.tilesR;
+    const int mt = bid % c.mTiles;
+    const int b = bid / c.mTiles;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int rw = wave & 3, pr = wave >> 2;
+    const int g4 = lane >> 4, l16 = lane & 15;
+    const int Mout = c.Mout;
+    const int m0 = mt * MT;
+    const int y0 = tr * 8 - u.hU;
+    const int x0 = tc * 32 - u.wU;
+    const int TRp = c.tileRows, TCp = c.tileCols;
+    const bool bpf = m.mode == COGCONV_BACK_PROJECT_FRAME_BLOCK_REDUCE_SUM;
+    const int L = m.T / m.N;
+    const int HW = g.H * g.W;
+    const float* Xb = X + (size_t)b * m.N * HW;
+    const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(Xb, (uint32_t)m.N * HW * 4u);  // the plan bounds this by 2^31
+    const int vc0 = stage_col_offset(g, x0 + lane);  // the window is at most 64 columns wide
+
+    floatx4 acc[2][MB][4];
+#pragma unroll
+    for (int pc = 0; pc < 2; pc++)
+#pragma unroll
+        for (int i = 0; i < MB; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[pc][i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    int boff[4];
+#pragma unroll
+    for (int pb = 0; pb < 4; pb++) boff[pb] = (2 * rw + (pb >> 1)) * TCp + (pb & 1) * 16 + l16;
+
+    for (int r0 = 0; r0 < m.R; r0 += c.RC) {
+        __syncthreads();
+        stage_input_f32<12>(g, m, xr, r0, c.RC, TRp, TCp, c.planeStride, y0, HW, vc0, STAGE_OOB, false, sB, sJunk,
+                            wave, lane);
+        // The whole filter of each (plane, filter) pair, its taps reordered
+        // phase by phase: tap (i, j) of the filter in the kernel's orientation
+        // (flipped for the convolution orientation) belongs to row phase
+        // HP ^ (i & 1) at sub-row i / 2 and column phase HP ^ (j & 1) at
+        // sub-column j / 2.
+        for (int q = tid; q < c.RC * MT; q += 512) {
+            const int mm = q % MT, rc = q / MT;
+            const int r = r0 + rc, mg = m0 + mm;
+            const bool ok = r < m.R && mg < Mout;
+            const int rr = min(r, m.R - 1), mgc = min(mg, Mout - 1);
+            const float* src = Wt + (size_t)(bpf ? rr * L + mgc : mgc * m.N + rr) * TAPS;
+            float t[TAPS];
+#pragma unroll
+            for (int i = 0; i < TAPS; i++) t[i] = src[i];
+            float* dst = sA + (size_t)rc * aStride + mm;
+#pragma unroll
+            for (int i = 0; i < TAPS; i++) {
+                const int ti = i / F, tj = i % F;
+                const int rp = HP ^ (ti & 1), cp = HP ^ (tj & 1);
+                const int base = (rp ? up2_taps(F, HP, 0) * F : 0) + (cp ? up2_taps(F, HP, rp) * up2_taps(F, HP, 0) : 0);
+                const int pos = base + (ti >> 1) * up2_taps(F, HP, cp) + (tj >> 1);
+                const float v = g.flip ? t[TAPS - 1 - i] : t[i];
+                dst[pos * MT] = ok ? v : 0.f;
+            }
+        }
+        __syncthreads();
+        if (pr == 0)
+            up2q_chunk<F, HP, 0, MB>(sB, sA, c.RC, c.planeStride, aStride, TCp, u.dr[0], u.dc[0], u.dc[1], boff, g4, l16,
+                                     acc);
+        else
+            up2q_chunk<F, HP, 1, MB>(sB, sA, c.RC, c.planeStride, aStride, TCp, u.dr[1], u.dc[0], u.dc[1], boff, g4, l16,
+                                     acc);
+    }
+    // Epilogue: lane (g4, l16) of block (mb, pb) holds output plane
+    // m0 + 16 mb + l16 at phase pixels C = 4 g4 .. 4 g4 + 3 of its 16-pixel run,
+    // column phase 0 in acc[0] and 1 in acc[1]: output columns 2C .. 2C + 7 of
+    // output row 2R + pr.  Two 16-byte stores where the row is 16-byte aligned
+    // and the eight columns are inside it; single stores otherwise.
+    const bool vec = (g.OW & 3) == 0 && (reinterpret_cast<uintptr_t>(O) & 15) == 0;
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) {
+        const int mm = m0 + mb * 16 + l16;
+#pragma unroll
+        for (int pb = 0; pb < 4; pb++) {
+            const int R = tr * 8 + 2 * rw + (pb >> 1);
+            const int C = tc * 32 + (pb & 1) * 16 + 4 * g4;
+            if (mm >= Mout || R >= u.OHp[pr] || C >= u.OWp[0]) continue;
+            const int orow = 2 * R + pr, ocol = 2 * C;
+            float* dst = O + ((size_t)(b * Mout + mm) * g.OH + orow) * g.OW + ocol;
+            const floatx4 s0 = acc[0][mb][pb], s1 = acc[1][mb][pb];
+            if (vec && ocol + 8 <= g.OW) {
+                *reinterpret_cast<float4*>(dst) = make_float4(s0[0], s1[0], s0[1], s1[1]);
+                *reinterpret_cast<float4*>(dst + 4) = make_float4(s0[2], s1[2], s0[3], s1[3]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 8; v++)
+                    if (ocol + v < g.OW) dst[v] = (v & 1) ? s1[v >> 1] : s0[v >> 1];
+            }
+        }
+    }
+    // The epilogue rewrites the thread's own stores (still in L2), so the
+    // accumulators above stay in registers whatever its code size.
+    if (e.n) {
+        for (int t = 0; t < MB * 4; t++) {
+            const int mb = t >> 2, pb = t & 3;
+            const int mm = m0 + mb * 16 + l16;
+            const int R = tr * 8 + 2 * rw + (pb >> 1);
+            const int C = tc * 32 + (pb & 1) * 16 + 4 * g4;
+            if (mm >= Mout || R >= u.OHp[pr] || C >= u.OWp[0]) continue;
+            const int orow = 2 * R + pr, ocol = 2 * C;
+            float* dst = O + ((size_t)(b * Mout + mm) * g.OH + orow) * g.OW + ocol;
+            const int pt = orow * g.OW + ocol;
+            if (vec && ocol + 8 <= g.OW) {
+                epi_rewrite4(e, dst, b * Mout + mm, pt);
+                epi_rewrite4(e, dst + 4, b * Mout + mm, pt + 4);
+            } else {
+                for (int v = 0; v < 8 && ocol + v < g.OW; v++) epi_rewrite1(e, dst + v, b * Mout + mm, pt + v);
+            }
+        }
+    }
+}
+
+bool igemm_f32_up2q_supported(int F) { return F == 3 || F == 5 || F == 7 || F == 9; }
+
+template <int F, int HP>
+static hipError_t launch_up2q_mb(const Geom& g, const PlaneMap& m, const UpFused& u, const float* x, const float* w,
+                                 float* out, hipStream_t s, dim3 grid, const Epi& e) {
+    if (u.c.MB == 1)
+        hipLaunchKernelGGL((k_igemm_f32_up2q<F, HP, 1>), grid, dim3(512), u.c.lds_bytes, s, g, m, u, x, w, out, e);
+    else
+        hipLaunchKernelGGL((k_igemm_f32_up2q<F, HP, 2>), grid, dim3(512), u.c.lds_bytes, s, g, m, u, x, w, out, e);
+    return hipGetLastError();
+}
+
+hipError_t launch_igemm_f32_up2q(const Geom& g, const PlaneMap& m, const UpFused& u, const float* x, const float* w,
+                                 float* out, hipStream_t s, const Epi& e) {
+    const long blocks = (long)u.c.tilesC * u.c.tilesR * u.c.mTiles * m.B;
+    if (blocks <= 0 || blocks > 0x7fffffffL || u.c.tileCols > 64 || (u.c.MB != 1 && u.c.MB != 2))
+        return hipErrorInvalidConfiguration;
+    const dim3 grid((unsigned)blocks);
+    switch (g.FH * 2 + (u.hp & 1)) {
+    case 6: return launch_up2q_mb<3, 0>(g, m, u, x, w, out, s, grid, e);
+    case 7: return launch_up2q_mb<3, 1>(g, m, u, x, w, out, s, grid, e);
+    case 10: return launch_up2q_mb<5, 0>(g, m, u, x, w, out, s, grid, e);
+    case 11: return launch_up2q_mb<5, 1>(g, m, u, x, w, out, s, grid, e);
+    case 14: return launch_up2q_mb<7, 0>(g, m, u, x, w, out, s, grid, e);
+    case 15: return launch_up2q_mb<7, 1>(g, m, u, x, w, out, s, grid, e);
+    case 18: return launch_up2q_mb<9, 0>(g, m, u, x, w, out, s, grid, e);
+    case 19: return launch_up2q_mb<9, 1>(g, m, u, x, w, out, s, grid, e);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace cogconv

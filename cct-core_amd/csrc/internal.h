@@ -220,12 +220,28 @@ struct Phase {
     int OHf, OWf;
 };
 
-// The phase launches of an up-sampled frame-mode plan (2x: four phases).
+// The four phases of 2x up-sampling in one launch (k_igemm_f32_up2q): a block
+// stages one window of stored samples and computes all four phases of a tile
+// of 8 x 32 phase pixels.  Row phase pr's sub-filter rows start dr[pr] rows
+// into the window (hU: the window's low halo in stored rows), column phase
+// pc's dc[pc] columns in; phase (pr, pc) covers OHp[pr] x OWp[pc] pixels.
+struct UpFused {
+    int on;
+    int hp;        // parity of the full filter's low halo (selects the tap split)
+    IgemmCfg c;    // tilesR / tilesC in phase pixels; tileRows / tileCols: the window
+    int hU, wU;
+    int dr[2], dc[2];
+    int OHp[2], OWp[2];
+};
+
+// The phase launches of an up-sampled frame-mode plan (2x: four phases), or
+// one fused launch of all four.
 struct PhaseSet {
     int n;
     Geom g[4];
     IgemmCfg c[4];
     Phase p[4];
+    UpFused f;
 };
 
 enum IgemmVariant {
@@ -256,6 +272,12 @@ size_t igemm_f32_exchange_floats(int MB);
 bool igemm_f32_supported_rc(int FR, int FC);
 // `tab` (device memory, one entry per image of the plan's batch): each image's
 // operand-1 planes and output planes come from its entry instead of x / out.
+// Taps of row (column) phase p of a 2x up-sampled F-tap filter whose low halo
+// has parity hp: the taps i with i = (hp ^ p) mod 2.
+constexpr int up2_taps(int F, int hp, int p) { return (F - ((hp ^ p) & 1) + 1) / 2; }
+bool igemm_f32_up2q_supported(int F);
+hipError_t launch_igemm_f32_up2q(const Geom& g, const PlaneMap& m, const UpFused& u, const float* x, const float* w,
+                                 float* out, hipStream_t s, const Epi& e);
 hipError_t launch_igemm_f32_phase(const Geom& g, const PlaneMap& m, const IgemmCfg& c, const Phase& ph, const float* x,
                                   const float* w, float* out, hipStream_t s, const Epi& e = kNoEpi,
                                   const cogconv_image_io* tab = nullptr);

@@ -675,6 +675,83 @@ bool choose_igemm_up(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ph
     return true;
 }
 
+// All four phases of a 2x up-sampled plan in one launch (k_igemm_f32_up2q):
+// the window of stored samples a tile of 8 x 32 phase pixels needs for every
+// phase, and the output block (MB) and plane chunk (RC) whose staging (the
+// window and the whole filter) fits in `limit` bytes of LDS.
+bool choose_igemm_up2q(const Geom& g, const PlaneMap& m, const PhaseSet& ps, UpFused* out, int mbOnly = 0,
+                       int rcOnly = 0, size_t limit = 80 * 1024) {
+    if (ps.n != 4 || g.FH != g.FW || !igemm_f32_up2q_supported(g.FH) || (g.hR & 1) != (g.hC & 1)) return false;
+    UpFused u;
+    std::memset(&u, 0, sizeof u);
+    u.hp = g.hR & 1;
+    const int F = g.FH;
+    int hr[2], hc[2];
+    for (int p = 0; p < 2; p++) {
+        hr[p] = ps.g[2 * p].hR;
+        hc[p] = ps.g[p].hC;
+        u.OHp[p] = ps.g[2 * p].OH;
+        u.OWp[p] = ps.g[p].OW;
+    }
+    for (int pr = 0; pr < 2; pr++)
+        for (int pc = 0; pc < 2; pc++) {
+            const Geom& q = ps.g[2 * pr + pc];
+            if (q.FH != up2_taps(F, u.hp, pr) || q.FW != up2_taps(F, u.hp, pc) || q.hR != hr[pr] || q.hC != hc[pc])
+                return false;
+        }
+    u.hU = std::max(hr[0], hr[1]);
+    u.wU = std::max(hc[0], hc[1]);
+    int UR = 0, UC = 0;
+    for (int p = 0; p < 2; p++) {
+        u.dr[p] = u.hU - hr[p];
+        u.dc[p] = u.wU - hc[p];
+        UR = std::max(UR, u.dr[p] + up2_taps(F, u.hp, p));
+        UC = std::max(UC, u.dc[p] + up2_taps(F, u.hp, p));
+    }
+    IgemmCfg& c = u.c;
+    c.variant = IV_F32_FRAME;
+    c.Mout = m.T / m.N;
+    c.tilesR = (std::max(u.OHp[0], u.OHp[1]) + 7) / 8;
+    c.tilesC = (std::max(u.OWp[0], u.OWp[1]) + 31) / 32;
+    c.tileRows = 7 + UR;
+    c.tileCols = 31 + UC;
+    if (c.tileCols > 64) return false;
+    int pstr = c.tileRows * c.tileCols;
+    pstr += ((16 - pstr % 32) + 32) % 32;  // 16 mod 32 words: the four k-lane groups hit different banks
+    c.planeStride = pstr;
+    const int maxRC = std::min(32, ((m.R + 7) / 8) * 8);
+    for (int MB = c.Mout <= 16 ? 1 : 2; MB >= 1; MB--) {
+        if (mbOnly && MB != mbOnly) continue;
+        int as = F * F * 16 * MB;
+        as += ((16 - as % 32) + 32) % 32;
+        int best = 0;
+        long bestSlots = 0;
+        size_t bestBytes = 0;
+        for (int rc = maxRC; rc >= 4; rc -= 4) {
+            if (rcOnly && rc != rcOnly) continue;
+            const size_t bytes = ((size_t)rc * (pstr + as) + 128) * sizeof(float);
+            if (bytes > limit) continue;
+            const long slots = (long)((m.R + rc - 1) / rc) * rc;
+            if (!best || slots < bestSlots) {
+                best = rc;
+                bestSlots = slots;
+                bestBytes = bytes;
+            }
+        }
+        if (best) {
+            c.MB = MB;
+            c.RC = best;
+            c.aStride = as;
+            c.lds_bytes = bestBytes;
+            c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
+            u.on = 1;
+            *out = u;
+            return true;
+        }
+    }
+    return false;
+}
+
 void describe_into(const cogconv_plan_s* plan, char* buf, size_t len);
 
 // The candidate configurations for `desc`, the heuristic choice first.  The
@@ -803,7 +880,11 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
     // the autotuner times both routes.
     const bool fft_first = fft_ok && !rt_ok && (long)g.FH * g.FW >= 31L * 31;
     // Heuristic default.
+    // The fused up-sampled launch first: all four phases from one staged window.
+    PhaseSet upq = up;
+    const bool upq_ok = up_ok && choose_igemm_up2q(g, m, up, &upq.f);
     if (fft_first) add_fft(fc);
+    else if (upq_ok) add_up(upq);
     else if (up_ok) add_up(up);
     else if (igemm_ok) add(KK_IGEMM, nullptr, nullptr, &ig, nullptr);
     else if (fa_ok) add(KK_FA_IGEMM, nullptr, nullptr, nullptr, &fa);
@@ -839,6 +920,22 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
     for (int rg : {4, 2, 1}) {
         const DirectCfg c = choose_direct(g, rg);
         add(KK_DIRECT, &c, nullptr, nullptr, nullptr);
+    }
+    if (upq_ok) {
+        add_up(up);
+        for (size_t limit : {(size_t)80 * 1024, (size_t)160 * 1024})
+            for (int mb : {2, 1})
+                for (int rc : {32, 16, 8}) {
+                    PhaseSet q = up;
+                    if (choose_igemm_up2q(g, m, up, &q.f, mb, rc, limit) &&
+                        (q.f.c.MB != upq.f.c.MB || q.f.c.RC != upq.f.c.RC)) {
+                        bool dup = false;
+                        for (const Variant& v : out)
+                            dup = dup || (v.kind == KK_IGEMM_UP && v.ps.f.on && v.ps.f.c.MB == q.f.c.MB &&
+                                          v.ps.f.c.RC == q.f.c.RC);
+                        if (!dup) add_up(q);
+                    }
+                }
     }
     if (fft_ok && !fft_first) add_fft(fc);
 }
@@ -1052,6 +1149,13 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
         break;
     }
     case KK_IGEMM_UP: {
+        if (plan->ps.f.on) {
+            const IgemmCfg& c = plan->ps.f.c;
+            snprintf(buf, len, "igemm_f32_up2 %s out=%dx%dx%d phases=4 fused window=%dx%d MB=%d RC=%d lds=%zu",
+                     mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, c.tileRows, c.tileCols, c.MB, c.RC,
+                     c.lds_bytes);
+            break;
+        }
         int n = snprintf(buf, len, "igemm_f32_up2 %s out=%dx%dx%d phases=%d", mode_name(plan->desc.mode), plan->m.nOut,
                          g.OH, g.OW, plan->ps.n);
         for (int i = 0; i < plan->ps.n && n > 0 && (size_t)n < len; i++)
@@ -1150,6 +1254,10 @@ int launch_all(cogconv_plan plan, const float* in0, const float* in1, float* out
                             LP_PACK_AND_RUN, ep);
         break;
     case KK_IGEMM_UP:
+        if (plan->ps.f.on) {
+            e = launch_igemm_f32_up2q(plan->g, plan->m, plan->ps.f, in0, in1, out, s, ep);
+            break;
+        }
         for (int i = 0; i < plan->ps.n && e == hipSuccess; i++)
             e = launch_igemm_f32_phase(plan->ps.g[i], plan->m, plan->ps.c[i], plan->ps.p[i], in0, in1, out, s, ep);
         break;

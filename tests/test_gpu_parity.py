@@ -365,6 +365,58 @@ def test_upsampled_implicit_gemm_gpu(mode, border, F):
     assert_parity(got, ref, mag, what="up2 %s b%d F%d" % (mode, border, F))
 
 
+# (mode, border, F, N, K, H, W): output widths 44 (a last 8-column group past
+# the row end), 2W + F - 1 odd (BorderFull), output planes 20 and 40 (two
+# output blocks), an input narrower than one 32-column tile.
+UPV_CASES = [("pf_brs", O.ZERO, 7, 12, 20, 13, 22), ("bpf_brs", O.FULL, 5, 6, 40, 9, 17),
+             ("pf_brs", O.CYCLIC, 9, 8, 16, 11, 37), ("pf_brs", O.VALID, 3, 5, 3, 12, 9),
+             ("bpf_brs", O.ZERO, 3, 16, 33, 6, 70)]
+
+
+@pytest.mark.parametrize("mode,border,F,N,K,H,W", UPV_CASES)
+def test_upsampled_every_variant_gpu(mode, border, F, N, K, H, W):
+    """Every up-sampled implicit-GEMM candidate (the fused four-phase launch at
+    each output block and plane chunk, and the four phase launches) against the
+    oracle; the fused form with an epilogue equals its plain form with the same
+    ops applied afterwards, bit for bit."""
+    rng = np.random.default_rng(zlib.crc32(repr(("upv", mode, border, F, N, K)).encode()))
+    B = 2
+    if mode == "pf_brs":
+        a = rng.uniform(-1, 1, (B * N, H, W)).astype(np.float32)
+        k = rng.uniform(-1, 1, (K * N, F, F)).astype(np.float32)
+    else:  # K = output planes per image: dy carries N planes per image
+        a = rng.uniform(-1, 1, (B * N, H, W)).astype(np.float32)
+        k = rng.uniform(-1, 1, (N * K, F, F)).astype(np.float32)
+    orient = O.CONV if mode == "bpf_brs" else O.XCORR
+    od, cd = desc_pair(MODES[mode], border, orient, O.UP, 2, B, a.shape, k.shape)
+    ref, mag = oracle_ref(od, a, k)
+    seen = 0
+    for v in range(L.variant_count(cd)):
+        plan = L.Plan(cd, v)
+        desc = plan.description
+        plan.close()
+        if not desc.startswith("igemm_f32_up2"):
+            continue
+        announce("up2 variant", mode, border, F, N, K, H, W, v, desc)
+        got = run_gpu(cd, a, k, variant=v)
+        assert_parity(got, ref, mag, what="up2 %s b%d F%d v%d %s" % (mode, border, F, v, desc))
+        if "fused" not in desc:
+            continue
+        seen += 1
+        plan = L.Plan(cd, v)
+        plan.set_epilogue(L.make_epilogue([(L.EPI_MUL_CONST, 3.0, 0), (L.EPI_MAX_CONST, 0.5, 0)]))
+        ba, bk = L.DeviceBuffer.from_array(a), L.DeviceBuffer.from_array(k)
+        bo = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
+        plan.launch(ba, bk, bo, None)
+        L.synchronize()
+        fused = bo.download(plan.out_shape)
+        assert np.array_equal(fused, np.maximum(got * np.float32(3.0), np.float32(0.5)))
+        for buf in (ba, bk, bo):
+            buf.close()
+        plan.close()
+    assert seen >= 1, "no fused up-sampled launch"
+
+
 def test_c4_plane_by_plane_full_size():
     """PlaneByPlane 1024x1024 scalar field, 11x11 Gaussian-plus-noise filter, BorderClamp."""
     rng = np.random.default_rng(44)
