@@ -421,7 +421,7 @@ hipError_t launch_igemm_f32(const Geom& g, const PlaneMap& m, const IgemmCfg& c,
 // phases, so a lane holds output columns 2C and 2C + 1 of its pixels and its
 // stores are 8 adjacent output columns (two 16-byte stores per row) without
 // an exchange through LDS.
-template <int F, int HP, int PR, int MB>
+template <int F, int HP, int PR, int MB, bool TWO>
 __device__ __forceinline__ void up2q_chunk(const float* sB, const float* sA, int RC, int planeStride, int aStride,
                                            int TCp, int dr, int dc0, int dc1, const int (&boff)[4], int g4, int l16,
                                            floatx4 (&acc)[2][MB][4]) {
@@ -432,7 +432,9 @@ __device__ __forceinline__ void up2q_chunk(const float* sB, const float* sA, int
     for (int p = 0; p < RC; p += 4) {
         const float* bb = sB + (p + g4) * planeStride + dr * TCp;
         const float* ab = sA + (size_t)(p + g4) * aStride + l16;
-#pragma unroll
+        // Two blocks a CU: one sub-filter row at a time, so that the LDS loads
+        // the compiler hoists stay within 128 registers.
+#pragma unroll(TWO ? 1 : FRp)
         for (int a = 0; a < FRp; a++) {
 #pragma unroll
             for (int pc = 0; pc < 2; pc++) {
@@ -458,9 +460,9 @@ __device__ __forceinline__ void up2q_chunk(const float* sB, const float* sA, int
     }
 }
 
-template <int F, int HP, int MB>
-__global__ __launch_bounds__(512) void k_igemm_f32_up2q(Geom g, PlaneMap m, UpFused u, const float* __restrict__ X,
-                                                        const float* __restrict__ Wt, float* __restrict__ O, Epi e) {
+template <int F, int HP, int MB, bool TWO>
+__device__ __forceinline__ void up2q_body(const Geom& g, const PlaneMap& m, const UpFused& u, const float* __restrict__ X,
+                                          const float* __restrict__ Wt, float* __restrict__ O, const Epi& e) {
     constexpr int TAPS = F * F;
     constexpr int MT = 16 * MB;
     const IgemmCfg& c = u.c;
@@ -521,26 +523,32 @@ __global__ __launch_bounds__(512) void k_igemm_f32_up2q(Geom g, PlaneMap m, UpFu
             const bool ok = r < m.R && mg < Mout;
             const int rr = min(r, m.R - 1), mgc = min(mg, Mout - 1);
             const float* src = Wt + (size_t)(bpf ? rr * L + mgc : mgc * m.N + rr) * TAPS;
-            float t[TAPS];
-#pragma unroll
-            for (int i = 0; i < TAPS; i++) t[i] = src[i];
             float* dst = sA + (size_t)rc * aStride + mm;
+            // One tap row at a time: row ti of the kernel's orientation is
+            // stored row ti (row F - 1 - ti reversed when flipped).
 #pragma unroll
-            for (int i = 0; i < TAPS; i++) {
-                const int ti = i / F, tj = i % F;
-                const int rp = HP ^ (ti & 1), cp = HP ^ (tj & 1);
-                const int base = (rp ? up2_taps(F, HP, 0) * F : 0) + (cp ? up2_taps(F, HP, rp) * up2_taps(F, HP, 0) : 0);
-                const int pos = base + (ti >> 1) * up2_taps(F, HP, cp) + (tj >> 1);
-                const float v = g.flip ? t[TAPS - 1 - i] : t[i];
-                dst[pos * MT] = ok ? v : 0.f;
+            for (int ti = 0; ti < F; ti++) {
+                float t[F];
+                const float* sr = src + (g.flip ? F - 1 - ti : ti) * F;
+#pragma unroll
+                for (int j = 0; j < F; j++) t[j] = sr[j];
+                const int rp = HP ^ (ti & 1);
+#pragma unroll
+                for (int tj = 0; tj < F; tj++) {
+                    const int cp = HP ^ (tj & 1);
+                    const int pos = (rp ? up2_taps(F, HP, 0) * F : 0) +
+                                    (cp ? up2_taps(F, HP, rp) * up2_taps(F, HP, 0) : 0) + (ti >> 1) * up2_taps(F, HP, cp) +
+                                    (tj >> 1);
+                    dst[pos * MT] = ok ? (g.flip ? t[F - 1 - tj] : t[tj]) : 0.f;
+                }
             }
         }
         __syncthreads();
         if (pr == 0)
-            up2q_chunk<F, HP, 0, MB>(sB, sA, c.RC, c.planeStride, aStride, TCp, u.dr[0], u.dc[0], u.dc[1], boff, g4, l16,
+            up2q_chunk<F, HP, 0, MB, TWO>(sB, sA, c.RC, c.planeStride, aStride, TCp, u.dr[0], u.dc[0], u.dc[1], boff, g4, l16,
                                      acc);
         else
-            up2q_chunk<F, HP, 1, MB>(sB, sA, c.RC, c.planeStride, aStride, TCp, u.dr[1], u.dc[0], u.dc[1], boff, g4, l16,
+            up2q_chunk<F, HP, 1, MB, TWO>(sB, sA, c.RC, c.planeStride, aStride, TCp, u.dr[1], u.dc[0], u.dc[1], boff, g4, l16,
                                      acc);
     }
     // Epilogue: lane (g4, l16) of block (mb, pb) holds output plane
@@ -592,15 +600,35 @@ __global__ __launch_bounds__(512) void k_igemm_f32_up2q(Geom g, PlaneMap m, UpFu
     }
 }
 
+// One block per CU, or (k_igemm_f32_up2q2, one output block) registers held to
+// 128 a lane so that two blocks share a CU and one block's stores run under the
+// other's MFMAs.
+template <int F, int HP, int MB>
+__global__ __launch_bounds__(512) void k_igemm_f32_up2q(Geom g, PlaneMap m, UpFused u, const float* __restrict__ X,
+                                                        const float* __restrict__ Wt, float* __restrict__ O, Epi e) {
+    up2q_body<F, HP, MB, false>(g, m, u, X, Wt, O, e);
+}
+
+template <int F, int HP, int MB>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void k_igemm_f32_up2q2(
+    Geom g, PlaneMap m, UpFused u, const float* __restrict__ X, const float* __restrict__ Wt, float* __restrict__ O,
+    Epi e) {
+    up2q_body<F, HP, MB, true>(g, m, u, X, Wt, O, e);
+}
+
 bool igemm_f32_up2q_supported(int F) { return F == 3 || F == 5 || F == 7 || F == 9; }
 
 template <int F, int HP>
 static hipError_t launch_up2q_mb(const Geom& g, const PlaneMap& m, const UpFused& u, const float* x, const float* w,
                                  float* out, hipStream_t s, dim3 grid, const Epi& e) {
-    if (u.c.MB == 1)
+    if (u.two) {  // one output block only: two spill at 128 registers
+        if (u.c.MB != 1) return hipErrorInvalidConfiguration;
+        hipLaunchKernelGGL((k_igemm_f32_up2q2<F, HP, 1>), grid, dim3(512), u.c.lds_bytes, s, g, m, u, x, w, out, e);
+    } else if (u.c.MB == 1) {
         hipLaunchKernelGGL((k_igemm_f32_up2q<F, HP, 1>), grid, dim3(512), u.c.lds_bytes, s, g, m, u, x, w, out, e);
-    else
+    } else {
         hipLaunchKernelGGL((k_igemm_f32_up2q<F, HP, 2>), grid, dim3(512), u.c.lds_bytes, s, g, m, u, x, w, out, e);
+    }
     return hipGetLastError();
 }
 

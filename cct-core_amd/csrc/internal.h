@@ -228,6 +228,7 @@ struct Phase {
 struct UpFused {
     int on;
     int hp;        // parity of the full filter's low halo (selects the tap split)
+    int two;       // registers held for two blocks a CU (LDS at most 80 KB)
     IgemmCfg c;    // tilesR / tilesC in phase pixels; tileRows / tileCols: the window
     int hU, wU;
     int dr[2], dc[2];

@@ -680,7 +680,7 @@ bool choose_igemm_up(const cogconv_desc* d, const Geom& g, const PlaneMap& m, Ph
 // phase, and the output block (MB) and plane chunk (RC) whose staging (the
 // window and the whole filter) fits in `limit` bytes of LDS.
 bool choose_igemm_up2q(const Geom& g, const PlaneMap& m, const PhaseSet& ps, UpFused* out, int mbOnly = 0,
-                       int rcOnly = 0, size_t limit = 80 * 1024) {
+                       int rcOnly = 0, size_t limit = 80 * 1024, bool two = false) {
     if (ps.n != 4 || g.FH != g.FW || !igemm_f32_up2q_supported(g.FH) || (g.hR & 1) != (g.hC & 1)) return false;
     UpFused u;
     std::memset(&u, 0, sizeof u);
@@ -720,7 +720,8 @@ bool choose_igemm_up2q(const Geom& g, const PlaneMap& m, const PhaseSet& ps, UpF
     pstr += ((16 - pstr % 32) + 32) % 32;  // 16 mod 32 words: the four k-lane groups hit different banks
     c.planeStride = pstr;
     const int maxRC = std::min(32, ((m.R + 7) / 8) * 8);
-    for (int MB = c.Mout <= 16 ? 1 : 2; MB >= 1; MB--) {
+    if (two) limit = std::min(limit, (size_t)80 * 1024);
+    for (int MB = c.Mout <= 16 || two ? 1 : 2; MB >= 1; MB--) {
         if (mbOnly && MB != mbOnly) continue;
         int as = F * F * 16 * MB;
         as += ((16 - as % 32) + 32) % 32;
@@ -745,6 +746,7 @@ bool choose_igemm_up2q(const Geom& g, const PlaneMap& m, const PhaseSet& ps, UpF
             c.lds_bytes = bestBytes;
             c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
             u.on = 1;
+            u.two = two ? 1 : 0;
             *out = u;
             return true;
         }
@@ -880,9 +882,13 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
     // the autotuner times both routes.
     const bool fft_first = fft_ok && !rt_ok && (long)g.FH * g.FW >= 31L * 31;
     // Heuristic default.
-    // The fused up-sampled launch first: all four phases from one staged window.
+    // The fused up-sampled launch first: all four phases from one staged window,
+    // one output block and two blocks a CU (2x 7x7, 32 -> 32 planes of 256 x 256:
+    // 0.49 ms, against 0.54 ms for two output blocks at one block a CU and
+    // 0.74 ms for the four phase launches).
     PhaseSet upq = up;
-    const bool upq_ok = up_ok && choose_igemm_up2q(g, m, up, &upq.f);
+    const bool upq_ok = up_ok && (choose_igemm_up2q(g, m, up, &upq.f, 1, 0, 80 * 1024, true) ||
+                                  choose_igemm_up2q(g, m, up, &upq.f));
     if (fft_first) add_fft(fc);
     else if (upq_ok) add_up(upq);
     else if (up_ok) add_up(up);
@@ -923,19 +929,18 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
     }
     if (upq_ok) {
         add_up(up);
-        for (size_t limit : {(size_t)80 * 1024, (size_t)160 * 1024})
-            for (int mb : {2, 1})
-                for (int rc : {32, 16, 8}) {
-                    PhaseSet q = up;
-                    if (choose_igemm_up2q(g, m, up, &q.f, mb, rc, limit) &&
-                        (q.f.c.MB != upq.f.c.MB || q.f.c.RC != upq.f.c.RC)) {
+        for (int two : {0, 1})
+            for (size_t limit : {(size_t)80 * 1024, (size_t)160 * 1024})
+                for (int mb : {2, 1})
+                    for (int rc : {32, 16, 8}) {
+                        PhaseSet q = up;
+                        if (!choose_igemm_up2q(g, m, up, &q.f, mb, rc, limit, two)) continue;
                         bool dup = false;
                         for (const Variant& v : out)
                             dup = dup || (v.kind == KK_IGEMM_UP && v.ps.f.on && v.ps.f.c.MB == q.f.c.MB &&
-                                          v.ps.f.c.RC == q.f.c.RC);
+                                          v.ps.f.c.RC == q.f.c.RC && v.ps.f.two == q.f.two);
                         if (!dup) add_up(q);
                     }
-                }
     }
     if (fft_ok && !fft_first) add_fft(fc);
 }
@@ -1151,9 +1156,9 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
     case KK_IGEMM_UP: {
         if (plan->ps.f.on) {
             const IgemmCfg& c = plan->ps.f.c;
-            snprintf(buf, len, "igemm_f32_up2 %s out=%dx%dx%d phases=4 fused window=%dx%d MB=%d RC=%d lds=%zu",
+            snprintf(buf, len, "igemm_f32_up2 %s out=%dx%dx%d phases=4 fused window=%dx%d MB=%d RC=%d%s lds=%zu",
                      mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, c.tileRows, c.tileCols, c.MB, c.RC,
-                     c.lds_bytes);
+                     plan->ps.f.two ? " 2/CU" : "", c.lds_bytes);
             break;
         }
         int n = snprintf(buf, len, "igemm_f32_up2 %s out=%dx%dx%d phases=%d", mode_name(plan->desc.mode), plan->m.nOut,
