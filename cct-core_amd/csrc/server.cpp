@@ -123,6 +123,8 @@ struct Req {
     cogconv_field_type out;
     size_t in0_bytes, in1_bytes, out_bytes;
     double flops;
+    cogconv_plan plan = nullptr;  // the server's plan for this request's descriptor
+    int bucket = -1;              // shape bucket (per-image descriptor and bank), interned at submit
     double t_submit = 0, t_done = 0;
     bool done = false;
     int waiters = 0;  // threads inside cogconv_server_wait on this request
@@ -175,7 +177,7 @@ struct cogconv_server_s {
     std::vector<Launch*> inflight;
     std::map<uint64_t, Req*> reqs;
     std::map<std::string, cogconv_plan> plans;
-    std::map<std::string, std::pair<size_t, double>> plan_info;  // desc -> (workspace, flops), for submit
+    std::map<std::string, int> buckets;  // bucket_key -> bucket id
     std::vector<hipEvent_t> event_pool;
     // Resident banks packed once per (plan, bank) for the plans that pack
     // operand 2 (the low-precision frame kernels): arena offset, kept for the
@@ -267,9 +269,7 @@ struct cogconv_server_s {
     // resident ones need a request-table launch, which the MFMA frame plans take.
     bool stack_ok(const Req* h) {
         if (!(h->r.flags & COGCONV_REQUEST_RESIDENT)) return true;
-        int err = 0;
-        cogconv_plan p = plan_for(h->r.desc, &err);
-        return p && plan_table_capable(p);
+        return h->plan && plan_table_capable(h->plan);
     }
 
     void fail_reqs(const std::vector<Req*>& g, int status, const std::string& msg) {
@@ -298,7 +298,7 @@ struct cogconv_server_s {
         d.batch = B;
         d.in0.planes = P0;
         int err = 0;
-        cogconv_plan plan = plan_for(d, &err);
+        cogconv_plan plan = g.size() == 1 ? h->plan : plan_for(d, &err);
         if (!plan) {
             fail_reqs(g, err, cogconv_last_error());
             return true;
@@ -559,9 +559,11 @@ struct cogconv_server_s {
                 std::vector<Req*> g{h};
                 std::vector<size_t> idx{0};
                 if (stackable(h->r.desc.mode) && cfg.max_stack > 1 && h->r.bank >= 0 && stack_ok(h)) {
-                    const std::string key = bucket_key(h);
-                    for (size_t j = 1; j < pending.size() && (int)g.size() < cfg.max_stack; j++) {
-                        if (bucket_key(pending[j]) == key) {
+                    // Bucket ids were interned at submit: the scan compares integers,
+                    // over a bounded window of the queue.
+                    const size_t window = std::min(pending.size(), (size_t)256);
+                    for (size_t j = 1; j < window && (int)g.size() < cfg.max_stack; j++) {
+                        if (pending[j]->bucket == h->bucket) {
                             g.push_back(pending[j]);
                             idx.push_back(j);
                         }
@@ -748,36 +750,38 @@ extern "C" int cogconv_server_submit(cogconv_server s, const cogconv_request* re
     q->in0_bytes = (size_t)d.in0.rows * d.in0.cols * d.in0.planes * sizeof(float);
     q->in1_bytes = (size_t)d.in1.rows * d.in1.cols * d.in1.planes * sizeof(float);
     q->out_bytes = (size_t)out.rows * out.cols * out.planes * sizeof(float);
+    // The request's plan (kept by the server for its life, one per descriptor)
+    // and its shape bucket, both resolved once here, so that the workers'
+    // admission and issue paths compare pointers and integers.
+    const std::string key(reinterpret_cast<const char*>(&d), sizeof d);
+    const std::string bkey = bucket_key(q);
     {
-        size_t ws = 0;
-        const std::string key(reinterpret_cast<const char*>(&d), sizeof d);
-        bool cached = false;
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            auto it = s->plan_info.find(key);
-            if (it != s->plan_info.end()) {
-                ws = it->second.first;
-                q->flops = it->second.second;
-                cached = true;
-            }
-        }
-        if (!cached) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->plans.find(key);
+        if (it != s->plans.end()) {
+            q->plan = it->second;
+        } else {
+            lk.unlock();
             cogconv_plan p = nullptr;
             const int pe = cogconv_plan_create(&d, &p);
             if (pe) {
                 delete q;
                 return pe;
             }
-            cogconv_plan_workspace_bytes(p, &ws);
-            cogconv_plan_flops(p, &q->flops);
-            cogconv_plan_destroy(p);
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->plan_info[key] = {ws, q->flops};
+            lk.lock();
+            auto ins = s->plans.emplace(key, p);
+            if (!ins.second) cogconv_plan_destroy(p);  // another submitter planned it meanwhile
+            q->plan = ins.first->second;
         }
-        const bool resident = (request->flags & COGCONV_REQUEST_RESIDENT) != 0;
-        const size_t need = resident ? Arena::round(ws)
-                                     : Arena::round(q->in0_bytes) + Arena::round(q->out_bytes) + Arena::round(ws) +
-                                           (request->bank < 0 ? Arena::round(q->in1_bytes) : 0);
+    }
+    size_t ws = 0;
+    cogconv_plan_workspace_bytes(q->plan, &ws);
+    cogconv_plan_flops(q->plan, &q->flops);
+    const bool resident = (request->flags & COGCONV_REQUEST_RESIDENT) != 0;
+    const size_t need = resident ? Arena::round(ws)
+                                 : Arena::round(q->in0_bytes) + Arena::round(q->out_bytes) + Arena::round(ws) +
+                                       (request->bank < 0 ? Arena::round(q->in1_bytes) : 0);
+    {
         std::lock_guard<std::mutex> lk(s->mu);
         const size_t room = s->arena.size - s->reserved;
         if (need > room) {
@@ -786,9 +790,8 @@ extern "C" int cogconv_server_submit(cogconv_server s, const cogconv_request* re
             return fail(COGCONV_ERR_BUDGET, "request needs %zu bytes, budget is %zu (%zu held by packed banks)", need,
                         s->arena.size, s->reserved);
         }
-    }
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
+        auto bit = s->buckets.emplace(bkey, (int)s->buckets.size()).first;
+        q->bucket = bit->second;
         q->ticket = s->next_ticket++;
         q->t_submit = now_s();
         if (!s->submitted) s->first_submit = q->t_submit;
