@@ -651,6 +651,8 @@ def serving_workload(args):
         # Time every distinct descriptor of the trace (both precisions) once, so the
         # server's plans come from the autotuning cache (a short sample per candidate).
         t0 = time.perf_counter()
+        if args.tune_cache and os.path.exists(args.tune_cache):
+            L.tune_cache_load(args.tune_cache)
         seen = {}
         for precision in (L.FP32, L.BF16):
             for i in range(n_req):
@@ -660,11 +662,18 @@ def serving_workload(args):
                     seen[key] = d
         for d in seen.values():
             L.autotune(d, warmup_steps=1, run_steps=3, flush_bytes=0)
+        if args.tune_cache:
+            L.tune_cache_save(args.tune_cache)
         tuned = {"descriptors": len(seen), "seconds": round(time.perf_counter() - t0, 1)}
         log("serving: tuned", tuned)
     results = {}
+    only_prec, only_run = (args.serving_only.split(":") + [""])[:2] if args.serving_only else (None, None)
     for transport in ("resident", "host"):
+        if only_prec and transport != "resident":
+            continue
         for label, precision in (("fp32", L.FP32), ("bf16", L.BF16)):
+            if only_prec and label != only_prec:
+                continue
             srv = L.Server(budget_bytes=8 << 30, max_inflight=16, streams=4, max_stack=8, graph_after=args.graph_after)
             for N, K, F, w in banks:
                 srv.register_bank(w)
@@ -683,6 +692,12 @@ def serving_workload(args):
                 # Offered-load sweep: the same request sequence at higher Poisson rates.
                 for rate in RATE_SWEEP:
                     runs.append(("open_loop_%d_per_s" % rate, arrivals * (2000.0 / rate)))
+            if only_run:
+                # A closed-loop pass first, unrecorded: plans made, banks packed, as in the full run.
+                srv.submit_timed(reqs, np.zeros(n_req))
+                srv.drain()
+                want = "closed_loop" if only_run == "closed" else "open_loop_%s_per_s" % only_run
+                runs = [(k, a) for k, a in runs if k == want]
             for mode_label, arr in runs:
                 srv.reset_stats()
                 # Native submission at the trace's arrival times (all at once for the closed loop).
@@ -701,6 +716,9 @@ def serving_workload(args):
                 results.setdefault(transport, {}).setdefault(label, {})[mode_label] = r
                 log(transport, label, mode_label, r)
             srv.close()
+    if only_prec:
+        print(json.dumps({"workload": "serving (one run)", "run": args.serving_only, "results": results}), flush=True)
+        return
     res = results["resident"]
     cap = res["bf16"]["closed_loop"]["requests_per_s"] / max(1e-9, res["fp32"]["closed_loop"]["requests_per_s"])
 
@@ -856,6 +874,9 @@ def main():
     ap.add_argument("--requests", type=int, default=3000, help="requests of the serving workload")
     ap.add_argument("--serving-tune", action="store_true",
                     help="serving: autotune every distinct request descriptor first (the plans then use the cache)")
+    ap.add_argument("--serving-only", default=None, metavar="PRECISION:RUN",
+                    help="serving workload: only this precision (fp32 / bf16) and run (closed / a rate), resident "
+                         "transport, e.g. bf16:closed (for profiling one run)")
     ap.add_argument("--graph-after", type=int, default=4,
                     help="serving: launches of one (plan, bank, stream) before it is captured into a hipGraph "
                          "(0 = never)")

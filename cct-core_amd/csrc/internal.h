@@ -334,6 +334,12 @@ bool tune_cache_lookup(const cogconv_desc* desc, int* variant, std::string* name
 
 }  // namespace cogconv
 
+// The layout of a plan's packed operand 2 (the low-precision frame kernels'
+// filter bank): plans with equal keys pack a bank identically (the field size
+// and batch do not enter), so one packed copy serves them all.  Empty for
+// plans that do not pack.
+std::string cogconv_plan_pack_key(const struct cogconv_plan_s* plan);
+
 struct cogconv_plan_s {
     cogconv_desc desc;
     cogconv_field_type out;

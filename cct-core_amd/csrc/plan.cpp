@@ -1222,6 +1222,15 @@ extern "C" int cogconv_plan_pack_operand(cogconv_plan plan, const float* in1, vo
     return COGCONV_OK;
 }
 
+std::string cogconv_plan_pack_key(const cogconv_plan_s* plan) {
+    if (!plan || plan->kind != KK_IGEMM_LP) return std::string();
+    // What k_pack_filters_lp reads: precision, mode, orientation, taps, planes
+    // and the padded output / input-plane counts of the kernel configuration.
+    const int key[] = {plan->desc.precision, plan->m.mode, plan->g.flip, plan->g.FH, plan->g.FW, plan->m.N,
+                       plan->m.T, plan->m.R, plan->icfg.Mout, plan->icfg.mTiles * 16 * plan->icfg.MB, plan->icfg.RC};
+    return std::string(reinterpret_cast<const char*>(key), sizeof key);
+}
+
 extern "C" int cogconv_launch_packed(cogconv_plan plan, const float* in0, float* out, void* workspace,
                                      cogconv_stream stream) {
     clear_error();

@@ -36,6 +36,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <unordered_map>
 #include <new>
 #include <string>
 #include <thread>
@@ -176,13 +177,23 @@ struct cogconv_server_s {
     std::deque<Req*> pending;
     std::vector<Launch*> inflight;
     std::map<uint64_t, Req*> reqs;
-    std::map<std::string, cogconv_plan> plans;
-    std::map<std::string, int> buckets;  // bucket_key -> bucket id
+    std::unordered_map<std::string, cogconv_plan> plans;
+    std::unordered_map<std::string, int> buckets;  // bucket_key -> bucket id
     std::vector<hipEvent_t> event_pool;
-    // Resident banks packed once per (plan, bank) for the plans that pack
-    // operand 2 (the low-precision frame kernels): arena offset, kept for the
-    // server's life; plans that do not pack map to SIZE_MAX.
-    std::map<std::pair<cogconv_plan, int>, size_t> packed;
+    // Resident banks packed once per (pack layout, bank) for the plans that
+    // pack operand 2 (the low-precision frame kernels): arena offset, kept for
+    // the server's life; plans that do not pack map to SIZE_MAX.  Plans of one
+    // bank that differ only in field size or batch share one packed copy.
+    std::map<std::pair<int, int>, size_t> packed;
+    std::unordered_map<cogconv_plan, int> pack_ids;   // plan -> pack layout id
+    std::unordered_map<std::string, int> pack_keys;   // cogconv_plan_pack_key -> id
+    int pack_id(cogconv_plan p) {
+        auto it = pack_ids.find(p);
+        if (it != pack_ids.end()) return it->second;
+        const int id = pack_keys.emplace(cogconv_plan_pack_key(p), (int)pack_keys.size()).first->second;
+        pack_ids.emplace(p, id);
+        return id;
+    }
     // Request tables: per stream one device table of max_stack entries (a
     // stream runs its launches in order, so the table copy of a launch lands
     // after the previous launch on that stream has read its own), and pinned
@@ -317,7 +328,7 @@ struct cogconv_server_s {
             // first time) and every later request runs on the packed form.
             size_t pk = SIZE_MAX;
             if (h->r.bank >= 0 && ws) {
-                const auto key = std::make_pair(plan, h->r.bank);
+                const auto key = std::make_pair(pack_id(plan), h->r.bank);
                 auto it = packed.find(key);
                 if (it == packed.end()) {
                     size_t o = 0;
