@@ -55,6 +55,17 @@ __device__ __forceinline__ float epi_eval(const Epi& e, float v, int plane, int 
     return v;
 }
 
+// Four adjacent output elements (16-byte aligned) as one store: non-temporal
+// (streamed to memory, no cache line kept) when no epilogue reads them back.
+__device__ __forceinline__ void store_out4(float* p, float a, float b, float c, float d, const Epi& e) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = {a, b, c, d};
+    if (e.n == 0)
+        __builtin_nontemporal_store(v, reinterpret_cast<f4*>(p));
+    else
+        *reinterpret_cast<f4*>(p) = v;
+}
+
 // The value stored for output element (plane, point): v itself without an epilogue.
 __device__ __forceinline__ float epi(const Epi& e, float v, int plane, int point) {
     return e.n == 0 ? v : epi_eval(e, v, plane, point);

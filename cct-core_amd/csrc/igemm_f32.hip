@@ -251,8 +251,8 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
                     const int orow = tr * 8 + 2 * rw + (pb >> 1);
                     const int ocol = tc * 32 + (pb & 1) * 16 + 4 * g4;
                     if (mm < Mout && orow < g.OH && ocol < g.OW)
-                        *reinterpret_cast<float4*>(O + ((size_t)(b * Mout + mm) * ph.OHf + ph.oRow0 + orow) * ph.OWf +
-                                                   ph.oCol0 + ocol) = make_float4(sum[0], sum[1], sum[2], sum[3]);
+                        store_out4(O + ((size_t)(b * Mout + mm) * ph.OHf + ph.oRow0 + orow) * ph.OWf + ph.oCol0 + ocol,
+                                   sum[0], sum[1], sum[2], sum[3], e);
                 }
             }
             // The epilogue rewrites the thread's own stores (still in L2), so the
@@ -569,8 +569,8 @@ __device__ __forceinline__ void up2q_body(const Geom& g, const PlaneMap& m, cons
             float* dst = O + ((size_t)(b * Mout + mm) * g.OH + orow) * g.OW + ocol;
             const floatx4 s0 = acc[0][mb][pb], s1 = acc[1][mb][pb];
             if (vec && ocol + 8 <= g.OW) {
-                *reinterpret_cast<float4*>(dst) = make_float4(s0[0], s1[0], s0[1], s1[1]);
-                *reinterpret_cast<float4*>(dst + 4) = make_float4(s0[2], s1[2], s0[3], s1[3]);
+                store_out4(dst, s0[0], s1[0], s0[1], s1[1], e);
+                store_out4(dst + 4, s0[2], s1[2], s0[3], s1[3], e);
             } else {
 #pragma unroll
                 for (int v = 0; v < 8; v++)

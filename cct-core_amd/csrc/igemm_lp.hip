@@ -539,8 +539,7 @@ __device__ __forceinline__ void igemm_lp_body(const Geom& g, const PlaneMap& m, 
             if (mm >= Mout || orow >= g.OH || ocol >= g.OW) continue;
             float* dst = O + ((size_t)(b * Mout + mm) * g.OH + orow) * g.OW + ocol;
             if (vec) {
-                *reinterpret_cast<float4*>(dst) =
-                    make_float4(acc[mb][pb][0], acc[mb][pb][1], acc[mb][pb][2], acc[mb][pb][3]);
+                store_out4(dst, acc[mb][pb][0], acc[mb][pb][1], acc[mb][pb][2], acc[mb][pb][3], e);
             } else {
 #pragma unroll
                 for (int v = 0; v < 4; v++)
