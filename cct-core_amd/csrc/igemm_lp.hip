@@ -277,7 +277,7 @@ __device__ __forceinline__ void igemm_lp_body(const Geom& g, const PlaneMap& m, 
             int vo = rowOn ? colByte + planeBytes * (r0 + 8 * grp) : LP_OOB;
 #pragma unroll
             for (int e = 0; e < 8; e++) {
-                v[u][e] = buffer_load_f32(xr, vo, rowByte);
+                v[u][e] = buffer_load_f32_nt(xr, vo, rowByte);
                 vo += planeBytes;
             }
             ty += 4;
@@ -353,7 +353,7 @@ __device__ __forceinline__ void igemm_lp_body(const Geom& g, const PlaneMap& m, 
                 int vo = on ? 4 * ((r0 + 8 * grp) * HW + y * g.W + colA) : LP_OOB;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
-                    q[pp][e] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, vo, 0, 0));
+                    q[pp][e] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(xr, vo, 0, LOAD_NT));
                     vo += planeBytes;
                 }
                 // Groups outside a Clamp / Cyclic field are written by the halo pass.
@@ -400,7 +400,7 @@ __device__ __forceinline__ void igemm_lp_body(const Geom& g, const PlaneMap& m, 
                 float hvals[8];
 #pragma unroll
                 for (int e8 = 0; e8 < 8; e8++) {
-                    hvals[e8] = buffer_load_f32(xr, vo, 0);
+                    hvals[e8] = buffer_load_f32_nt(xr, vo, 0);
                     vo += planeBytes;
                 }
                 V hv, lv;

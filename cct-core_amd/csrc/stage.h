@@ -19,6 +19,15 @@ __device__ __forceinline__ float buffer_load_f32(__amdgpu_buffer_rsrc_t r, int v
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vbytes, sbytes, 0));
 }
 
+// Cache policy of the loads that stage input fields: non-temporal (nt), so
+// that the streamed field does not push out of the L2 what every block re-reads
+// (the filter bank) -- c4 BF16 7x7 92 -> 83 us.
+constexpr int LOAD_NT = 2;
+
+__device__ __forceinline__ float buffer_load_f32_nt(__amdgpu_buffer_rsrc_t r, int vbytes, int sbytes) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vbytes, sbytes, LOAD_NT));
+}
+
 // Stages IU rows of up to two 64-lane passes into LDS: every load first (from
 // in-bounds element offsets off[u] + col of buffer `src`, with no branch
 // between the loads), then the masked stores.  Row u goes to lds + dst[u] + lane*pitch (second pass: lane
@@ -31,8 +40,8 @@ __device__ __forceinline__ void stage_rows(__amdgpu_buffer_rsrc_t src, const int
     float v0[IU], v1[IU];
 #pragma unroll
     for (int u = 0; u < IU; u++) {
-        v0[u] = buffer_load_f32(src, 4 * col0, 4 * off[u]);
-        if (TWO) v1[u] = buffer_load_f32(src, 4 * col1, 4 * off[u]);
+        v0[u] = buffer_load_f32_nt(src, 4 * col0, 4 * off[u]);
+        if (TWO) v1[u] = buffer_load_f32_nt(src, 4 * col1, 4 * off[u]);
     }
     __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll
@@ -67,8 +76,8 @@ __device__ __forceinline__ void stage_rows_fn(__amdgpu_buffer_rsrc_t src, RowFn 
         int soff;
         bool on;
         row(u, soff, on);
-        v0[u] = buffer_load_f32(src, on ? vcol0 : STAGE_OOB, soff);
-        if (TWO) v1[u] = buffer_load_f32(src, on ? vcol1 : STAGE_OOB, soff);
+        v0[u] = buffer_load_f32_nt(src, on ? vcol0 : STAGE_OOB, soff);
+        if (TWO) v1[u] = buffer_load_f32_nt(src, on ? vcol1 : STAGE_OOB, soff);
     }
     __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll
@@ -102,8 +111,8 @@ __device__ __forceinline__ void load_rows_fn(__amdgpu_buffer_rsrc_t src, RowFn r
         int soff;
         bool on;
         row(u, soff, on);
-        if (TWO) v1[u] = buffer_load_f32(src, on ? vcol1 : STAGE_OOB, soff);
-        v0[u] = buffer_load_f32(src, on ? vcol0 : STAGE_OOB, soff);
+        if (TWO) v1[u] = buffer_load_f32_nt(src, on ? vcol1 : STAGE_OOB, soff);
+        v0[u] = buffer_load_f32_nt(src, on ? vcol0 : STAGE_OOB, soff);
     }
 }
 
