@@ -176,9 +176,18 @@ struct cogconv_server_s {
     std::condition_variable cv_work, cv_done;
     std::deque<Req*> pending;
     std::vector<Launch*> inflight;
-    std::map<uint64_t, Req*> reqs;
+    std::unordered_map<uint64_t, Req*> reqs;
     std::unordered_map<std::string, cogconv_plan> plans;
-    std::unordered_map<std::string, int> buckets;  // bucket_key -> bucket id
+    // Shape buckets: (plan, bank) of one-image requests and bucket_key of
+    // batched ones -> bucket id.
+    struct PlanBankHash {
+        size_t operator()(const std::pair<cogconv_plan, int>& k) const {
+            return std::hash<const void*>()(k.first) * 31 + std::hash<int>()(k.second);
+        }
+    };
+    std::unordered_map<std::pair<cogconv_plan, int>, int, PlanBankHash> plan_buckets;
+    std::unordered_map<std::string, int> buckets;
+    size_t bucket_count = 0;
     std::vector<hipEvent_t> event_pool;
     // Resident banks packed once per (pack layout, bank) for the plans that
     // pack operand 2 (the low-precision frame kernels): arena offset, kept for
@@ -739,78 +748,77 @@ extern "C" int cogconv_server_submit(cogconv_server s, const cogconv_request* re
     clear_error();
     if (!s || !request || !ticket) return fail(COGCONV_ERR_INVALID_ARG, "null argument");
     if (!request->host_in0 || !request->host_out) return fail(COGCONV_ERR_INVALID_ARG, "null buffer");
-    cogconv_desc d = request->desc;
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (request->bank >= 0) {
-            if (request->bank >= (int32_t)s->banks.size())
-                return fail(COGCONV_ERR_INVALID_ARG, "unknown filter bank %d", request->bank);
-            d.in1 = s->banks[request->bank].type;
-        } else if (!request->host_in1) {
-            return fail(COGCONV_ERR_INVALID_ARG, "request without bank needs host_in1");
-        }
-    }
-    cogconv_field_type out;
-    int e = cogconv_output_type(&d, &out);
-    if (e) return e;
     Req* q = new (std::nothrow) Req;
     if (!q) return fail(COGCONV_ERR_OOM, "host allocation failed");
     q->r = *request;
-    q->r.desc = d;
-    q->out = out;
+    cogconv_desc& d = q->r.desc;
+    // One critical section on the common path: the request's plan (kept by the
+    // server for its life, one per descriptor; made outside the lock the first
+    // time) and its shape bucket are resolved here, so that the workers'
+    // admission and issue paths compare pointers and integers.
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (request->bank >= 0) {
+        if (request->bank >= (int32_t)s->banks.size()) {
+            delete q;
+            return fail(COGCONV_ERR_INVALID_ARG, "unknown filter bank %d", request->bank);
+        }
+        d.in1 = s->banks[request->bank].type;
+    } else if (!request->host_in1) {
+        delete q;
+        return fail(COGCONV_ERR_INVALID_ARG, "request without bank needs host_in1");
+    }
+    const std::string key(reinterpret_cast<const char*>(&d), sizeof d);
+    auto it = s->plans.find(key);
+    if (it == s->plans.end()) {
+        lk.unlock();
+        cogconv_plan p = nullptr;
+        const int pe = cogconv_plan_create(&d, &p);  // also rejects an illegal descriptor
+        if (pe) {
+            delete q;
+            return pe;
+        }
+        lk.lock();
+        auto ins = s->plans.emplace(key, p);
+        if (!ins.second) cogconv_plan_destroy(p);  // another submitter planned it meanwhile
+        it = ins.first;
+    }
+    q->plan = it->second;
+    q->out = q->plan->out;
+    q->flops = q->plan->flops;
     q->in0_bytes = (size_t)d.in0.rows * d.in0.cols * d.in0.planes * sizeof(float);
     q->in1_bytes = (size_t)d.in1.rows * d.in1.cols * d.in1.planes * sizeof(float);
-    q->out_bytes = (size_t)out.rows * out.cols * out.planes * sizeof(float);
-    // The request's plan (kept by the server for its life, one per descriptor)
-    // and its shape bucket, both resolved once here, so that the workers'
-    // admission and issue paths compare pointers and integers.
-    const std::string key(reinterpret_cast<const char*>(&d), sizeof d);
-    const std::string bkey = bucket_key(q);
-    {
-        std::unique_lock<std::mutex> lk(s->mu);
-        auto it = s->plans.find(key);
-        if (it != s->plans.end()) {
-            q->plan = it->second;
-        } else {
-            lk.unlock();
-            cogconv_plan p = nullptr;
-            const int pe = cogconv_plan_create(&d, &p);
-            if (pe) {
-                delete q;
-                return pe;
-            }
-            lk.lock();
-            auto ins = s->plans.emplace(key, p);
-            if (!ins.second) cogconv_plan_destroy(p);  // another submitter planned it meanwhile
-            q->plan = ins.first->second;
-        }
-    }
-    size_t ws = 0;
-    cogconv_plan_workspace_bytes(q->plan, &ws);
-    cogconv_plan_flops(q->plan, &q->flops);
+    q->out_bytes = (size_t)q->out.rows * q->out.cols * q->out.planes * sizeof(float);
+    const size_t ws = q->plan->workspace;
     const bool resident = (request->flags & COGCONV_REQUEST_RESIDENT) != 0;
     const size_t need = resident ? Arena::round(ws)
                                  : Arena::round(q->in0_bytes) + Arena::round(q->out_bytes) + Arena::round(ws) +
                                        (request->bank < 0 ? Arena::round(q->in1_bytes) : 0);
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        const size_t room = s->arena.size - s->reserved;
-        if (need > room) {
-            s->rejected++;
-            delete q;
-            return fail(COGCONV_ERR_BUDGET, "request needs %zu bytes, budget is %zu (%zu held by packed banks)", need,
-                        s->arena.size, s->reserved);
-        }
-        auto bit = s->buckets.emplace(bkey, (int)s->buckets.size()).first;
-        q->bucket = bit->second;
-        q->ticket = s->next_ticket++;
-        q->t_submit = now_s();
-        if (!s->submitted) s->first_submit = q->t_submit;
-        s->submitted++;
-        s->reqs[q->ticket] = q;
-        s->pending.push_back(q);
-        *ticket = q->ticket;
+    const size_t room = s->arena.size - s->reserved;
+    if (need > room) {
+        s->rejected++;
+        delete q;
+        return fail(COGCONV_ERR_BUDGET, "request needs %zu bytes, budget is %zu (%zu held by packed banks)", need,
+                    s->arena.size, s->reserved);
     }
+    // Shape bucket: a one-image request's plan and bank name it; a batched
+    // request's per-image descriptor does (a string key, the rare case).
+    if (d.batch == 1) {
+        auto bit = s->plan_buckets.emplace(std::make_pair(q->plan, request->bank), (int)s->bucket_count).first;
+        if (bit->second == (int)s->bucket_count) s->bucket_count++;
+        q->bucket = bit->second;
+    } else {
+        auto bit = s->buckets.emplace(bucket_key(q), (int)s->bucket_count).first;
+        if (bit->second == (int)s->bucket_count) s->bucket_count++;
+        q->bucket = bit->second;
+    }
+    q->ticket = s->next_ticket++;
+    q->t_submit = now_s();
+    if (!s->submitted) s->first_submit = q->t_submit;
+    s->submitted++;
+    s->reqs[q->ticket] = q;
+    s->pending.push_back(q);
+    *ticket = q->ticket;
+    lk.unlock();
     s->cv_work.notify_all();
     return COGCONV_OK;
 }
