@@ -104,3 +104,44 @@ def test_server_resident_packed_bank_matches_launch():
     for b in ins + outs + refs + [bw] + ([ws] if ws else []):
         b.close()
     plan.close()
+
+
+@pytest.mark.parametrize("precision", (L.BF16, L.FP16))
+def test_server_packed_bank_shared_across_field_sizes(precision):
+    """One registered bank serves resident requests of several field sizes and
+    both frame modes in one mix: plans that pack the bank identically share one
+    packed copy (the packed layout does not depend on the field size), and every
+    result equals a plain launch of its own plan bit for bit."""
+    print("CASE server shared packed bank", NAMES[precision], flush=True)
+    rng = np.random.default_rng(88 + precision)
+    N, K, F = 32, 32, 5
+    w = rng.uniform(-1, 1, (K * N, F, F)).astype(np.float32)
+    srv = L.Server(budget_bytes=256 << 20, max_inflight=8, streams=4, max_stack=1)
+    bid = srv.register_bank(w)
+    bw = L.DeviceBuffer.from_array(w)
+    cases = []
+    for i, (mode, side) in enumerate([(m, s) for s in (24, 40, 72, 37) for m in (O.PF_BRS, O.BPF_BRS)] * 2):
+        orient = O.CONV if mode == O.BPF_BRS else O.XCORR
+        x = rng.uniform(-1, 1, (N if mode == O.PF_BRS else K, side, side)).astype(np.float32)
+        _, cd = desc_pair(mode, O.ZERO, orient, O.NONE, 1, 1, x.shape, w.shape, precision=precision)
+        plan = L.Plan(cd)
+        ws = L.DeviceBuffer(plan.workspace_bytes) if plan.workspace_bytes else None
+        bx = L.DeviceBuffer.from_array(x)
+        bo = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
+        br = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
+        plan.launch(bx, bw, br, ws)
+        cases.append((cd, plan, ws, bx, bo, br))
+    L.synchronize()
+    tickets = [srv.submit(cd, bx, bo, bank=bid) for cd, _, _, bx, bo, _ in cases]
+    for t in tickets:
+        srv.wait(t)
+    assert srv.stats()["completed"] == len(cases)
+    for cd, plan, ws, bx, bo, br in cases:
+        assert np.array_equal(bo.download(plan.out_shape), br.download(plan.out_shape)), plan.description
+    srv.close()
+    for cd, plan, ws, bx, bo, br in cases:
+        for b in (ws, bx, bo, br):
+            if b is not None:
+                b.close()
+        plan.close()
+    bw.close()
