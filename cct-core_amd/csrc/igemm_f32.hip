@@ -109,12 +109,15 @@ __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg 
     float* sJunk = sA + (size_t)c.RC * aStride;  // 128 scratch slots for masked-off stores
 
     int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    // Output-plane blocks of one tile are adjacent block indices (the same XCD
+    // after the swizzle, and close in time), so the tile's second and later
+    // blocks find its input in L2.
+    const int mt = bid % c.mTiles;
+    bid /= c.mTiles;
     const int tc = bid % c.tilesC;
     bid /= c.tilesC;
     const int tr = bid % c.tilesR;
-    bid /= c.tilesR;
-    const int mt = bid % c.mTiles;
-    const int b = bid / c.mTiles;
+    const int b = bid / c.tilesR;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -473,12 +476,15 @@ __device__ __forceinline__ void up2q_body(const Geom& g, const PlaneMap& m, cons
     float* sJunk = sA + (size_t)c.RC * aStride;
 
     int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    // Output-plane blocks of one tile are adjacent block indices (the same XCD
+    // after the swizzle, and close in time), so the tile's second and later
+    // blocks find its input in L2.
+    const int mt = bid % c.mTiles;
+    bid /= c.mTiles;
     const int tc = bid % c.tilesC;
     bid /= c.tilesC;
     const int tr = bid % c.tilesR;
-    bid /= c.tilesR;
-    const int mt = bid % c.mTiles;
-    const int b = bid / c.mTiles;
+    const int b = bid / c.tilesR;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
