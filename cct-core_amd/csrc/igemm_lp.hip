@@ -104,13 +104,15 @@ __device__ __forceinline__ void igemm_lp_body(const Geom& g, const PlaneMap& m, 
     // width, after the filter slice(s): TERMS x 64 lanes x 8 elements.
     const size_t sJunk = (size_t)TERMS * 4 * npix * 8 + (size_t)(RING ? 2 : 1) * gElems;
 
+    // Output-plane blocks of one tile are adjacent block indices (same XCD,
+    // close in time): the tile's later blocks find its input in L2.
     int bid = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int mt = bid % c.mTiles;
+    bid /= c.mTiles;
     const int tc = bid % c.tilesC;
     bid /= c.tilesC;
     const int tr = bid % c.tilesR;
-    bid /= c.tilesR;
-    const int mt = bid % c.mTiles;
-    const int b = bid / c.mTiles;
+    const int b = bid / c.tilesR;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
