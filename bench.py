@@ -120,14 +120,15 @@ def triple_descs(L, B=16, N=16, K=32, H=64, precision=0, extras=False):
 
 # ---------------------------------------------------------------- timing
 
-def tuned_plan(L, desc, args):
+def tuned_plan(L, desc, args, flush_bytes=0):
     """The plan for `desc`, with its kernel configuration chosen by the
-    autotuner (cogconv_autotune) unless --no-tune."""
+    autotuner (cogconv_autotune) unless --no-tune.  Candidates are timed with
+    warm caches (flush_bytes=0), as in the timed steps, where the inputs are
+    reused every step, or after a cache flush of flush_bytes per launch."""
     if not args.no_tune:
         if args.tune_cache:
             L.tune_cache_load(args.tune_cache)
-        # Warm caches, as in the timed steps (the inputs are reused every step).
-        r = L.autotune(desc, warmup_steps=5, run_steps=60, flush_bytes=0)
+        r = L.autotune(desc, warmup_steps=5, run_steps=60 if not flush_bytes else 20, flush_bytes=flush_bytes)
         log("tuned:", r["description"], "%.2f us" % r["avg_us"], "(cached)" if r["from_cache"] else
             "(%d candidates)" % r["variants"])
         if args.tune_cache:
@@ -387,8 +388,10 @@ def precision_workload(args):
                  (2, 3, 3, K * N), bw),
                 ("filterAdjoint+BRS", L.FILTER_ADJOINT_BLOCK_REDUCE_SUM, L.BORDER_VALID, L.CROSS_CORRELATION, x, bx,
                  f(B * K, H - 2), bdyv)):
+            # Candidates timed after a cache flush: this workload compares the ops'
+            # flushed medians.
             plan = tuned_plan(L, L.make_desc(mode, border, orient, L.NO_SAMPLING, 1, B, f(a.shape[0]), t1, prec),
-                              args)
+                              args, flush_bytes=FLUSH_BYTES)
             op = Op(L, opname, plan, ab, b1)
             ms, lo, hi = flushed_median_ms(L, op, stream, flush, reps=args.op_reps)
             warm = timed_steps(L, [op], stream, 20, 3)  # back to back, operands left in the caches
