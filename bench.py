@@ -12,7 +12,8 @@ BorderZero).  Inputs are seeded and resident in HBM before the timed region.
 
 Prints one JSON line: `value` = FP32 GFLOP/s of the timed steps; `bf16` = the
 same step with the BF16-MFMA plans where they are offered (the two Clamp
-ProjectFrames; the others stay FP32); per-op medians of >= 20 launches, each
+ProjectFrames and the up-sampled ProjectFrame; PlaneByPlane stays FP32);
+per-op medians of >= 20 launches, each
 after a 512 MiB cache-flush write, timed with HIP events on the op's stream;
 the roofline of the dominant FP32 kernel; the CPU baseline (the oracle's FP32
 restatement, OpenMP, on a bounded sample of the same workload, on every host
@@ -69,17 +70,17 @@ def large_inputs(seed=4, B=8, N=32, K=32, H=256):
 
 def large_ops(L, precision, B=8, N=32, K=32, H=256):
     """(name, desc, operand-1 key, operand-2 key) of one step of the large-field
-    workload.  BF16 is offered for the Clamp ProjectFrames; the up-sampled
-    ProjectFrame and PlaneByPlane run FP32 in both steps."""
+    workload.  BF16 is offered for the Clamp and the up-sampled ProjectFrames;
+    PlaneByPlane runs FP32 in both steps."""
     pn = {L.FP32: "fp32", L.BF16: "bf16"}[precision]
     ops = []
     for F in (7, 9):
         ops.append(("projectFrame+BRS clamp %dx%d %s" % (F, F, pn),
                     L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_CLAMP, L.CROSS_CORRELATION,
                                 L.NO_SAMPLING, 1, B, (2, H, H, B * N), (2, F, F, K * N), precision), "x", F))
-    ops.append(("projectFrame+BRS up2 zero 7x7 fp32",
+    ops.append(("projectFrame+BRS up2 zero 7x7 %s" % pn,
                 L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CROSS_CORRELATION,
-                            L.UPSAMPLE_INPUT, 2, B, (2, H, H, B * N), (2, 7, 7, K * N)), "x", 7))
+                            L.UPSAMPLE_INPUT, 2, B, (2, H, H, B * N), (2, 7, 7, K * N), precision), "x", 7))
     ops.append(("planeByPlane clamp 1024x1024 11x11 fp32",
                 L.make_desc(L.PLANE_BY_PLANE, L.BORDER_CLAMP, L.CONVOLUTION, L.NO_SAMPLING, 1, 1,
                             (2, 1024, 1024, 1), (2, 11, 11, 1)), "img", "pbp"))
@@ -452,8 +453,8 @@ def precision_workload(args):
 
 
 def large_errors(L, x, banks, N=32, H=256):
-    """Normwise and rel-L2 errors of the Clamp ProjectFrames (FP32 and BF16)
-    against the FP64 oracle on image 0."""
+    """Normwise and rel-L2 errors of the Clamp and the up-sampled ProjectFrames
+    (FP32 and BF16) against the FP64 oracle on image 0."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     errs = {}
@@ -477,6 +478,23 @@ def large_errors(L, x, banks, N=32, H=256):
             op.close()
             plan.close()
         wb.close()
+    # The up-sampled 7x7 ProjectFrame (BorderZero) on image 0.
+    od = O.make_desc(O.PF_BRS, O.ZERO, O.XCORR, O.UP, 2, 1, xs.shape, banks[7].shape)
+    ref = O.conv(od, xs, banks[7], threads=host_cores())
+    wb = L.DeviceBuffer.from_array(banks[7])
+    for prec, pn in ((L.FP32, "fp32"), (L.BF16, "bf16")):
+        plan = L.Plan(L.make_desc(L.PROJECT_FRAME_BLOCK_REDUCE_SUM, L.BORDER_ZERO, L.CROSS_CORRELATION,
+                                  L.UPSAMPLE_INPUT, 2, 1, (2, H, H, N), (2, 7, 7, banks[7].shape[0]), prec))
+        op = Op(L, "", plan, a, wb)
+        op.launch(None)
+        L.synchronize()
+        got = op.out.download(plan.out_shape).astype(np.float64)
+        d = np.abs(got - ref)
+        errs["up2 7x7 %s" % pn] = {"max_abs": float(d.max()), "max_rel_normwise": float(d.max() / np.abs(ref).max()),
+                                   "rel_l2": float(np.sqrt((d ** 2).sum() / (ref ** 2).sum()))}
+        op.close()
+        plan.close()
+    wb.close()
     a.close()
     return errs
 
@@ -549,6 +567,8 @@ def large_workload(args):
         a = fp32["per_op"]["projectFrame+BRS clamp %dx%d fp32" % (F, F)]["ms"]
         b = bf16["per_op"]["projectFrame+BRS clamp %dx%d bf16" % (F, F)]["ms"]
         ratios["clamp %dx%d" % (F, F)] = round(a / b, 2)
+    ratios["up2 zero 7x7"] = round(fp32["per_op"]["projectFrame+BRS up2 zero 7x7 fp32"]["ms"] /
+                                   bf16["per_op"]["projectFrame+BRS up2 zero 7x7 bf16"]["ms"], 2)
     errs = large_errors(L, x, banks)
     extras = None if args.no_extras else triple_extras(L, args, stream, flush)
     for st in steps.values():
@@ -578,8 +598,8 @@ def large_workload(args):
                    "step": "the four ops back to back on one stream"},
         "per_op": fp32["per_op"],
         "bf16": {"value": round(bf16_value, 2), "unit": "GFLOP/s", "ms_per_step": round(bf16["ms_per_step"], 5),
-                 "step": "the Clamp ProjectFrames as BF16-MFMA plans (FP32 accumulation); the up-sampled op and "
-                         "PlaneByPlane stay FP32",
+                 "step": "the Clamp and the up-sampled ProjectFrames as BF16-MFMA plans (FP32 accumulation); "
+                         "PlaneByPlane stays FP32",
                  "per_op": {k: v for k, v in bf16["per_op"].items() if "bf16" in k},
                  "bf16_over_fp32_per_op": ratios, "roofline": broof},
         "errors_vs_fp64_image0": errs,

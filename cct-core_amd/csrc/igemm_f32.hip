@@ -79,21 +79,6 @@ __device__ __forceinline__ void stage_input_f32(const Geom& g, const PlaneMap& m
     }
 }
 
-// Per-lane column of a staged tile starting at stored column x0 + 64 p, with
-// the border applied: the byte offset of the column, or STAGE_OOB where the
-// lane's column is off the field (BorderZero and the like).
-__device__ __forceinline__ int stage_col_offset(const Geom& g, int x) {
-    bool in = (unsigned)x < (unsigned)g.W;
-    if (g.border == COGCONV_BORDER_CYCLIC) {
-        x %= g.W;
-        x += x < 0 ? g.W : 0;
-        in = true;
-    } else {
-        in = in || g.border == COGCONV_BORDER_CLAMP;
-        x = min(max(x, 0), g.W - 1);
-    }
-    return in ? 4 * x : STAGE_OOB;
-}
 
 template <int FR, int FC, int MB, bool PH>
 __global__ __launch_bounds__(512) void k_igemm_f32(Geom g, PlaneMap m, IgemmCfg c, Phase ph, const float* __restrict__ X,
@@ -437,7 +422,8 @@ __device__ __forceinline__ void up2q_chunk(const float* sB, const float* sA, int
         const float* ab = sA + (size_t)(p + g4) * aStride + l16;
         // Two blocks a CU: one sub-filter row at a time, so that the LDS loads
         // the compiler hoists stay within 128 registers.
-#pragma unroll(TWO ? 1 : FRp)
+        constexpr int UNROLL = TWO ? 1 : FRp;
+#pragma unroll UNROLL
         for (int a = 0; a < FRp; a++) {
 #pragma unroll
             for (int pc = 0; pc < 2; pc++) {

@@ -671,6 +671,25 @@ static hipError_t launch_lp1(const Geom& g, const PlaneMap& m, const IgemmCfg& c
     }
 }
 
+hipError_t launch_pack_filters_lp(int precision, const Geom& g, const PlaneMap& m, int Mout, int Mpad, int Rpad,
+                                  const float* w, void* wp, hipStream_t s) {
+    const long packed = (long)g.FH * g.FW * Mpad * Rpad;
+    const long pblocks = std::min((packed + 255) / 256, 4096L);
+    if (pblocks <= 0) return hipErrorInvalidConfiguration;
+    switch (precision) {
+    case COGCONV_BF16:
+        hipLaunchKernelGGL((k_pack_filters_lp<COGCONV_BF16>), dim3((unsigned)pblocks), dim3(256), 0, s, g, m, Mout, Mpad,
+                           Rpad, w, static_cast<LpType<COGCONV_BF16>::T*>(wp));
+        break;
+    case COGCONV_FP16:
+        hipLaunchKernelGGL((k_pack_filters_lp<COGCONV_FP16>), dim3((unsigned)pblocks), dim3(256), 0, s, g, m, Mout, Mpad,
+                           Rpad, w, static_cast<LpType<COGCONV_FP16>::T*>(wp));
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 bool igemm_lp_supported(int F) { return F == 1 || F == 3 || F == 5 || F == 7 || F == 9; }
 
 hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& cfg, const float* x,

@@ -299,6 +299,16 @@ size_t igemm_lp_lds_bytes(const IgemmCfg& c, int F, int precision);
 bool igemm_lp_two_blocks(const IgemmCfg& c);
 size_t igemm_lp_workspace_bytes(const IgemmCfg& c, int F, int precision);
 enum LpStage { LP_PACK_AND_RUN = 0, LP_PACK_ONLY = 1, LP_RUN_ONLY = 2 };
+// The BF16 / FP16 filter pack of k_igemm_lp (k_pack_filters_lp) on its own.
+hipError_t launch_pack_filters_lp(int precision, const Geom& g, const PlaneMap& m, int Mout, int Mpad, int Rpad,
+                                  const float* w, void* wp, hipStream_t s);
+// BF16 / FP16 fused four-phase 2x up-sampling (up2q_lp.hip): LDS bytes and
+// packed-filter workspace of a configuration; the launch packs the bank into
+// ws and runs the kernel.
+size_t up2q_lp_lds_bytes(const IgemmCfg& c, int F);
+size_t up2q_lp_workspace_bytes(const IgemmCfg& c, int F);
+hipError_t launch_up2q_lp(int precision, const Geom& g, const PlaneMap& m, const UpFused& u, const float* x,
+                          const float* w, float* out, void* ws, hipStream_t s, const Epi& e);
 hipError_t launch_igemm_lp(int precision, const Geom& g, const PlaneMap& m, const IgemmCfg& c, const float* x,
                            const float* w, float* out, void* ws, hipStream_t s, int stage = LP_PACK_AND_RUN,
                            const Epi& e = kNoEpi, const cogconv_image_io* tab = nullptr, bool x16 = false);

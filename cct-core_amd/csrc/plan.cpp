@@ -754,6 +754,32 @@ bool choose_igemm_up2q(const Geom& g, const PlaneMap& m, const PhaseSet& ps, UpF
     return false;
 }
 
+// The BF16 / FP16 form of the fused up-sampled launch (k_up2q_lp): the same
+// window as the FP32 form, 32-plane chunks, the whole filter slice of a chunk
+// in LDS beside the window (one block a CU).
+bool choose_up2q_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, const PhaseSet& ps, UpFused* out,
+                    int mbOnly) {
+    if (d->precision != COGCONV_BF16 && d->precision != COGCONV_FP16) return false;
+    UpFused u;
+    if (!choose_igemm_up2q(g, m, ps, &u, 1, 8, 160 * 1024)) return false;  // the window and phase offsets
+    if ((long)(m.N + 8) * g.H * g.W * 4 >= (1L << 31)) return false;  // staging offsets 8 planes past the image
+    if (u.c.tileRows > 64) return false;                                  // one lane per window row
+    IgemmCfg& c = u.c;
+    c.RC = (m.R + 31) / 32 * 32;
+    for (int MB = c.Mout <= 16 ? 1 : 2; MB >= 1; MB--) {
+        if (mbOnly && MB != mbOnly) continue;
+        c.MB = MB;
+        c.mTiles = (c.Mout + 16 * MB - 1) / (16 * MB);
+        c.aStride = 0;
+        c.lds_bytes = up2q_lp_lds_bytes(c, g.FH);
+        if (c.lds_bytes <= 160 * 1024 && up2q_lp_workspace_bytes(c, g.FH) < (1UL << 31)) {
+            *out = u;
+            return true;
+        }
+    }
+    return false;
+}
+
 void describe_into(const cogconv_plan_s* plan, char* buf, size_t len);
 
 // The candidate configurations for `desc`, the heuristic choice first.  The
@@ -813,6 +839,26 @@ void enumerate_variants(const cogconv_desc* desc, const Geom& g, const PlaneMap&
         out.push_back(v);
     };
     if (desc->precision != COGCONV_FP32) {
+        // 2x up-sampled frame modes: the fused four-phase launch in BF16 / FP16.
+        PhaseSet up;
+        if (choose_igemm_up(desc, g, m, &up)) {
+            bool any = false;
+            for (int mb : {2, 1}) {
+                PhaseSet q = up;
+                if (!choose_up2q_lp(desc, g, m, up, &q.f, mb)) continue;
+                bool dup = false;
+                for (const Variant& v : out) dup = dup || (v.kind == KK_IGEMM_UP && v.ps.f.c.MB == q.f.c.MB);
+                if (dup) continue;
+                Variant v;
+                std::memset(&v, 0, sizeof v);
+                v.kind = KK_IGEMM_UP;
+                v.ps = q;
+                v.workspace = up2q_lp_workspace_bytes(q.f.c, g.FH);
+                out.push_back(v);
+                any = true;
+            }
+            if (any) return;
+        }
         FaCfg flp, frl;
         const bool rows = choose_fa_rows_lp(desc, g, m, &frl);
         if (rows) {
@@ -1155,10 +1201,11 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
     }
     case KK_IGEMM_UP: {
         if (plan->ps.f.on) {
+            static const char* pn[] = {"f32", "bf16", "fp16", "split_bf16"};
             const IgemmCfg& c = plan->ps.f.c;
-            snprintf(buf, len, "igemm_f32_up2 %s out=%dx%dx%d phases=4 fused window=%dx%d MB=%d RC=%d%s lds=%zu",
-                     mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, c.tileRows, c.tileCols, c.MB, c.RC,
-                     plan->ps.f.two ? " 2/CU" : "", c.lds_bytes);
+            snprintf(buf, len, "igemm_%s_up2 %s out=%dx%dx%d phases=4 fused window=%dx%d MB=%d RC=%d%s lds=%zu",
+                     pn[plan->desc.precision], mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW, c.tileRows,
+                     c.tileCols, c.MB, c.RC, plan->ps.f.two ? " 2/CU" : "", c.lds_bytes);
             break;
         }
         int n = snprintf(buf, len, "igemm_f32_up2 %s out=%dx%dx%d phases=%d", mode_name(plan->desc.mode), plan->m.nOut,
@@ -1268,6 +1315,10 @@ int launch_all(cogconv_plan plan, const float* in0, const float* in1, float* out
                             LP_PACK_AND_RUN, ep);
         break;
     case KK_IGEMM_UP:
+        if (plan->ps.f.on && plan->desc.precision != COGCONV_FP32) {
+            e = launch_up2q_lp(plan->desc.precision, plan->g, plan->m, plan->ps.f, in0, in1, out, workspace, s, ep);
+            break;
+        }
         if (plan->ps.f.on) {
             e = launch_igemm_f32_up2q(plan->g, plan->m, plan->ps.f, in0, in1, out, s, ep);
             break;

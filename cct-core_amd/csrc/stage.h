@@ -19,9 +19,12 @@ __device__ __forceinline__ float buffer_load_f32(__amdgpu_buffer_rsrc_t r, int v
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vbytes, sbytes, 0));
 }
 
-// Cache policy of the loads that stage input fields: non-temporal (nt), so
-// that the streamed field does not push out of the L2 what every block re-reads
-// (the filter bank) -- c4 BF16 7x7 92 -> 83 us.
+// Cache policy of the loads that stage input fields in the BF16 / FP16 frame
+// kernels and the FP32 FilterAdjoint: non-temporal (nt), so that the streamed
+// field does not push out of the L2 what every block re-reads (the packed
+// filter bank: c4 BF16 7x7 92 -> 83 us).  The FP32 frame kernels (MFMA-bound)
+// keep cached loads: with nt their halo re-reads go to HBM (c4 9x9: 204 ->
+// 326 MB a launch) for no gain in time.
 constexpr int LOAD_NT = 2;
 
 __device__ __forceinline__ float buffer_load_f32_nt(__amdgpu_buffer_rsrc_t r, int vbytes, int sbytes) {
@@ -40,8 +43,8 @@ __device__ __forceinline__ void stage_rows(__amdgpu_buffer_rsrc_t src, const int
     float v0[IU], v1[IU];
 #pragma unroll
     for (int u = 0; u < IU; u++) {
-        v0[u] = buffer_load_f32_nt(src, 4 * col0, 4 * off[u]);
-        if (TWO) v1[u] = buffer_load_f32_nt(src, 4 * col1, 4 * off[u]);
+        v0[u] = buffer_load_f32(src, 4 * col0, 4 * off[u]);
+        if (TWO) v1[u] = buffer_load_f32(src, 4 * col1, 4 * off[u]);
     }
     __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll
@@ -60,6 +63,21 @@ __device__ __forceinline__ void stage_rows(__amdgpu_buffer_rsrc_t src, const int
 // (each under 2^31 bytes), so the buffer unit's range check returns 0.
 constexpr int STAGE_OOB = (int)0x80000000u;
 
+// Byte offset of stored column x of a staged tile with the border applied, or
+// STAGE_OOB where the column is off the field (BorderZero and the like).
+__device__ __forceinline__ int stage_col_offset(const Geom& g, int x) {
+    bool in = (unsigned)x < (unsigned)g.W;
+    if (g.border == COGCONV_BORDER_CYCLIC) {
+        x %= g.W;
+        x += x < 0 ? g.W : 0;
+        in = true;
+    } else {
+        in = in || g.border == COGCONV_BORDER_CLAMP;
+        x = min(max(x, 0), g.W - 1);
+    }
+    return in ? 4 * x : STAGE_OOB;
+}
+
 // Row staging with the row data computed on the fly: row(u) gives the row's
 // wave-uniform byte offset and whether it is in the field; dst(u) its LDS
 // element offset (< 0: not live, store to `junk`).  vcol0 / vcol1 are the
@@ -76,8 +94,8 @@ __device__ __forceinline__ void stage_rows_fn(__amdgpu_buffer_rsrc_t src, RowFn 
         int soff;
         bool on;
         row(u, soff, on);
-        v0[u] = buffer_load_f32_nt(src, on ? vcol0 : STAGE_OOB, soff);
-        if (TWO) v1[u] = buffer_load_f32_nt(src, on ? vcol1 : STAGE_OOB, soff);
+        v0[u] = buffer_load_f32(src, on ? vcol0 : STAGE_OOB, soff);
+        if (TWO) v1[u] = buffer_load_f32(src, on ? vcol1 : STAGE_OOB, soff);
     }
     __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first store
 #pragma unroll

@@ -6,6 +6,8 @@ Tolerances, normwise (max |gpu - oracle_f64| / max |oracle_f64|):
 on uniform[-1,1) inputs and on wide-range inputs |v| = 10^U(-3,3) with random
 signs.  All borders, both block-reduce frame modes, 1/3/5/7 filters.
 """
+import zlib
+
 import numpy as np
 import pytest
 
@@ -332,3 +334,60 @@ def test_filter_adjoint_rows_every_candidate(precision, xs, K, OD):
         else:
             by_split[key] = (desc, got)
     assert seen >= 2, "row-streaming kernel not offered"
+
+
+# (mode, border, F, N, K, H, W): widths 44 and 74 output columns (last 8-column
+# group past the row end, 4-byte-aligned rows), 40 output planes (two blocks),
+# 36 planes (a partial 32-plane chunk), BorderFull / Valid halos.
+UP2_LP_CASES = [("pf_brs", O.ZERO, 7, 36, 20, 13, 22), ("bpf_brs", O.FULL, 5, 16, 40, 9, 17),
+                ("pf_brs", O.CYCLIC, 9, 8, 16, 11, 37), ("pf_brs", O.VALID, 3, 5, 3, 12, 9),
+                ("bpf_brs", O.ZERO, 3, 40, 33, 6, 70)]
+_UP2_MODES = {"pf_brs": O.PF_BRS, "bpf_brs": O.BPF_BRS}
+
+
+@pytest.mark.parametrize("precision", (L.BF16, L.FP16))
+@pytest.mark.parametrize("mode,border,F,N,K,H,W", UP2_LP_CASES)
+def test_upsampled_lp_every_candidate(precision, mode, border, F, N, K, H, W):
+    """UpsampleInputConvolution(2) in BF16 / FP16: the fused four-phase kernel at
+    each output block, against the FP64 oracle on the inputs rounded to the MFMA
+    type (normwise <= 2e-6: only FP32 summation differs), against the oracle on
+    the FP32 inputs (the precision bound), bit for bit between output blocks,
+    and with an epilogue equal to the same ops applied afterwards."""
+    rng = np.random.default_rng(zlib.crc32(repr(("uplp", precision, mode, border, F, N, K)).encode()))
+    B = 2
+    a = inputs(rng, (B * N, H, W), False)
+    k = inputs(rng, (K * N, F, F), False)
+    orient = O.CONV if mode == "bpf_brs" else O.XCORR
+    od, cd = desc_pair(_UP2_MODES[mode], border, orient, O.UP, 2, B, a.shape, k.shape, precision=precision)
+    ref = O.conv(od, a, k, threads=16)
+    ref_r = O.conv(od, rounded(a, precision), rounded(k, precision), threads=16)
+    first = None
+    seen = 0
+    for v in range(L.variant_count(cd)):
+        plan = L.Plan(cd, v)
+        desc = plan.description
+        plan.close()
+        print("CASE up2 lp", NAMES[precision], mode, border, F, N, K, H, W, v, desc, flush=True)
+        assert "_up2 " in desc and "fused" in desc, desc
+        seen += 1
+        got = run_gpu(cd, a, k, variant=v)
+        assert got.shape == ref.shape
+        assert normwise(got, ref_r) <= 2e-6, (desc, normwise(got, ref_r))
+        assert normwise(got, ref) <= TOL[precision], desc
+        if first is None:
+            first = got
+            plan = L.Plan(cd, v)
+            plan.set_epilogue(L.make_epilogue([(L.EPI_MUL_CONST, 3.0, 0), (L.EPI_MAX_CONST, 0.5, 0)]))
+            ba, bk = L.DeviceBuffer.from_array(a), L.DeviceBuffer.from_array(k)
+            bo = L.DeviceBuffer(4 * int(np.prod(plan.out_shape)))
+            ws = L.DeviceBuffer(plan.workspace_bytes)
+            plan.launch(ba, bk, bo, ws)
+            L.synchronize()
+            fused = bo.download(plan.out_shape)
+            assert np.array_equal(fused, np.maximum(got * np.float32(3.0), np.float32(0.5)))
+            for buf in (ba, bk, bo, ws):
+                buf.close()
+            plan.close()
+        else:
+            assert np.array_equal(got, first), desc
+    assert seen >= 1
