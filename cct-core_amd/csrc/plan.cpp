@@ -1017,9 +1017,9 @@ int make_plan(const cogconv_desc* desc, int vi, cogconv_plan* plan) {
     if (vars.empty())
         return fail(COGCONV_ERR_UNSUPPORTED,
                     "precision %d is offered for ProjectFrame/BackProjectFrame with block reduction on 2-D fields with "
-                    "a square 1/3/5/7/9 filter, no input up-sampling and a 32-column tile that fits one wavefront, and "
-                    "for FilterAdjoint with block reduction, BorderValid and at most 64 planes per image in dy; "
-                    "not for this %s",
+                    "a square 1/3/5/7/9 filter and a 32-column tile that fits one wavefront, without input "
+                    "up-sampling or (BF16 / FP16, odd filters from 3) with 2x up-sampling, and for FilterAdjoint "
+                    "with block reduction, BorderValid and at most 64 planes per image in dy; not for this %s",
                     desc->precision, mode_name(desc->mode));
     const long outPts = (long)g.OH * g.OW;
     int pick = -1;
