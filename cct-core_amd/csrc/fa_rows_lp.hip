@@ -43,32 +43,23 @@ constexpr int FR_PD = 4;                   // steps of row loads in flight (regi
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// Eight consecutive floats at lane byte offset `vo` + uniform offset `so`:
-// two 16-byte loads (vec 4, rows 16-byte aligned) or eight 4-byte loads (a
-// plan-time, block-uniform choice).  (No 8-byte form: this toolchain lowers
-// the 8-byte raw buffer load builtin to a single 4-byte load.)
-__device__ __forceinline__ void load8(__amdgpu_buffer_rsrc_t r, int vo, int so, int vec, float (&v)[8]) {
-    if (vec == 4) {
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, vo + 16 * h, so, 0));
-            v[4 * h] = q.x;
-            v[4 * h + 1] = q.y;
-            v[4 * h + 2] = q.z;
-            v[4 * h + 3] = q.w;
-        }
-    } else {
-#pragma unroll
-        for (int e = 0; e < 8; e++) v[e] = buffer_load_f32(r, vo + 4 * e, so);
-    }
+// Four consecutive floats at lane byte offset `vo` (4-byte aligned: the
+// 16-byte load needs no more here).  The range check is per dword, so a
+// piece that runs past the end of the buffer keeps the pixels before it.
+__device__ __forceinline__ void load4(__amdgpu_buffer_rsrc_t r, int vo, float (&v)[4]) {
+    const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, 0, 0));
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
 }
 
+// Four rounded elements, as the 8-byte half of an 8-element LDS slot.
 template <int PREC>
-__device__ __forceinline__ typename LpType<PREC>::V round8(const float (&v)[8]) {
-    typename LpType<PREC>::V h;
-#pragma unroll
-    for (int e = 0; e < 8; e++) h[e] = (typename LpType<PREC>::T)v[e];
-    return h;
+__device__ __forceinline__ void store4(typename LpType<PREC>::T* dst, const float (&v)[4]) {
+    typedef typename LpType<PREC>::T T;
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    *reinterpret_cast<T4*>(dst) = T4{(T)v[0], (T)v[1], (T)v[2], (T)v[3]};
 }
 
 // Elements j .. j + 7 of the 16-element run cur ++ nxt (16-bit elements).
@@ -107,6 +98,7 @@ __global__ __launch_bounds__(64 * NB * KH) void k_fa_rows_lp(Geom g, PlaneMap m,
     extern __shared__ __attribute__((aligned(16))) unsigned char fr_smem[];
     T* sD = reinterpret_cast<T*>(fr_smem);  // [2][4][KT][8]: dy rows, double-buffered
     T* sX = sD + 2 * 4 * KT * 8;            // [RB][FR_XS][NT][8]: x row ring
+    T* sJunk = sX + RB * FR_XS * NT * 8;    // [8]: where lanes without a piece store theirs
 
     const int tiles = c.nTiles * c.kTiles;
     const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
@@ -117,7 +109,7 @@ __global__ __launch_bounds__(64 * NB * KH) void k_fa_rows_lp(Geom g, PlaneMap m,
     const int K = c.K, N = m.N;
     const int dyH = g.FH, dyW = g.FW, H = g.H, W = g.W;
     const int x0 = ch * 32;
-    const int y0 = rs * c.rowsPer, y1 = min(dyH, y0 + c.rowsPer);
+    const int yBeg = rs * c.rowsPer, yEnd = min(dyH, yBeg + c.rowsPer);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -128,62 +120,76 @@ __global__ __launch_bounds__(64 * NB * KH) void k_fa_rows_lp(Geom g, PlaneMap m,
     const __amdgpu_buffer_rsrc_t dyr = buffer_rsrc(DY + (size_t)b * K * dyH * dyW, (uint32_t)K * dyH * dyW * 4u);
     const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + (size_t)b * N * H * W, (uint32_t)N * H * W * 4u);
 
-    // Staging units.  dy: (slot gs, plane k), plane fastest, 8 pixels x0 + 8 gs
-    // .. +7 of the row, 4 KT units; x: (slot gs, plane n), 8 pixels of the
-    // entering row, FR_XS NT units; unit u = tid + NTH d.  The whole offset is
-    // the lane offset, so the buffer's range check covers it (a load past the
-    // image reads 0); pixels past the row are zeroed after the load (the loads
-    // run on into the next row).
-    constexpr int DU = (4 * KT + NTH - 1) / NTH;
-    constexpr int XU = (FR_XS * NT + NTH - 1) / NTH;
-    bool dUnit[DU], xUnit[XU];
-    int dPix[DU], dVo[DU], dDst[DU], xPix[XU], xVo[XU], xDst[XU];
+    // Staging pieces: 4 pixels (16 bytes) a lane, piece fastest, so the lanes
+    // of one plane's row segment sit side by side and each cache line of the
+    // segment is requested by one load instruction (lines that several waves
+    // request while the first fill is pending stall the vector L1).  dy:
+    // piece h (pixels x0 + 4h .. +3) of plane k, 8 KT pieces; x: piece h of
+    // plane n of the entering row, 2 FR_XS NT pieces; piece u = tid + NTH d.
+    // The whole offset is the lane offset, so the buffer's range check covers
+    // it (a load past the image reads 0); pixels past the row are zeroed
+    // after the load (the loads run on into the next row).  A piece lands in
+    // half h % 2 of the 8-pixel LDS slot h / 2.
+    constexpr int DP = (8 * KT + NTH - 1) / NTH;
+    constexpr int XP = (2 * FR_XS * NT + NTH - 1) / NTH;
+    bool dUnit[DP], xUnit[XP];
+    int dPix[DP], dVo[DP], dDst[DP], xPix[XP], xVo[XP], xDst[XP];
 #pragma unroll
-    for (int d = 0; d < DU; d++) {
+    for (int d = 0; d < DP; d++) {
         const int u = tid + NTH * d;
-        const int dk = u % KT, dgs = u / KT;
+        const int h = u % 8, dk = u / 8;
         const int dPlane = kt * KT + dk;
-        dUnit[d] = u < 4 * KT;
-        dPix[d] = x0 + 8 * dgs;
+        dUnit[d] = u < 8 * KT;
+        dPix[d] = x0 + 4 * h;
         dVo[d] = dUnit[d] && dPlane < K ? 4 * (dPlane * dyH * dyW + dPix[d]) : FR_OOB;
-        dDst[d] = (dgs * KT + dk) * 8;  // + buffer * 4 * KT * 8
+        dDst[d] = ((h / 2) * KT + dk) * 8 + 4 * (h % 2);  // + buffer * 4 * KT * 8
     }
 #pragma unroll
-    for (int d = 0; d < XU; d++) {
+    for (int d = 0; d < XP; d++) {
         const int u = tid + NTH * d;
-        const int xn = u % NT, xgs = u / NT;
+        const int h = u % (2 * FR_XS), xn = u / (2 * FR_XS);
         const int xPlane = nt * NT + xn;
-        xUnit[d] = u < FR_XS * NT;
-        xPix[d] = x0 + 8 * xgs;
+        xUnit[d] = u < 2 * FR_XS * NT;
+        xPix[d] = x0 + 4 * h;
         xVo[d] = xUnit[d] && xPlane < N ? 4 * (xPlane * H * W + xPix[d]) : FR_OOB;
-        xDst[d] = (xgs * NT + xn) * 8;  // + ring row * FR_XS * NT * 8
+        xDst[d] = ((h / 2) * NT + xn) * 8 + 4 * (h % 2);  // + ring row * FR_XS * NT * 8
     }
 
-    auto load_dy = [&](int y, float (&v)[DU][8]) {
+    // The queue's loads stay in flight across steps only if nothing between
+    // a load and its store branches or reads the loaded registers (the
+    // compiler's wait counts are merged over every path into a point, and any
+    // such path drains the queue).  So: pixels past the row are masked at the
+    // store; a row past the split (on == false) is loaded at an offset past
+    // every buffer (it reads 0 without touching memory); every lane stores,
+    // lanes without a piece into sJunk.
+    auto row_off = [&](int y, int pitch, bool on) { return on ? 4u * (unsigned)(y * pitch) : (unsigned)FR_OOB; };
+    auto load_dy = [&](int y, bool on, float (&v)[DP][4]) {
+        const unsigned ro = row_off(y, dyW, on);
 #pragma unroll
-        for (int d = 0; d < DU; d++) {
-            load8(dyr, dVo[d] + 4 * y * dyW, 0, c.vecDy, v[d]);
+        for (int d = 0; d < DP; d++) load4(dyr, (int)((unsigned)dVo[d] + ro), v[d]);
+    };
+    auto store_dy = [&](int buf, const float (&v)[DP][4]) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) v[d][e] = dPix[d] + e < dyW ? v[d][e] : 0.f;
+        for (int d = 0; d < DP; d++) {
+            float m[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) m[e] = dPix[d] + e < dyW ? v[d][e] : 0.f;
+            store4<PREC>(dUnit[d] ? sD + buf * 4 * KT * 8 + dDst[d] : sJunk, m);
         }
     };
-    auto store_dy = [&](int buf, const float (&v)[DU][8]) {
+    auto load_x = [&](int y, bool on, float (&v)[XP][4]) {
+        const unsigned ro = row_off(y, W, on);
 #pragma unroll
-        for (int d = 0; d < DU; d++)
-            if (dUnit[d]) *reinterpret_cast<V*>(sD + buf * 4 * KT * 8 + dDst[d]) = round8<PREC>(v[d]);
+        for (int d = 0; d < XP; d++) load4(xr, (int)((unsigned)xVo[d] + ro), v[d]);
     };
-    auto load_x = [&](int y, float (&v)[XU][8]) {
+    auto store_x = [&](int row, const float (&v)[XP][4]) {
 #pragma unroll
-        for (int d = 0; d < XU; d++) {
-            load8(xr, xVo[d] + 4 * y * W, 0, c.vecX, v[d]);
+        for (int d = 0; d < XP; d++) {
+            float m[4];
 #pragma unroll
-            for (int e = 0; e < 8; e++) v[d][e] = xPix[d] + e < W ? v[d][e] : 0.f;
+            for (int e = 0; e < 4; e++) m[e] = xPix[d] + e < W ? v[d][e] : 0.f;
+            store4<PREC>(xUnit[d] ? sX + row * FR_XS * NT * 8 + xDst[d] : sJunk, m);
         }
-    };
-    auto store_x = [&](int row, const float (&v)[XU][8]) {
-#pragma unroll
-        for (int d = 0; d < XU; d++)
-            if (xUnit[d]) *reinterpret_cast<V*>(sX + row * FR_XS * NT * 8 + xDst[d]) = round8<PREC>(v[d]);
     };
 
     floatx4 acc[KBW][OD][OD];
@@ -194,29 +200,29 @@ __global__ __launch_bounds__(64 * NB * KH) void k_fa_rows_lp(Geom g, PlaneMap m,
 #pragma unroll
             for (int j = 0; j < OD; j++) acc[kb][i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
+    // Rows y0 .. y1 - 1, streamed down with a fresh prologue.
+    auto stream = [&](const int y0, const int y1) {
     if (y0 < y1) {  // block-uniform
         // Prologue: x rows y0 .. y0 + OD - 1 into ring rows (y0 + i) % RB, dy row y0 into buffer 0.
         for (int i = 0; i < OD; i++) {
-            float v[XU][8];
-            load_x(y0 + i, v);
+            float v[XP][4];
+            load_x(y0 + i, true, v);
             store_x((y0 + i) % RB, v);
         }
         {
-            float v[DU][8];
-            load_dy(y0, v);
+            float v[DP][4];
+            load_dy(y0, true, v);
             store_dy(0, v);
         }
         // Register queue of FR_PD stages: the rows stored at the end of step t
         // (dy row y + 1, x row y + OD) were loaded FR_PD - 1 steps earlier into
         // stage (t + 1) % FR_PD, so that many steps of MFMAs cover their memory
         // latency.  Steps run in groups of FR_PD (stage indices are constants).
-        float qd[FR_PD][DU][8], qx[FR_PD][XU][8];
+        float qd[FR_PD][DP][4], qx[FR_PD][XP][4];
 #pragma unroll
         for (int k = 1; k < FR_PD; k++) {
-            if (y0 + k < y1) {  // block-uniform
-                load_dy(y0 + k, qd[k]);
-                load_x(y0 + k + OD - 1, qx[k]);
-            }
+            load_dy(y0 + k, y0 + k < y1, qd[k]);
+            load_x(y0 + k + OD - 1, y0 + k < y1, qx[k]);
         }
         __syncthreads();
         int ring0 = y0 % RB;  // ring row of x row y
@@ -228,10 +234,8 @@ __global__ __launch_bounds__(64 * NB * KH) void k_fa_rows_lp(Geom g, PlaneMap m,
                 const int t = y - y0;
                 // Stage u (last stored at the end of the previous step) takes the
                 // rows for the end of step t + FR_PD - 1.
-                if (y + FR_PD < y1) {
-                    load_dy(y + FR_PD, qd[u]);
-                    load_x(y + FR_PD + OD - 1, qx[u]);
-                }
+                load_dy(y + FR_PD, y + FR_PD < y1, qd[u]);
+                load_x(y + FR_PD + OD - 1, y + FR_PD < y1, qx[u]);
                 const T* dBuf = sD + (t & 1) * 4 * KT * 8;
                 V a[KBW];
 #pragma unroll
@@ -260,10 +264,11 @@ __global__ __launch_bounds__(64 * NB * KH) void k_fa_rows_lp(Geom g, PlaneMap m,
                         tap(std::integral_constant<int, 4>{});
                     }
                 }
-                if (y + 1 < y1) {
+                {
                     // dy buffer (t + 1) & 1 and the ring row of x row y - 1 were
                     // last read by the previous step, which every wave finished
-                    // before that step's barrier.
+                    // before that step's barrier.  (After the last row these
+                    // stores write rows that no step reads.)
                     const int nx = (u + 1) % FR_PD;  // a constant once the group loop is unrolled
                     store_dy((t + 1) & 1, qd[nx]);
                     int rw = ring0 + OD;
@@ -275,6 +280,17 @@ __global__ __launch_bounds__(64 * NB * KH) void k_fa_rows_lp(Geom g, PlaneMap m,
             }
         }
     }
+    };
+    // The blocks of one image start their walk at a row of their own, 11 b
+    // modulo the run, and wrap: blocks that run together then read rows that
+    // lie apart in memory (plane strides are often multiples of a large power
+    // of two, so rows at one position of every plane share their low address
+    // bits).  The last step of a walk ends with a barrier, so the second
+    // walk's prologue may overwrite the LDS rows at once.
+    const int run = yEnd - yBeg;
+    const int off = run > 0 ? (b * 11) % run : 0;
+    stream(yBeg + off, yEnd);
+    if (off) stream(yBeg, yBeg + off);
     // The split's partial: ws[s][k][n][i][j] (every (k, n, i, j) of the tile,
     // zeros for a block without rows).
     const size_t total = (size_t)K * N * P;
@@ -297,7 +313,7 @@ __global__ __launch_bounds__(64 * NB * KH) void k_fa_rows_lp(Geom g, PlaneMap m,
 
 size_t fa_rows_lp_lds_bytes(const FaCfg& c, int OD) {
     const int KT = c.kh * c.kbw * 16, NT = c.nb * 16;
-    return (size_t)2 * 4 * KT * 16 + (size_t)(OD + 1) * FR_XS * NT * 16;
+    return (size_t)2 * 4 * KT * 16 + (size_t)(OD + 1) * FR_XS * NT * 16 + 16;  // + the junk slot
 }
 
 // Tiles offered (x-plane blocks NB, dy-plane block groups KH, dy-plane blocks a wave KBW).

@@ -202,8 +202,8 @@ struct FaCfg {
     // Row-streaming BF16 / FP16 kernel (k_fa_rows_lp, rows = 1): nb x kh waves,
     // kbw dy-plane blocks a wave; tiles of kh*kbw*16 dy planes by nb*16 x
     // planes; 32-pixel column chunks of the dy rows; row splits of rowsPer
-    // rows; load widths (floats) of the dy and x rows.
-    int rows, nb, kh, kbw, nTiles, kTiles, chunks, rowSplits, rowsPer, vecDy, vecX;
+    // rows.
+    int rows, nb, kh, kbw, nTiles, kTiles, chunks, rowSplits, rowsPer;
     size_t lds_bytes;
 };
 

@@ -496,10 +496,6 @@ bool choose_fa_rows_lp(const cogconv_desc* d, const Geom& g, const PlaneMap& m, 
     c.rowSplits = rowSplits;
     c.rowsPer = (g.FH + rowSplits - 1) / rowSplits;
     c.splits = m.B * c.chunks * rowSplits;
-    // Load widths: 16-byte loads where rows start 16-byte aligned, else
-    // 4-byte (the launch checks the buffers' own alignment).
-    c.vecDy = g.FW % 4 == 0 ? 4 : 1;
-    c.vecX = g.W % 4 == 0 ? 4 : 1;
     c.lds_bytes = fa_rows_lp_lds_bytes(c, g.OH);
     *out = c;
     return true;
@@ -1219,10 +1215,9 @@ void describe_into(const cogconv_plan_s* plan, char* buf, size_t len) {
         static const char* pn[] = {"fp32", "bf16", "fp16", "split_bf16"};
         if (plan->fcfg.rows) {
             const FaCfg& f = plan->fcfg;
-            snprintf(buf, len, "fa_rows_%s %s out=%dx%dx%d tile=%dx%d waves=%d chunks=%d rowsplits=%d splits=%d ld=%d/%d lds=%zu",
+            snprintf(buf, len, "fa_rows_%s %s out=%dx%dx%d tile=%dx%d waves=%d chunks=%d rowsplits=%d splits=%d lds=%zu",
                      pn[plan->desc.precision], mode_name(plan->desc.mode), plan->m.nOut, g.OH, g.OW,
-                     f.kh * f.kbw * 16, f.nb * 16, f.nb * f.kh, f.chunks, f.rowSplits, f.splits, f.vecDy, f.vecX,
-                     f.lds_bytes);
+                     f.kh * f.kbw * 16, f.nb * 16, f.nb * f.kh, f.chunks, f.rowSplits, f.splits, f.lds_bytes);
             break;
         }
         snprintf(buf, len, "fa_igemm_%s %s out=%dx%dx%d MB=%d CBW=%d RR=%d CW=%d groups=%d splits=%d waves=%d lds=%zu",
@@ -1328,12 +1323,7 @@ int launch_all(cogconv_plan plan, const float* in0, const float* in1, float* out
         break;
     case KK_FA_IGEMM_LP:
         if (plan->fcfg.rows) {
-            // The row loads also need the buffers aligned to their width.
-            FaCfg f = plan->fcfg;
-            const uintptr_t ax = reinterpret_cast<uintptr_t>(in0), ad = reinterpret_cast<uintptr_t>(in1);
-            if (ax % (4 * f.vecX)) f.vecX = 1;
-            if (ad % (4 * f.vecDy)) f.vecDy = 1;
-            e = launch_fa_rows_lp(plan->desc.precision, plan->g, plan->m, f, in0, in1, out,
+            e = launch_fa_rows_lp(plan->desc.precision, plan->g, plan->m, plan->fcfg, in0, in1, out,
                                   static_cast<float*>(workspace), s, ep);
         } else {
             e = launch_fa_igemm_lp(plan->desc.precision, plan->g, plan->m, plan->fcfg, in0, in1, out,
