@@ -174,6 +174,30 @@ def flushed_median_ms(L, op, stream, flush, reps=20, warm=2):
     return statistics.median(times), min(times), max(times)
 
 
+def read_flushed_median_ms(L, op, stream, reps=20):
+    """Median over `reps` launches of `op`, each timed alone after a read of
+    FLUSH_BYTES (a torch sum over a resident buffer): the operands are gone
+    from the L2 and the last-level cache as after a write flush, but the
+    evicted lines are clean, so the op's own loads do not also pay for writing
+    the flush's dirty lines back to HBM."""
+    import torch
+    buf = torch.ones(FLUSH_BYTES // 4, dtype=torch.float32, device="cuda")
+    e0, e1 = L.Event(), L.Event()
+    times = []
+    for _ in range(reps):
+        buf.sum()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        op.launch(stream)
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_ms(e1))
+    e0.close()
+    e1.close()
+    del buf
+    return statistics.median(times)
+
+
 def timed_steps(L, ops, stream, steps, warmup):
     """`warmup` untimed steps, then exactly `steps` steps (every op once, back
     to back on `stream`) bracketed by device synchronisations; returns ms per
@@ -365,6 +389,7 @@ def precision_workload(args):
     --op-reps launches, each after a 512 MiB cache-flush write, HIP events;
     errors against the FP64 oracle on a 2-image slice (uniform[-1,1) and
     wide-range 10^U(-3,3) with random signs)."""
+    import torch  # noqa: F401  (for the read flush; imported before the library loads the HIP runtime)
     import cogconv._lib as L
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -395,13 +420,15 @@ def precision_workload(args):
                               args, flush_bytes=FLUSH_BYTES)
             op = Op(L, opname, plan, ab, b1)
             ms, lo, hi = flushed_median_ms(L, op, stream, flush, reps=args.op_reps)
+            rms = read_flushed_median_ms(L, op, stream, reps=args.op_reps)
             warm = timed_steps(L, [op], stream, 20, 3)  # back to back, operands left in the caches
             res.setdefault(names[prec], {})[opname] = {
                 "ms": round(ms, 4), "min_ms": round(lo, 4), "max_ms": round(hi, 4),
                 "gflops": round(plan.flops / (ms * 1e-3) / 1e9, 1),
                 "warm_ms": round(warm, 4), "warm_gflops": round(plan.flops / (warm * 1e-3) / 1e9, 1),
-                "kernel": plan.description}
-            log(names[prec], opname, plan.description, "%.4f ms flushed, %.4f ms warm" % (ms, warm))
+                "read_flushed_ms": round(rms, 4), "kernel": plan.description}
+            log(names[prec], opname, plan.description, "%.4f ms flushed, %.4f ms read-flushed, %.4f ms warm" %
+                (ms, rms, warm))
             op.close()
             plan.close()
     # Errors on a 2-image slice, both input sets.
@@ -446,8 +473,10 @@ def precision_workload(args):
                                                       res["fp32"]["projectFrame+BRS"]["warm_gflops"], 2),
             "warm_bf16_over_fp32_filterAdjoint": round(res["bf16"]["filterAdjoint+BRS"]["warm_gflops"] /
                                                        res["fp32"]["filterAdjoint+BRS"]["warm_gflops"], 2),
-            "timing": "ms: median of --op-reps launches, each after a 512 MiB cache-flush write; warm_ms: mean "
-                      "of 20 back-to-back launches after 3 (operands resident in the last-level cache)",
+            "timing": "ms: median of --op-reps launches, each after a 512 MiB cache-flush write; read_flushed_ms: "
+                      "the same after a 512 MiB read instead (caches emptied of the operands but left clean, so "
+                      "the op does not pay for the flush's write-back); warm_ms: mean of 20 back-to-back launches "
+                      "after 3 (operands resident in the last-level cache)",
             "peaks": {"fp32_tflops": PEAK_FP32_TFLOPS, "bf16_tflops": PEAK_BF16_TFLOPS, "hbm_gbs": PEAK_HBM_GBS}}
     print(json.dumps(line), flush=True)
 
