@@ -290,9 +290,10 @@ def rounded(a, precision):
 
 ROWS_CASES = [
     # (B, N, H, W), K: dW side H - dh + 1 with dh = H - OD + 1
-    ((2, 64, 34, 36), 64, 3),   # c3 planes; dy rows 34 wide: a full and a 2-pixel chunk
-    ((3, 20, 23, 70), 40, 3),   # partial x- and dy-plane tiles; x rows 70 wide (4-byte loads), dy rows 68 (16-byte)
-    ((2, 16, 19, 37), 24, 1),   # 1x1 dW; odd widths (4-byte loads)
+    ((2, 64, 34, 36), 64, 3),   # c3 planes; dy rows 34 wide: a full and a 2-pixel chunk, the last
+                                # 16-byte piece of the last row running past the image's buffer
+    ((3, 20, 23, 70), 40, 3),   # partial x- and dy-plane tiles; x rows 70 wide (rows not 16-byte aligned), dy rows 68
+    ((2, 16, 19, 37), 24, 1),   # 1x1 dW; odd widths
     ((2, 12, 21, 45), 20, 5),   # 5x5 dW; dy rows 41 wide
 ]
 
